@@ -1,0 +1,219 @@
+/* ramp.h - C-ABI of libramp.so, the MI355X implementation of RAMP's
+ * per-neutron Monte Carlo ray-tracing path.
+ *
+ * The reference drives its OpenCL kernels from Python (PyOpenCL):
+ *   Instrument.execute(N)            mcramp/mcramp.py:403-455
+ *   source.gen_prg(queue, N, ...)    mcramp/moderator/flat_mod.py:24-32
+ *   geom.intersect_prg(...)          mcramp/geom/plane.py:48-58
+ *   scat.scatter_prg(...)            mcramp/scat/chopper.py:58-70
+ *   scat.data(queue)                 mcramp/scat/detector1D.py:114-116
+ * Those per-kernel launches over AoS float16/float8 buffers are replaced
+ * here by one plan (the instrument flattened into POD descriptors) and a
+ * run call that executes fused, stream-compacted HIP segment kernels over
+ * structure-of-arrays FP32 state.  All functions return 0 on success and a
+ * negative RAMP_E_* code on failure; ramp_last_error() gives the message of
+ * the last failure on the calling thread.  No function throws or exits.
+ *
+ * Memory: device memory is owned by the context/plan, host buffers by the
+ * caller.  A context is bound to one HIP device and one stream and is not
+ * thread-safe; separate contexts are independent.
+ */
+#ifndef RAMP_H
+#define RAMP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes -------------------------------------------------------- */
+#define RAMP_OK 0
+#define RAMP_E_INVALID (-1)   /* bad argument / descriptor */
+#define RAMP_E_DEVICE (-2)    /* HIP runtime failure */
+#define RAMP_E_NOMEM (-3)     /* allocation failure */
+#define RAMP_E_RANGE (-4)     /* size beyond a documented limit */
+#define RAMP_E_STATE (-5)     /* call not valid in the current state */
+
+/* ---- component type codes ---------------------------------------------- */
+/* sources (mcramp/moderator) */
+#define RAMP_SRC_FLAT 1      /* MFlat     flat_mod.cl:4-51 */
+#define RAMP_SRC_CONT 2      /* MCont     cont_mod.cl:11-73 (+ optional 1/E tail) */
+#define RAMP_SRC_ISIS 3      /* MISIS     isis_mod.cl:15-105 */
+#define RAMP_SRC_LAMBDA 4    /* MLambda   wavelength-uniform, weight ~ 1/lambda (no reference counterpart) */
+#define RAMP_SRC_PARALLEL 5  /* MParallel parallel beam (no reference counterpart) */
+
+/* geometry (mcramp/geom) */
+#define RAMP_G_ARM 1         /* GArm         geom/arm.cl:3-23 */
+#define RAMP_G_PLANE 2       /* GPlane       geom/plane.cl:3-53 */
+#define RAMP_G_BOX 3         /* GBox         geom/box.cl:3-54 */
+#define RAMP_G_SPHERE 4      /* GSphere      geom/sphere.cl:3-44 */
+#define RAMP_G_SPHERE_INT 5  /* GSphereInt   geom/sphere_int.cl:3-43 */
+#define RAMP_G_CYLINDER 6    /* GCylinderExt geom/cylinder_ext.cl:3-53 */
+#define RAMP_G_BANANA 7      /* GBanana      geom/banana.cl:3-55 */
+#define RAMP_G_MESH 8        /* GMesh        geom/mesh.cl:3-130 (BVH on the GPU) */
+
+/* scattering (mcramp/scat) */
+#define RAMP_S_ARM 1         /* SArm           scat/arm.cl:3-30 */
+#define RAMP_S_BEAMSTOP 2    /* SBeamstop      scat/beamstop.cl:3-32 */
+#define RAMP_S_CHOPPER 3     /* SChopper       scat/chopper.cl:4-59 */
+#define RAMP_S_GUIDE 4       /* SGuide         scat/guide.cl:9-150 */
+#define RAMP_S_ISOTROPIC 5   /* SIsotropic     scat/isotropic.cl:75-323 */
+#define RAMP_S_POWDER1 6     /* SPowder1       scat/powder1.cl:5-101 */
+#define RAMP_S_POWDERN 7     /* SPowderN       scat/powderN.cl:5-157 */
+#define RAMP_S_MONO 8        /* SMonochromator scat/monochromator.cl:22-197 */
+#define RAMP_S_DET1D 9       /* SDetector1D    scat/detector1D.cl:20-77 */
+#define RAMP_S_DET2D 10      /* SDetector2D    scat/detector2D.cl:31-186 */
+#define RAMP_S_COUNTER 11    /* SCounter       scat/counter.cl:18-49 */
+#define RAMP_S_INCOHERENT 12 /* SIncoherent    constant-Sigma incoherent sample (no reference counterpart) */
+#define RAMP_S_ABSORBER 13   /* SAbsorber      Beer-Lambert absorber (no reference counterpart) */
+#define RAMP_S_DETPIX 14     /* SDetectorTubes tube x pixel x TOF bank (no reference counterpart) */
+
+/* ---- plan descriptors --------------------------------------------------- */
+#define RAMP_NP 48  /* float parameters per component part */
+#define RAMP_NI 8   /* integer parameters per component part */
+#define RAMP_NT 4   /* table references per component part */
+
+/* Parameters are stored as float32 exactly as the reference wrappers convert
+ * them (np.float32 / np.uint32); the per-type slot meaning is listed in
+ * ramp_amd/components.py next to each class. */
+typedef struct {
+    uint32_t type;
+    uint32_t ni[RAMP_NI];
+    float p[RAMP_NP];
+    uint64_t tab_off[RAMP_NT]; /* offsets (in floats) into ramp_plan_desc.tables */
+    uint64_t tab_len[RAMP_NT];
+} ramp_part_desc;
+
+typedef struct {
+    ramp_part_desc geom;
+    ramp_part_desc scat;
+    uint32_t comp_idx;   /* reference component index (mcramp.py:275,112) */
+    uint32_t restore;    /* component-level restore_neutron (terminator.cl:17) */
+    int32_t det;         /* histogram slot written by the scatter part, or -1 */
+    uint32_t pad;
+    float pos[3];        /* component position (after 'relative' resolution) */
+    float rot[3];        /* ZYX Euler angles in radians */
+} ramp_component_desc;
+
+typedef struct {
+    uint32_t linear;     /* 1: linear block, 0: non-linear ('multi') block */
+    uint32_t multi;      /* sweeps of a non-linear block */
+    uint32_t first;      /* first component of the block */
+    uint32_t count;      /* number of components */
+} ramp_block_desc;
+
+typedef struct {
+    uint64_t n_bins;     /* bins per moment (w and w^2 are stored separately) */
+} ramp_det_desc;
+
+typedef struct {
+    ramp_part_desc source;
+    uint32_t source_block;   /* block holding the source (execution starts there) */
+    uint32_t n_blocks;
+    uint32_t n_comps;
+    uint32_t n_dets;
+    const ramp_block_desc *blocks;
+    const ramp_component_desc *comps;
+    const ramp_det_desc *dets;
+    const float *tables;
+    uint64_t n_tables;
+} ramp_plan_desc;
+
+/* ---- run options -------------------------------------------------------- */
+#define RAMP_HIST_ATOMIC 0        /* float atomics, order-dependent sums */
+#define RAMP_HIST_DETERMINISTIC 1 /* int64 fixed-point sums, order-independent */
+
+typedef struct {
+    uint32_t hist_mode;      /* RAMP_HIST_* */
+    uint32_t keep_state;     /* 1: keep the final per-neutron state of the last chunk */
+    uint32_t wg;             /* workgroup size (0: default) */
+    uint32_t npt;            /* neutrons per thread in the source segment (0: default) */
+    uint32_t max_segments;   /* compaction boundaries allowed (0: automatic) */
+    uint32_t pad[3];
+    double weight_bound;     /* deterministic mode: bound on one event's weight (0: derive) */
+} ramp_run_opts;
+
+typedef struct {
+    uint64_t n_hist;         /* histories run */
+    uint64_t n_chunks;
+    uint64_t n_events;       /* detector events accumulated */
+    uint64_t n_oob;          /* detector events dropped as out of range (reference writes them out of bounds) */
+    uint64_t n_segments;     /* segment launches */
+    uint64_t state_in;       /* neutrons read from compacted queues */
+    uint64_t state_out;      /* neutrons written to compacted queues */
+    uint64_t n_det_overflow; /* deterministic mode: events beyond the fixed-point range */
+    double ms_total;         /* device time of the last run (HIP events) */
+    double ms_source_segment;/* device time of the first (source) segment of the last run */
+} ramp_run_stats;
+
+typedef struct ramp_ctx ramp_ctx;
+typedef struct ramp_plan ramp_plan;
+
+/* Context bound to HIP device `device`.  Replaces the PyOpenCL context and
+ * queue passed to Instrument(fn, ctx, queue) (mcramp.py:271). */
+int ramp_ctx_create(int device, ramp_ctx **out);
+int ramp_ctx_destroy(ramp_ctx *ctx);
+
+/* Upload an instrument plan.  Replaces the per-component program builds and
+ * parameter/histogram buffer uploads done by the wrapper constructors
+ * (mcramp.py:46-116, e.g. detector1D.py:58-70). */
+int ramp_plan_create(ramp_ctx *ctx, const ramp_plan_desc *desc, ramp_plan **out);
+int ramp_plan_destroy(ramp_plan *plan);
+
+/* Run n_hist histories in chunks of `chunk` (0: one chunk).  seeds[c] is the
+ * source seed of chunk c (the reference draws randint(0,4096) per chunk,
+ * flat_mod.py:32); n_seeds must cover ceil(n_hist/chunk) chunks.
+ * Replaces Instrument.execute (mcramp.py:403-455).  Histograms accumulate
+ * across runs, as the reference's do (detector1D.py:59-70). */
+int ramp_run(ramp_plan *plan, uint64_t n_hist, uint64_t chunk, const uint32_t *seeds,
+             uint32_t n_seeds, const ramp_run_opts *opts);
+
+/* Read detector slot `det` (w and w^2 sums, n values each).  Replaces the
+ * wrappers' _cached_copy readback (detector1D.py:130-138). */
+int ramp_hist_read(ramp_plan *plan, uint32_t det, double *w, double *w2, uint64_t n);
+
+/* Zero every histogram of the plan (the reference has no counterpart). */
+int ramp_hist_reset(ramp_plan *plan);
+
+/* Final per-neutron state of the last chunk of the last run with
+ * keep_state=1, in the reference's AoS layouts: neutrons as 16 floats
+ * (consts.h:91-121), intersections as 8 floats, iidx as uint32.  Replaces
+ * the trace/debug copies of mcramp.py:141-165. */
+int ramp_state_read(ramp_plan *plan, float *neutrons16, float *isect8, uint32_t *iidx, uint64_t n);
+
+int ramp_stats_read(const ramp_plan *plan, ramp_run_stats *out);
+
+/* Evaluate the shared FP32 math library (fn = RAMP_FN_*) on n inputs, on the
+ * host (on_device = 0) or on the GPU of ctx (on_device = 1).  Test hook for
+ * the bit-identity of host and device arithmetic; no reference counterpart. */
+#define RAMP_FN_SIN 1
+#define RAMP_FN_COS 2
+#define RAMP_FN_EXP 3
+#define RAMP_FN_LOG 4
+#define RAMP_FN_POW 5
+#define RAMP_FN_TANH 6
+#define RAMP_FN_ATAN2 7
+#define RAMP_FN_ASIN 8
+#define RAMP_FN_ACOS 9
+#define RAMP_FN_FMOD 10
+#define RAMP_FN_ROUND 11
+#define RAMP_FN_SQRT 12
+#define RAMP_FN_DIV 13
+int ramp_math_eval(ramp_ctx *ctx, int fn, const float *x, const float *y, float *out, uint64_t n,
+                   int on_device);
+
+/* Draws of the per-neutron stream (mcramp/include/rand.h:4-20): out[i] is
+ * the u01 value of thread id tid[i] at counter ctr[i].  Host or device. */
+int ramp_rng_eval(ramp_ctx *ctx, const uint32_t *tid, const float *ctr, float *out, uint64_t n,
+                  int on_device);
+
+const char *ramp_last_error(void);
+const char *ramp_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

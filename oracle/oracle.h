@@ -1,0 +1,43 @@
+/* oracle.h - CPU oracle of the RAMP neutron ray-tracing path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the
+ * reference's OpenCL kernels and host loop (the .cl files under mcramp/ and mcramp/mcramp.py),
+ * one neutron at a time in the reference's AoS layout, used by the tests,
+ * by __graft_entry__.smoke() and by bench.py's CPU-baseline leg as the
+ * checker of the HIP path.  The product (ramp_amd, libramp.so) never links,
+ * imports or calls it.
+ *
+ * Arithmetic follows the reference: float where the kernels use float,
+ * double where the kernels promote to double (M_PI and unsuffixed literals
+ * are double on fp64-capable OpenCL devices).  OpenCL builtins are the
+ * project math library (ramp_amd/csrc/rmath.h, rvec.h), the RNG is the
+ * Threefry stream of ramp_amd/csrc/rrng.h (pinned against the reference's
+ * own Random123 header by tests/test_rng.py).
+ */
+#ifndef RAMP_ORACLE_H
+#define RAMP_ORACLE_H
+
+#include <stdint.h>
+#include "../include/ramp.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Run n_hist histories of `desc` in chunks (chunk = 0: one chunk).
+ * hist_w / hist_w2: arrays of n_dets pointers to caller buffers of
+ * dets[d].n_bins doubles, accumulated into (not cleared).
+ * neutrons16 / isect8 / iidx (may be NULL): final state of the last chunk.
+ * n_threads <= 0: OpenMP default.  Returns 0 or a negative error. */
+int oracle_run(const ramp_plan_desc *desc, uint64_t n_hist, uint64_t chunk,
+               const uint32_t *seeds, uint32_t n_seeds, double **hist_w, double **hist_w2,
+               float *neutrons16, float *isect8, uint32_t *iidx, int n_threads,
+               uint64_t *n_oob_out);
+
+const char *oracle_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,80 @@
+"""ctypes front end of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Runs an instrument plan (ramp_amd.Instrument.plan_desc()) through
+liboracle.so, the plain-C restatement of the reference kernels.  Imported
+only by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        from ramp_amd import _capi as capi
+        lib = C.CDLL(LIB)
+        lib.oracle_run.restype = C.c_int
+        lib.oracle_run.argtypes = [C.POINTER(capi.PlanDesc), C.c_uint64, C.c_uint64,
+                                   C.POINTER(C.c_uint32), C.c_uint32,
+                                   C.POINTER(C.POINTER(C.c_double)),
+                                   C.POINTER(C.POINTER(C.c_double)),
+                                   C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                   C.POINTER(C.c_uint32), C.c_int, C.POINTER(C.c_uint64)]
+        lib.oracle_last_error.restype = C.c_char_p
+        lib.oracle_threefry4x32.restype = None
+        lib.oracle_threefry4x32.argtypes = [C.POINTER(C.c_uint32)] * 3
+        _lib = lib
+    return _lib
+
+
+def run(inst, n_hist, seeds, chunk=0, keep_state=False, threads=0):
+    """Run `inst` (a ramp_amd.Instrument) on the oracle.
+
+    Returns dict: 'hist' list of (w, w2) float64 per detector slot, 'oob',
+    and with keep_state the final AoS state (n16, isect8, iidx) of the
+    last chunk."""
+    lib = load()
+    desc = inst.plan_desc()
+    dets = [int(desc.dets[i].n_bins) for i in range(desc.n_dets)]
+    hw = [np.zeros(n, np.float64) for n in dets]
+    hw2 = [np.zeros(n, np.float64) for n in dets]
+    nd = max(1, len(dets))
+    pw = (C.POINTER(C.c_double) * nd)()
+    pw2 = (C.POINTER(C.c_double) * nd)()
+    for i in range(len(dets)):
+        pw[i] = hw[i].ctypes.data_as(C.POINTER(C.c_double))
+        pw2[i] = hw2[i].ctypes.data_as(C.POINTER(C.c_double))
+    seeds = np.ascontiguousarray(seeds, dtype=np.uint32)
+    ch = int(chunk) if chunk else int(n_hist)
+    last = int(n_hist) - (int(np.ceil(n_hist / ch)) - 1) * ch if n_hist else 0
+    n16 = i8 = ii = None
+    if keep_state:
+        n16 = np.zeros((last, 16), np.float32)
+        i8 = np.zeros((last, 8), np.float32)
+        ii = np.zeros(last, np.uint32)
+    oob = C.c_uint64(0)
+    rc = lib.oracle_run(C.byref(desc), int(n_hist), int(chunk),
+                        seeds.ctypes.data_as(C.POINTER(C.c_uint32)), len(seeds), pw, pw2,
+                        n16.ctypes.data_as(C.POINTER(C.c_float)) if keep_state else None,
+                        i8.ctypes.data_as(C.POINTER(C.c_float)) if keep_state else None,
+                        ii.ctypes.data_as(C.POINTER(C.c_uint32)) if keep_state else None,
+                        int(threads), C.byref(oob))
+    if rc != 0:
+        raise RuntimeError(f"oracle error {rc}: {lib.oracle_last_error().decode()}")
+    out = {"hist": list(zip(hw, hw2)), "oob": oob.value}
+    if keep_state:
+        out["state"] = (n16, i8, ii)
+    return out

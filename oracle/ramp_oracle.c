@@ -1,0 +1,1361 @@
+/* ramp_oracle.c - CPU oracle: a plain-C restatement of the reference's
+ * per-neutron kernels and host loop.  TEST INFRASTRUCTURE ONLY (see
+ * oracle.h): the product never links or calls this file.
+ *
+ * Each function cites the reference kernel it restates.  A neutron is the
+ * reference's float16 record (consts.h:91-109), an intersection its float8
+ * (consts.h:111-120).  Expressions are kept in the reference's order and
+ * types; sites where the reference promotes to double (M_PI, unsuffixed
+ * literals) are evaluated in double here and marked "FP64".
+ *
+ * Parity status of each function (how it is pinned):
+ *   rand stream            golden vectors from the reference's Random123
+ *                          header (tests/test_rng.py)
+ *   frame transforms       orthonormality / round-trip closed forms
+ *   geometry kernels       analytic ray-shape solutions (tests/test_oracle.py)
+ *   sources                sampled moments; weight closed forms
+ *   chopper                analytic open windows
+ *   guide                  R=1 transmission; reflectivity closed form
+ *   powder                 Bragg 2theta = 2 asin(lambda/2d)
+ *   incoherent/absorber    Beer-Lambert closed forms
+ *   detectors              binning arithmetic, weight conservation
+ *   isotropic, mono, ISIS  parity unpinned beyond moments / conservation
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "oracle.h"
+#include "../ramp_amd/csrc/rmath.h"
+#include "../ramp_amd/csrc/rvec.h"
+#include "../ramp_amd/csrc/rrng.h"
+#include "../ramp_amd/csrc/rconst.h"
+#include "../ramp_amd/csrc/rslots.h"
+
+static __thread char g_err[256];
+const char *oracle_last_error(void) { return g_err; }
+
+/* ---- neutron record (consts.h:91-120) ---------------------------------- */
+typedef struct {
+    float s[16];     /* 0-2 pos, 3-5 vel, 6-8 pol, 9 weight, 10 tof, 11 rng counter,
+                        12 prevcomp, 13/14 scratch, 15 dead flag */
+    float is[8];     /* pos1, t1, pos2, t2 */
+    uint32_t iidx;
+    uint32_t tid;
+} on_t;
+
+#define N_X s[0]
+#define N_Y s[1]
+#define N_Z s[2]
+#define N_VX s[3]
+#define N_VY s[4]
+#define N_VZ s[5]
+#define N_P s[9]
+#define N_TOF s[10]
+#define N_SB s[11]
+#define N_SC s[12]
+#define N_SD s[13]
+#define N_SE s[14]
+#define N_DIE s[15]
+
+static inline rf3 n_pos(const on_t *n) { return rf3_make(n->s[0], n->s[1], n->s[2]); }
+static inline rf3 n_vel(const on_t *n) { return rf3_make(n->s[3], n->s[4], n->s[5]); }
+static inline rf3 n_pol(const on_t *n) { return rf3_make(n->s[6], n->s[7], n->s[8]); }
+static inline void set_pos(on_t *n, rf3 v) { n->s[0] = v.x; n->s[1] = v.y; n->s[2] = v.z; }
+static inline void set_vel(on_t *n, rf3 v) { n->s[3] = v.x; n->s[4] = v.y; n->s[5] = v.z; }
+static inline void set_pol(on_t *n, rf3 v) { n->s[6] = v.x; n->s[7] = v.y; n->s[8] = v.z; }
+static inline rf3 is_pos1(const on_t *n) { return rf3_make(n->is[0], n->is[1], n->is[2]); }
+static inline rf3 is_pos2(const on_t *n) { return rf3_make(n->is[4], n->is[5], n->is[6]); }
+static inline void set_pos1(on_t *n, rf3 v) { n->is[0] = v.x; n->is[1] = v.y; n->is[2] = v.z; }
+static inline void set_pos2(on_t *n, rf3 v) { n->is[4] = v.x; n->is[5] = v.y; n->is[6] = v.z; }
+
+static inline void isect_reset(on_t *n)
+{
+    /* (0,0,0,1e5,0,0,0,1e5), e.g. scat/arm.cl:27-28 */
+    for (int i = 0; i < 8; i++) n->is[i] = 0.0f;
+    n->is[3] = RC_SENTINEL;
+    n->is[7] = RC_SENTINEL;
+}
+
+static inline float ornd(on_t *n) { return rr_draw(&n->N_SB, n->tid); }
+
+/* OpenCL min/max for floats as the spec defines them (box.cl:36-37) */
+static inline float ocl_min(float x, float y) { return (y < x) ? y : x; }
+static inline float ocl_max(float x, float y) { return (x < y) ? y : x; }
+
+/* 2*M_PI*u evaluated as the reference does: double product, float result */
+static inline float twopi_u(float u) { return (float)(2 * RC_M_PI_D * (double)u); }
+
+typedef struct {
+    const ramp_plan_desc *d;
+    double **hw, **hw2;
+    uint64_t oob;
+} octx;
+
+static inline void hist_add(octx *c, int det, uint64_t idx, float w)
+{
+    /* reference: CAS float atomics (detector1D.cl:3-16); here double sums */
+    uint64_t nb = c->d->dets[det].n_bins;
+    if (idx >= nb) {
+#pragma omp atomic
+        c->oob++;
+        return;
+    }
+    double wd = (double)w, w2d = (double)(w * w);
+#pragma omp atomic
+    c->hw[det][idx] += wd;
+#pragma omp atomic
+    c->hw2[det][idx] += w2d;
+}
+
+static inline const float *tab(const octx *c, const ramp_part_desc *p, int k)
+{
+    return c->d->tables + p->tab_off[k];
+}
+
+/* ======================= sources ======================================== */
+
+/* shared tail of flat_mod.cl:28-50 / cont_mod.cl:49-72: position, direction
+ * toward the target window, polarisation on the unit sphere */
+static void src_position_direction(on_t *n, const float *p, float vel)
+{
+    n->N_X = p[SP_MOD_W] * (0.5f - ornd(n));
+    n->N_Y = p[SP_MOD_H] * (0.5f - ornd(n));
+    n->N_Z = 0.0f;
+    float Dx = p[SP_TGT_W] * (0.5f - ornd(n)) - n->N_X;
+    float Dy = p[SP_TGT_H] * (0.5f - ornd(n)) - n->N_Y;
+    rf3 dir = v3_norm(rf3_make(Dx, Dy, p[SP_TGT_D]));
+    set_vel(n, v3_scale(vel, dir));
+}
+
+static void src_polarisation(on_t *n)
+{
+    /* flat_mod.cl:39-43; FP64: theta = 2*M_PI*rand() */
+    float theta = twopi_u(ornd(n));
+    float phi = rm_acos(1 - 2 * ornd(n));
+    n->s[6] = rm_sin(phi) * rm_cos(theta);
+    n->s[7] = rm_sin(phi) * rm_sin(theta);
+    n->s[8] = rm_cos(phi);
+}
+
+static void src_clear(on_t *n, int seed)
+{
+    /* flat_mod.cl:19: zero the record, sb = (float)seed + old sb */
+    float sb = n->N_SB;
+    for (int i = 0; i < 16; i++) n->s[i] = 0.0f;
+    n->N_SB = (float)seed + sb;
+}
+
+/* MFlat: flat_mod.cl:4-51 */
+static void src_flat(on_t *n, const float *p, int num_sim, int seed)
+{
+    src_clear(n, seed);
+    float E_range = p[SP_EMAX] - p[SP_EMIN];
+    float E_val = p[SP_EMIN] + E_range * ornd(n);
+    n->N_P = 1.0f / (float)num_sim;
+    float vel = RC_SE2V * rm_sqrt(E_val);
+    src_position_direction(n, p, vel);
+    src_polarisation(n);
+    n->N_DIE = 0.0f;
+    isect_reset(n);
+}
+
+/* maxwell_energy_distn: cont_mod.cl:4-9.
+ * FP64: 2.0f*rootE/sqrt(M_PI) is float/double; the products stay double. */
+static float maxwell_ref(float E, float T)
+{
+    float rootE = rm_sqrt(E);
+    float overKT = 1.0f / (RC_K_B * T);
+    double r = (double)(2.0f * rootE) / sqrt(RC_M_PI_D);
+    r = r * (double)rm_pow(overKT, (float)(3.0f / 2.0f));
+    r = r * (double)rm_exp(-E * overKT);
+    return (float)r;
+}
+
+/* MCont: cont_mod.cl:11-73; SP_TAIL_JOIN > 0 adds the 1/E tail (no
+ * reference counterpart): for E above j = TAIL_JOIN*k_B*T1 the first
+ * Maxwellian term is replaced by its value at j times j/E. */
+static void src_cont(on_t *n, const float *p, int num_sim, int seed)
+{
+    src_clear(n, seed);
+    float E_range = p[SP_EMAX] - p[SP_EMIN];
+    float E_val = p[SP_EMIN] + E_range * ornd(n);
+    float E_J = RC_MEV2J * E_val;
+    float T1 = p[SP_T1], T2 = p[SP_T2], T3 = p[SP_T3];
+    float dm1 = maxwell_ref(RC_K_B * T1, T1);
+    float dm2 = maxwell_ref(RC_K_B * T2, T2);
+    float dm3 = maxwell_ref(RC_K_B * T3, T3);
+    float term1;
+    float join = p[SP_TAIL_JOIN];
+    float EjJ = join * (RC_K_B * T1);
+    if (join > 0.0f && E_J > EjJ)
+        term1 = p[SP_I1] * (maxwell_ref(EjJ, T1) * (EjJ / E_J)) / dm1;
+    else
+        term1 = p[SP_I1] * maxwell_ref(E_J, T1) / dm1;
+    float M = term1;
+    M += p[SP_I2] * maxwell_ref(E_J, T2) / dm2;
+    M += p[SP_I3] * maxwell_ref(E_J, T3) / dm3;
+    n->N_P = M / (float)num_sim;
+    float vel = RC_SE2V * rm_sqrt(E_val);
+    src_position_direction(n, p, vel);
+    src_polarisation(n);
+    n->N_DIE = 0.0f;
+    isect_reset(n);
+}
+
+/* find_closest_in_cdf: isis_mod.cl:4-13 (linear scan) */
+static int isis_find(const float *arr, int len, float val)
+{
+    for (int i = 0; i < len; i++)
+        if ((arr[i] - val) > 0.0f) return i - 1;
+    return 0;
+}
+
+/* MISIS: isis_mod.cl:15-105 */
+static void src_isis(const octx *c, on_t *n, const ramp_part_desc *sd, int num_sim, int seed)
+{
+    const float *p = sd->p;
+    const float *flux = tab(c, sd, ST_ISIS_FLUX), *tb = tab(c, sd, ST_ISIS_TBINS);
+    const float *eb = tab(c, sd, ST_ISIS_EBINS), *Eint = tab(c, sd, ST_ISIS_EINT);
+    int nt = (int)sd->ni[SI_ISIS_NT], ne = (int)sd->ni[SI_ISIS_NE];
+    src_clear(n, seed);
+    float deviate = p[SP_ISIS_TOTAL] * ornd(n);
+    /* the reference scans nt*ne flux entries although the table holds
+     * (ne-1)*nt; the scan is bounded by the table here */
+    int nflux = (int)sd->tab_len[ST_ISIS_FLUX];
+    int idx = isis_find(flux, nt * ne < nflux ? nt * ne : nflux, deviate);
+    int Epnt = isis_find(Eint, ne, deviate);
+    int Tpnt = (int)rm_fmod((float)idx, (float)nt);
+    if (Tpnt >= nt - 1) Tpnt -= 1;
+    /* idx = -1 (deviate below flux[0]) reads time_bins[-1] in the
+     * reference; the index is clamped to 0 here */
+    if (Tpnt < 0) Tpnt = 0;
+    float time_val = tb[Tpnt];
+    float time_range = tb[Tpnt + 1] - tb[Tpnt];
+    float time_spread = flux[Epnt * nt + Tpnt + 1] - flux[Epnt * nt + Tpnt];
+    float R = deviate - flux[Epnt * nt + Tpnt];
+    R /= time_spread;
+    time_val += time_range * R;
+    float ener_spread = Eint[Epnt + 1] - Eint[Epnt];
+    int is = (Epnt > 3) ? Epnt - 3 : 0;
+    int ie = ((ne - Epnt - 1) > 3) ? Epnt + 3 : (ne - 1);
+    deviate = Eint[Epnt] + ener_spread * ornd(n);
+    float acc = 0.0f;
+    for (int i = is; i <= ie; i++) {
+        float Pj = 1.0f;
+        for (int j = is; j <= ie; j++) {
+            if (j != i) Pj *= (deviate - Eint[j]) / (Eint[i] - Eint[j]);
+            else Pj *= 1;
+        }
+        Pj *= eb[i];
+        acc += Pj;
+    }
+    float ener_val = acc;
+    float vel = RC_SE2V * rm_sqrt(ener_val);
+    src_position_direction(n, p, vel);
+    src_polarisation(n);
+    n->N_P = p[SP_ISIS_STR_AREA] * p[SP_ISIS_TOTAL] * RC_ISIS_WSCALE / (float)num_sim;
+    n->N_TOF = time_val;
+    n->N_DIE = 0.0f;
+    isect_reset(n);
+}
+
+/* MLambda (no reference counterpart): lambda ~ U[lmin, lmax], E = 81.80421/
+ * lambda^2 meV via v = K2V*2pi/lambda, weight (1/lambda)*LNORM/num_sim so
+ * that the expected total weight is 1; draw order as MFlat. */
+static void src_lambda(on_t *n, const float *p, int num_sim, int seed)
+{
+    src_clear(n, seed);
+    float lam = p[SP_LMIN] + (p[SP_LMAX] - p[SP_LMIN]) * ornd(n);
+    n->N_P = (1.0f / lam) * p[SP_LNORM] / (float)num_sim;
+    float vel = RC_K2V * (6.28318530717958647692f / lam);
+    src_position_direction(n, p, vel);
+    src_polarisation(n);
+    n->N_DIE = 0.0f;
+    isect_reset(n);
+}
+
+/* MParallel (no reference counterpart): uniform on the w x h face, velocity
+ * along +z, lambda ~ U[lmin, lmax], weight 1/num_sim. Draws: lambda, x, y,
+ * then the polarisation pair. */
+static void src_parallel(on_t *n, const float *p, int num_sim, int seed)
+{
+    src_clear(n, seed);
+    float lam = p[SP_LMIN] + (p[SP_LMAX] - p[SP_LMIN]) * ornd(n);
+    n->N_P = 1.0f / (float)num_sim;
+    n->N_X = p[SP_MOD_W] * (0.5f - ornd(n));
+    n->N_Y = p[SP_MOD_H] * (0.5f - ornd(n));
+    n->N_Z = 0.0f;
+    float vel = RC_K2V * (6.28318530717958647692f / lam);
+    n->N_VX = 0.0f;
+    n->N_VY = 0.0f;
+    n->N_VZ = vel;
+    src_polarisation(n);
+    n->N_DIE = 0.0f;
+    isect_reset(n);
+}
+
+/* ======================= frame transforms =============================== */
+
+/* frametransform.cl:4-23 */
+static void o_transform(on_t *n, const ramp_component_desc *c)
+{
+    if (n->N_DIE > 0.0f) return;
+    rmat3 R = rmat3_euler(c->rot[0], c->rot[1], c->rot[2]);
+    rf3 p = v3_sub(n_pos(n), rf3_make(c->pos[0], c->pos[1], c->pos[2]));
+    set_pos(n, rmat3_derotate(&R, p));
+    set_vel(n, rmat3_derotate(&R, n_vel(n)));
+    set_pol(n, rmat3_derotate(&R, n_pol(n)));
+}
+
+/* frametransform.cl:25-43 */
+static void o_untransform(on_t *n, const ramp_component_desc *c)
+{
+    if (n->N_DIE > 0.0f) return;
+    rmat3 R = rmat3_euler(c->rot[0], c->rot[1], c->rot[2]);
+    rf3 p = rmat3_rotate(&R, n_pos(n));
+    set_pos(n, v3_add(p, rf3_make(c->pos[0], c->pos[1], c->pos[2])));
+    set_vel(n, rmat3_rotate(&R, n_vel(n)));
+    set_pol(n, rmat3_rotate(&R, n_pol(n)));
+}
+
+/* terminator.cl:3-22 */
+static void o_terminate(on_t *n, uint32_t restore)
+{
+    if (n->N_DIE > 0.0f) return;
+    if (v3_len(is_pos2(n)) == 0.0f && restore == 0) n->N_DIE = 1.0f;
+}
+
+/* ======================= geometry ======================================= */
+
+/* geom/arm.cl:3-23 */
+static void g_arm(on_t *n, uint32_t ci)
+{
+    if (n->N_DIE > 0.0f) return;
+    set_pos2(n, rf3_make(1.0f, 1.0f, 1.0f));
+    n->iidx = ci;
+}
+
+/* geom/plane.cl:3-53 */
+static void g_plane(on_t *n, const ramp_part_desc *g, uint32_t ci)
+{
+    if (n->N_DIE > 0.0f) return;
+    rf3 vel = n_vel(n), pos = n_pos(n);
+    float t = 0.0f, x = 0.0f, y = 0.0f;
+    uint32_t o = g->ni[GI_ORIENT];
+    if (o == 0) {
+        t = (-pos.z) / vel.z;
+        x = pos.x + t * vel.x;
+        y = pos.y + t * vel.y;
+    } else if (o == 1) {
+        t = (-pos.x) / vel.x;
+        x = pos.z + t * vel.z;
+        y = pos.y + t * vel.y;
+    }
+    if ((rm_fabs(x) < g->p[GP_W] / 2.0f) && (rm_fabs(y) < g->p[GP_H] / 2.0f) && t < n->is[3] &&
+        t < n->is[7] && t > 0.0f && v3_dot(vel, rf3_make(0.0f, 0.0f, 1.0f)) > 0.0f) {
+        rf3 h = v3_add(pos, v3_scale(t, vel));
+        set_pos1(n, h);
+        set_pos2(n, h);
+        n->is[3] = t;
+        n->is[7] = t;
+        n->iidx = ci;
+    }
+}
+
+/* geom/box.cl:3-54; the private intersection is taken as zeros on a miss */
+static void g_box(on_t *n, const ramp_part_desc *g, uint32_t ci)
+{
+    if (n->N_DIE > 0.0f) return;
+    rf3 pos = n_pos(n), vel = n_vel(n);
+    float w = g->p[GP_W], h = g->p[GP_H], d = g->p[GP_D];
+    rf3 A = rf3_make(-w / 2.0f, -h / 2.0f, -d / 2.0f);
+    rf3 B = rf3_make(w / 2.0f, h / 2.0f, d / 2.0f);
+    rf3 dA = v3_sub(A, pos), dB = v3_sub(B, pos);
+    float t1 = dA.x / vel.x, t2 = dB.x / vel.x, t3 = dA.y / vel.y;
+    float t4 = dB.y / vel.y, t5 = dA.z / vel.z, t6 = dB.z / vel.z;
+    float tmin = ocl_max(ocl_max(ocl_min(t1, t2), ocl_min(t3, t4)), ocl_min(t5, t6));
+    float tmax = ocl_min(ocl_min(ocl_max(t1, t2), ocl_max(t3, t4)), ocl_max(t5, t6));
+    float is[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (tmax < 0.0f) {
+    } else if (tmin > tmax) {
+    } else {
+        rf3 p1 = v3_add(pos, v3_scale(tmin, vel)), p2 = v3_add(pos, v3_scale(tmax, vel));
+        is[0] = p1.x; is[1] = p1.y; is[2] = p1.z; is[3] = tmin;
+        is[4] = p2.x; is[5] = p2.y; is[6] = p2.z; is[7] = tmax;
+        n->iidx = ci;
+    }
+    memcpy(n->is, is, sizeof is);
+}
+
+/* geom/sphere.cl:3-44 (exterior) and geom/sphere_int.cl:3-43 (interior) */
+static void g_sphere(on_t *n, const ramp_part_desc *g, uint32_t ci, int interior)
+{
+    if (n->N_DIE > 0.0f) return;
+    rf3 pos = n_pos(n), vel = n_vel(n);
+    float r = g->p[GP_R];
+    float a = v3_dot(vel, vel);
+    float b = 2.0f * v3_dot(vel, pos);
+    float c = v3_dot(pos, pos) - r * r;
+    float q = b * b - 4.0f * a * c;
+    if (q > 0.0f) {
+        float d1 = (-b - rm_sqrt(q)) / (2.f * a);
+        float d2 = (-b + rm_sqrt(q)) / (2.f * a);
+        if (!interior) {
+            if (d1 < n->is[3] && d1 > 0.f) {
+                set_pos1(n, v3_add(pos, v3_scale(d1, vel)));
+                n->is[3] = d1;
+                n->iidx = ci;
+                set_pos2(n, v3_add(pos, v3_scale(d2, vel)));
+                n->is[7] = d2;
+            }
+        } else {
+            if (d1 < 0.0f && d2 < n->is[3]) {
+                set_pos1(n, v3_add(pos, v3_scale(d2, vel)));
+                n->is[3] = d2;
+                n->iidx = ci;
+                set_pos2(n, v3_add(pos, v3_scale(d2, vel)));
+                n->is[7] = d2;
+            }
+        }
+    }
+}
+
+/* geom/cylinder_ext.cl:3-53 (banana = 0) and geom/banana.cl:3-55 (banana = 1) */
+static void g_cyl(on_t *n, const ramp_part_desc *g, uint32_t ci, int banana)
+{
+    if (n->N_DIE > 0.0f) return;
+    rf2 pp = rf2_make(n->N_X, n->N_Z), pv = rf2_make(n->N_VX, n->N_VZ);
+    float r = g->p[GP_R], hgt = g->p[GP_HEIGHT];
+    float a = v2_dot(pv, pv);
+    float b = 2.0f * v2_dot(pp, pv);
+    float c = v2_dot(pp, pp) - r * r;
+    float q = b * b - 4.0f * a * c;
+    float t1 = (-b - rm_sqrt(q)) / (2.f * a);
+    float t2 = (-b + rm_sqrt(q)) / (2.f * a);
+    float yh = n->N_Y + t2 * n->N_VY;
+    int cond = (q > 0.0f) && ((-hgt / 2.0f) < yh) && (yh < (hgt / 2));
+    if (banana) {
+        /* banana.cl:38: (mintheta < theta2 < maxtheta) is a chained C
+         * comparison: the 0/1 result of the first compare is compared with
+         * maxtheta */
+        rf2 e = v2_norm(rf2_make(pp.x + t2 * pv.x, pp.y + t2 * pv.y));
+        float theta2 = rm_acos(v2_dot(e, rf2_make(0.0f, 1.0f)));
+        float first = (g->p[GP_MINTH] < theta2) ? 1.0f : 0.0f;
+        cond = cond && (first < g->p[GP_MAXTH]);
+    }
+    if (!cond) return;
+    int hit = banana ? (t2 < n->is[3] && t1 < 0.0f && t2 > 0.0f)
+                     : (t1 < n->is[3] && t1 > 0.0f && t2 > 0.0f);
+    if (hit) {
+        rf3 pos = n_pos(n), vel = n_vel(n);
+        set_pos1(n, v3_add(pos, v3_scale(t1, vel)));
+        n->is[3] = t1;
+        n->iidx = ci;
+        set_pos2(n, v3_add(pos, v3_scale(t2, vel)));
+        n->is[7] = t2;
+    }
+}
+
+/* geom/mesh.cl:3-130: AABB precheck then brute-force Moller-Trumbore */
+static void g_mesh(const octx *cx, on_t *n, const ramp_part_desc *g, uint32_t ci)
+{
+    if (n->N_DIE > 0.0f) return;
+    rf3 pos = n_pos(n), vel = n_vel(n);
+    rf3 ll = rf3_make(g->p[GP_XMIN], g->p[GP_YMIN], g->p[GP_ZMIN]);
+    rf3 ur = rf3_make(g->p[GP_XMAX], g->p[GP_YMAX], g->p[GP_ZMAX]);
+    rf3 dA = v3_sub(ll, pos), dB = v3_sub(ur, pos);
+    float t1 = dA.x / vel.x, t2 = dB.x / vel.x, t3 = dA.y / vel.y;
+    float t4 = dB.y / vel.y, t5 = dA.z / vel.z, t6 = dB.z / vel.z;
+    float tmin = ocl_max(ocl_max(ocl_min(t1, t2), ocl_min(t3, t4)), ocl_min(t5, t6));
+    float tmax = ocl_min(ocl_min(ocl_max(t1, t2), ocl_max(t3, t4)), ocl_max(t5, t6));
+    if (tmax < 0.0f || tmin > tmax) return;
+    const float *pts = tab(cx, g, GT_POINTS);
+    uint32_t ntri = g->ni[GI_NTRI];
+    const float eps = 0.000000001f;
+    float out[8] = {0.0f, 0.0f, 0.0f, 1e8f, 0.0f, 0.0f, 0.0f, 1e8f};
+    uint32_t nint = 0;
+    for (uint32_t i = 0; i < ntri; i++) {
+        const float *P = pts + 9 * (size_t)i;
+        rf3 v0 = rf3_make(P[0], P[1], P[2]), v1 = rf3_make(P[3], P[4], P[5]);
+        rf3 v2 = rf3_make(P[6], P[7], P[8]);
+        rf3 e1 = v3_sub(v1, v0), e2 = v3_sub(v2, v0);
+        rf3 h = v3_cross(vel, e2);
+        float a = v3_dot(e1, h);
+        if (rm_fabs(a) < eps) continue;
+        float f = 1.0f / a;
+        rf3 s = v3_sub(pos, v0);
+        float u = f * v3_dot(s, h);
+        if (u < 0.0f || u > 1.0f) continue;
+        rf3 q = v3_cross(s, e1);
+        float v = f * v3_dot(vel, q);
+        if (v < 0.0f || u + v > 1.0f) continue;
+        float t = f * v3_dot(e2, q);
+        if (t > eps) nint++;
+        rf3 nrm = v3_cross(e1, e2);
+        float nv = v3_dot(nrm, vel);
+        if ((t > eps) && (t < out[3]) && (nv < 0.0f)) {
+            rf3 hp = v3_add(pos, v3_mulv(vel, t));
+            out[0] = hp.x; out[1] = hp.y; out[2] = hp.z; out[3] = t;
+        }
+        if ((t > eps) && (t < out[7]) && (nv > 0.0f)) {
+            rf3 hp = v3_add(pos, v3_mulv(vel, t));
+            out[4] = hp.x; out[5] = hp.y; out[6] = hp.z; out[7] = t;
+        }
+    }
+    uint32_t interior = g->ni[GI_INTERIOR];
+    if (interior == 1 && (nint % 2) == 1) {
+        out[4] = out[0]; out[5] = out[1]; out[6] = out[2]; out[7] = out[3];
+    } else if (interior == 1 && (nint % 2) == 0) {
+        return;
+    }
+    if (out[3] < n->is[3]) {
+        memcpy(n->is, out, sizeof out);
+        n->iidx = ci;
+    }
+}
+
+static void o_geom(const octx *cx, on_t *n, const ramp_component_desc *c)
+{
+    const ramp_part_desc *g = &c->geom;
+    switch (g->type) {
+    case RAMP_G_ARM: g_arm(n, c->comp_idx); break;
+    case RAMP_G_PLANE: g_plane(n, g, c->comp_idx); break;
+    case RAMP_G_BOX: g_box(n, g, c->comp_idx); break;
+    case RAMP_G_SPHERE: g_sphere(n, g, c->comp_idx, 0); break;
+    case RAMP_G_SPHERE_INT: g_sphere(n, g, c->comp_idx, 1); break;
+    case RAMP_G_CYLINDER: g_cyl(n, g, c->comp_idx, 0); break;
+    case RAMP_G_BANANA: g_cyl(n, g, c->comp_idx, 1); break;
+    case RAMP_G_MESH: g_mesh(cx, n, g, c->comp_idx); break;
+    default: break;
+    }
+}
+
+/* ======================= scattering ===================================== */
+
+/* common epilogue: iidx = 0, prevcomp = comp, reset intersection */
+static inline void s_done(on_t *n, uint32_t ci)
+{
+    n->iidx = 0;
+    n->N_SC = (float)ci;
+    isect_reset(n);
+}
+
+/* scat/arm.cl:3-30 (leaves iidx = comp_idx, does not set prevcomp) */
+static void s_arm(on_t *n, uint32_t ci)
+{
+    n->iidx = ci;
+    isect_reset(n);
+}
+
+/* scat/beamstop.cl:3-32 */
+static void s_beamstop(on_t *n, uint32_t ci)
+{
+    n->N_DIE = 1.0f;
+    s_done(n, ci);
+}
+
+/* scat/chopper.cl:4-59 */
+static void s_chopper(on_t *n, const ramp_part_desc *s, uint32_t ci)
+{
+    const float *p = s->p;
+    set_pos(n, is_pos2(n));
+    n->N_TOF += n->is[7];
+    /* FP64: Tg = 2.0f*M_PI / (float)n_slits */
+    float Tg = (float)((double)2.0f * RC_M_PI_D / (double)(float)s->ni[CI_NSLITS]);
+    float pha_error = p[CP_JITTER] * 2.0f * (ornd(n) - 0.5f);
+    float thi_chop = p[CP_FREQ] * (n->N_TOF - rm_fabs(p[CP_PHASE])) + pha_error;
+    float thi_neut = rm_atan2(n->N_X, n->N_Y + p[CP_RADIUS]);
+    float thi_diff = rm_fabs(thi_chop - thi_neut);
+    float ang_diff = rm_fmod(thi_diff, Tg);
+    if (rm_fabs(ang_diff - Tg) < ang_diff) ang_diff = rm_fabs(ang_diff - Tg);
+    float width = rm_atan2(p[CP_SLIT_W], n->N_Y + p[CP_RADIUS]);
+    if (ang_diff > width / 2.0f) n->N_DIE = 1.0f;
+    s_done(n, ci);
+}
+
+/* reflectivity_func: include/ref.h:3-54.  beta is uninitialised in the
+ * reference unless W == alpha == 0; it is taken as 0 (McStas semantics). */
+static float reflectivity_ref(float q, float R0, float Qc, float alpha, float m, float W)
+{
+    float beta = 0.0f, arg, R = 0.0f;
+    if (W == 0.0 && alpha == 0.0) {
+        /* FP64: double literals, results stored in float variables */
+        m = (float)((double)m * 0.9853 + 0.1978);
+        W = (float)(-0.0002 * (double)m + 0.0022);
+        alpha = (float)(0.2304 * (double)m + 5.0944);
+        beta = (float)(-7.6251 * (double)m + 68.1137);
+        if (m <= 3) {
+            alpha = m;
+            beta = 0;
+        }
+    }
+    arg = W > 0 ? (q - m * Qc) / W : 11;
+    if (arg > 10 || m <= 0 || Qc <= 0 || R0 <= 0) return 0.0f;
+    if (m < 1) {
+        Qc *= m;
+        m = 1;
+    }
+    if (q <= Qc) return R0;
+    /* FP64: R0*0.5 is double; the product chain is double, stored as float */
+    double Rd = (double)R0 * 0.5 * (double)(1 - rm_tanh(arg));
+    Rd = Rd * (double)(1 - alpha * (q - Qc) + beta * (q - Qc) * (q - Qc));
+    R = (float)Rd;
+    return R;
+}
+
+/* scat/guide.cl:9-150 (float real_t build) */
+static void s_guide(on_t *n, const ramp_part_desc *s, uint32_t ci)
+{
+    const float *p = s->p;
+    float w1 = p[GU_W1], h1 = p[GU_H1], w2 = p[GU_W2], h2 = p[GU_H2], l = p[GU_L];
+    set_pos(n, is_pos2(n));
+    n->N_TOF += n->is[7];
+    /* FP64 (exact): ww = 0.5*(w2 - w1) etc. */
+    float ww = (float)(0.5 * (double)(w2 - w1)), hh = (float)(0.5 * (double)(h2 - h1));
+    float whalf = (float)(0.5 * (double)w1), hhalf = (float)(0.5 * (double)h1);
+    uint32_t i = 0;
+    uint32_t max_bounces = s->ni[GUI_MAXB];
+    float q = 0.0f;
+    while (1 && i < max_bounces) {
+        rf3 pos = n_pos(n), vel = n_vel(n);
+        float av = l * vel.x, bv = ww * vel.z;
+        float ah = l * vel.y, bh = hh * vel.z;
+        float vdotn_v1 = bv + av, vdotn_v2 = bv - av;
+        float vdotn_h1 = bh + ah, vdotn_h2 = bh - ah;
+        float cv1 = -whalf * l - pos.z * ww, cv2 = pos.x * l;
+        float ch1 = -hhalf * l - pos.z * hh, ch2 = pos.y * l;
+        float t1 = (l - pos.z) / vel.z, t2;
+        i = 0;
+        if (vdotn_v1 < 0.0 && (t2 = (cv1 - cv2) / vdotn_v1) < t1) { t1 = t2; i = 1; }
+        if (vdotn_v2 < 0.0 && (t2 = (cv1 + cv2) / vdotn_v2) < t1) { t1 = t2; i = 2; }
+        if (vdotn_h1 < 0.0 && (t2 = (ch1 - ch2) / vdotn_h1) < t1) { t1 = t2; i = 3; }
+        if (vdotn_h2 < 0.0 && (t2 = (ch1 + ch2) / vdotn_h2) < t1) { t1 = t2; i = 4; }
+        if (i == 0) break;
+        set_pos(n, v3_add(n_pos(n), v3_scale(t1, vel)));
+        n->N_TOF += t1;
+        float nlen2, d;
+        /* FP64: q = V2K*(-2.0)*vdotn/sqrt(nlen2); d = 2.0*vdotn/nlen2 */
+        switch (i) {
+        case 1:
+            nlen2 = l * l + ww * ww;
+            q = (float)((double)RC_V2K * (-2.0) * (double)vdotn_v1 / (double)rm_sqrt(nlen2));
+            d = (float)(2.0 * (double)vdotn_v1 / (double)nlen2);
+            n->N_VX = n->N_VX - d * l;
+            n->N_VZ = n->N_VZ - d * ww;
+            break;
+        case 2:
+            nlen2 = l * l + ww * ww;
+            q = (float)((double)RC_V2K * (-2.0) * (double)vdotn_v2 / (double)rm_sqrt(nlen2));
+            d = (float)(2.0 * (double)vdotn_v2 / (double)nlen2);
+            n->N_VX = n->N_VX + d * l;
+            n->N_VZ = n->N_VZ - d * ww;
+            break;
+        case 3:
+            nlen2 = l * l + hh * hh;
+            q = (float)((double)RC_V2K * (-2.0) * (double)vdotn_h1 / (double)rm_sqrt(nlen2));
+            d = (float)(2.0 * (double)vdotn_h1 / (double)nlen2);
+            n->N_VY = n->N_VY - d * l;
+            n->N_VZ = n->N_VZ - d * hh;
+            break;
+        case 4:
+            nlen2 = l * l + hh * hh;
+            q = (float)((double)RC_V2K * (-2.0) * (double)vdotn_h2 / (double)rm_sqrt(nlen2));
+            d = (float)(2.0 * (double)vdotn_h2 / (double)nlen2);
+            n->N_VY = n->N_VY + d * l;
+            n->N_VZ = n->N_VZ - d * hh;
+            break;
+        }
+        float refl = reflectivity_ref(q, p[GU_R0], p[GU_QC], p[GU_ALPHA], p[GU_M], p[GU_W]);
+        n->N_P *= refl;
+        /* FP64: refl < 1e-12 compares in double */
+        if ((double)refl < 1e-12 || refl > 1.0f) n->N_P = 0.0f;
+        i++;
+    }
+    if (i >= max_bounces) n->N_DIE = 1.0f;
+    s_done(n, ci);
+}
+
+/* sample_distn: scat/isotropic.cl:5-73 (nearest-CDF search, one extra draw
+ * per improvement) */
+static void iso_sample(on_t *n, const float *q, const float *w, const float *pw_cdf,
+                       const float *pq_cdf, uint32_t nq, uint32_t nw, float *omega_out,
+                       float *Q_out)
+{
+    float mindiff, u, v, omega = 0.0f, Q = 0.0f;
+    uint32_t w_index = 0;
+    u = ornd(n);
+    v = ornd(n);
+    mindiff = 1.0f;
+    for (uint32_t i = 0; i < nw; i++) {
+        if (pw_cdf[i] == 1.0) break;
+        if (rm_fabs(pw_cdf[i] - v) < mindiff) {
+            mindiff = rm_fabs(pw_cdf[i] - v);
+            float wprev = (i > 0 ? w[i - 1] : 0.0f);
+            omega = wprev + (ornd(n)) * (w[i] - wprev);
+            w_index = i;
+        }
+    }
+    mindiff = 1.0f;
+    for (uint32_t i = 0; i < nq; i++) {
+        if (pq_cdf[w_index * nq + i] == 1.0) break;
+        if (rm_fabs(pq_cdf[w_index * nq + i] - u) < mindiff) {
+            mindiff = rm_fabs(pq_cdf[w_index * nq + i] - u);
+            float qprev = (i > 0 ? q[i - 1] : 0.0f);
+            Q = qprev + (ornd(n)) * (q[i] - qprev);
+        }
+    }
+    *omega_out = omega;
+    *Q_out = Q;
+}
+
+/* scat/isotropic.cl:75-323 */
+static void s_isotropic(const octx *cx, on_t *n, const ramp_part_desc *s, uint32_t ci)
+{
+    const float *p = s->p;
+    rf3 path;
+    if (v3_eq(is_pos1(n), is_pos2(n))) path = v3_sub(is_pos2(n), n_pos(n));
+    else path = v3_sub(is_pos2(n), is_pos1(n));
+    float ki = RC_V2K * v3_len(n_vel(n));
+    rf3 normvel = v3_norm(n_vel(n));
+    float cs = p[IS_SSCAT(0)], is_ = p[IS_SSCAT(1)], ms = p[IS_SSCAT(2)];
+    /* FP64: / (2.*ki*ki), * 2200. / length, * 100. */
+    double twok2 = 2. * (double)ki * (double)ki;
+    float sigma_s = (float)((double)(cs + is_ + ms) / twok2);
+    float sigma_a = (float)((double)(p[IS_SABS(0)] + p[IS_SABS(1)] + p[IS_SABS(2)]) * 2200. /
+                            (double)v3_len(n_vel(n)));
+    float sigma_tot = sigma_a + sigma_s;
+    float mu = (float)((double)((p[IS_RHO(0)] + p[IS_RHO(1)] + p[IS_RHO(2)]) * sigma_tot) * 100.);
+    float p_trans = rm_exp(-mu * v3_len(path));
+    if (ornd(n) < p_trans) {
+        n->N_P *= p_trans;
+        set_pos(n, v3_add(is_pos2(n), v3_scale(0.0001f, v3_norm(path))));
+        n->N_TOF += n->is[7];
+        if (s->ni[ISI_TRANSMIT] == 0) n->N_DIE = 1.0f;
+        n->N_SC = (float)ci;
+        n->iidx = 0;
+        isect_reset(n);
+        return;
+    } else {
+        n->N_P *= (1 - p_trans) * sigma_s / sigma_tot;
+    }
+    float eta = ornd(n);
+    /* FP64: (-1./mu)*log(...) */
+    float path_length = (float)((-1. / (double)mu) *
+                                (double)rm_log(1 - eta * (1 - rm_exp(-mu * v3_len(path)))));
+    float u = ornd(n);
+    u *= sigma_s;
+    uint32_t flag = 0;
+    double cth = (double)cs / twok2, ith = (double)(cs + is_) / twok2;
+    if ((double)u < cth) flag = 1;
+    else if ((double)u > cth && (double)u < ith) flag = 2;
+    else if ((double)u > ith) flag = 3;
+    float omega = 0.0f, Q = 0.0f;
+    if (flag >= 1 && flag <= 3) {
+        int ch = (int)flag - 1;
+        const float *base = tab(cx, s, ch);
+        uint32_t nq = s->ni[ISI_NQ(ch)], nw = s->ni[ISI_NW(ch)];
+        iso_sample(n, base, base + nq, base + nq + nw, base + nq + 2 * nw, nq, nw, &omega, &Q);
+    }
+    if (ki * ki < (RC_E2KS * omega)) {
+        n->N_DIE = 1;
+        n->iidx = 0;
+        isect_reset(n);
+        return;
+    }
+    float kf;
+    /* FP64: 0.5*exp(-omega/(kB*temperature)) with kB a double expression */
+    double db = 0.5 * exp(-(double)omega / (RC_KB_D * (double)p[IS_TEMP]));
+    if ((double)ornd(n) > db) {
+        kf = rm_sqrt(ki * ki - RC_E2KS * omega);
+        n->N_SE = omega;
+    } else {
+        kf = rm_sqrt(ki * ki + RC_E2KS * omega);
+        n->N_SE = -omega;
+    }
+    float arg = (ki * ki + kf * kf - Q * Q) / (2 * ki * kf);
+    if (rm_fabs(arg) > 1) {
+        n->N_DIE = 1;
+        n->iidx = 0;
+        isect_reset(n);
+        return;
+    }
+    float theta = rm_acos(arg);
+    rf3 oldvel = n_vel(n);
+    float x = 1.0f, y = 1.0f, z = -(n->N_VX + n->N_VY) / (n->N_VZ);
+    rf3 perp = v3_norm(rf3_make(x, y, z));
+    float alpha = twopi_u(ornd(n)); /* FP64: 2*M_PI*rand() */
+    perp = rot_about_axis(alpha, normvel, perp);
+    alpha = theta;
+    normvel = rot_about_axis(alpha, perp, normvel);
+    float TOF = path_length / v3_len(n_vel(n));
+    n->N_SD = Q;
+    if (v3_eq(is_pos1(n), is_pos2(n))) {
+        set_pos(n, v3_scale(path_length, path));
+        n->N_TOF += TOF;
+    } else {
+        set_pos(n, v3_add(is_pos1(n), v3_scale(path_length, path)));
+        n->N_TOF += n->is[3] + TOF;
+    }
+    set_vel(n, v3_mulv(v3_mulv(normvel, kf), RC_K2V));
+    rf3 q_norm = v3_norm(v3_mulv(v3_sub(oldvel, n_vel(n)), RC_V2K));
+    if (flag == 2) {
+        set_pol(n, v3_mulv(n_pol(n), -1.0f / 3.0f));
+    } else if (flag == 3) {
+        float dp = v3_dot(n_pol(n), q_norm);
+        set_pol(n, v3_mulv(rf3_make(-q_norm.x, -q_norm.y, -q_norm.z), dp));
+    }
+    n->N_SC = (float)ci;
+    n->iidx = 0;
+    isect_reset(n);
+}
+
+/* azimuth draw of powder1.cl:68-79 / powderN.cl:117-130 (FP64 sites) */
+static float powder_phi(on_t *n, float d_phi)
+{
+    double phi;
+    if (d_phi != 0.0f) {
+        double span = (double)d_phi * RC_M_PI_D / 180.0f;
+        float ph = (float)((double)(2.0f * ornd(n) - 1.0f) * span);
+        if (ornd(n) > 0.5f) ph = (float)((double)ph + RC_M_PI_D);
+        ph = (float)((double)ph + RC_M_PI_D / 2.0f);
+        phi = ph;
+    } else {
+        phi = (float)(2.0f * RC_M_PI_D * (double)ornd(n));
+    }
+    return (float)phi;
+}
+
+static void powder_dirs(rf3 vel, rf3 *para_out, rf3 *perp_out)
+{
+    rf3 beam_para = v3_norm(vel);
+    rf3 beam_perp = v3_cross(beam_para, rf3_make(0.0f, 1.0f, 0.0f));
+    if (v3_len(beam_perp) < 1e-3f)
+        beam_perp = v3_norm(v3_cross(beam_para, rf3_make(1.0f, 0.0f, 0.0f)));
+    else
+        beam_perp = v3_norm(beam_perp);
+    *para_out = beam_para;
+    *perp_out = beam_perp;
+}
+
+/* scat/powder1.cl:5-101 */
+static void s_powder1(on_t *n, const ramp_part_desc *s, uint32_t ci)
+{
+    const float *p = s->p;
+    n->N_TOF += n->is[3];
+    rf3 path = v3_sub(is_pos2(n), is_pos1(n));
+    float full = v3_len(path);
+    float vel = v3_len(n_vel(n));
+    float sigma_scat = p[P1_SSCAT_V2] / (vel * vel * RC_V2K * RC_V2K);
+    float sigma_abs = (float)((double)p[P1_SABS_V] * 2200.0 / (double)vel); /* FP64 */
+    float p_scat = sigma_scat / (sigma_abs + sigma_scat);
+    float p_inter = 1 - rm_exp(-(sigma_scat + sigma_abs) * full);
+    float arg = p[P1_Q] * RC_K2V / (2.0f * vel);
+    if (arg > 1.0f) {
+        s_done(n, ci);
+        return;
+    }
+    float twotheta = 2.0f * rm_asin(arg);
+    n->N_P *= p_scat;
+    float pen = -rm_log(1.0f - ornd(n) * (1.0f - p_inter)) / (sigma_abs + sigma_scat);
+    rf3 para, perp;
+    powder_dirs(n_vel(n), &para, &perp);
+    float phi = powder_phi(n, p[P1_DPHI]);
+    perp = rot_about_axis(phi, para, perp);
+    para = rot_about_axis(twotheta, perp, para);
+    set_pos(n, v3_add(is_pos1(n), v3_scale(pen, path)));
+    set_vel(n, v3_scale(vel, v3_norm(para)));
+    n->N_TOF += n->is[7] - n->is[3];
+    n->N_P *= full * sigma_scat * rm_exp(-(sigma_scat + sigma_abs) * full);
+    s_done(n, ci);
+}
+
+/* scat/powderN.cl:5-157 */
+static void s_powderN(const octx *cx, on_t *n, const ramp_part_desc *s, uint32_t ci)
+{
+    const float *p = s->p;
+    const float *ssv2 = tab(cx, s, PNT_SSCAT_V2), *qs = tab(cx, s, PNT_Q);
+    uint32_t nl = s->ni[PNI_NLINES];
+    rf3 path;
+    if (v3_eq(is_pos1(n), is_pos2(n))) path = v3_sub(is_pos2(n), n_pos(n));
+    else path = v3_sub(is_pos2(n), is_pos1(n));
+    float full = v3_len(path);
+    float vel = v3_len(n_vel(n));
+    float sigma_abs = (float)((double)p[PN_SABS_V] * 2200.0 / (double)vel); /* FP64 */
+    float sum = 0.0f, sigma_scat = 0.0f;
+    for (uint32_t i = 0; i < nl; i++) {
+        sigma_scat = ssv2[i] / (vel * vel * RC_V2K * RC_V2K);
+        sum += sigma_scat;
+    }
+    float p_inter = 1 - rm_exp(-(sum + sigma_abs) * full);
+    if (ornd(n) > p_inter) {
+        n->N_P *= 1 - p_inter;
+        set_pos(n, v3_add(is_pos2(n), v3_scale(0.0001f, v3_norm(path))));
+        n->N_TOF += n->is[7];
+        if (s->ni[PNI_TRANSMIT] == 0) n->N_DIE = 1.0f;
+        n->N_SC = (float)ci;
+        n->iidx = 0;
+        isect_reset(n);
+        return;
+    }
+    uint32_t attempts = 0;
+    float q_chosen = 0.0f, arg;
+    while (1) {
+        float u = ornd(n), v = 0.0f;
+        uint32_t i = 0;
+        /* powderN.cl:81-87: the scan can run past num_lines when u is close
+         * to 1; it is bounded here by the table length (reads past the end
+         * are undefined in the reference) */
+        while (v < u) {
+            if (i >= nl) break;
+            sigma_scat = ssv2[i] / (vel * vel * RC_V2K * RC_V2K);
+            q_chosen = qs[i];
+            v += sigma_scat / sum;
+            i++;
+        }
+        arg = q_chosen * RC_K2V / (2.0f * vel);
+        if (arg < 1.0f) break;
+        else if (arg > 1.0f && attempts > 2 * nl) {
+            n->N_DIE = 1;
+            n->iidx = 0;
+            isect_reset(n);
+            return;
+        }
+        attempts++;
+    }
+    float twotheta = 2.0f * rm_asin(arg);
+    float pen = -rm_log(1.0f - ornd(n) * (1.0f - p_inter)) / (sigma_abs + sum);
+    rf3 para, perp;
+    powder_dirs(n_vel(n), &para, &perp);
+    float phi = powder_phi(n, p[PN_DPHI]);
+    perp = rot_about_axis(phi, para, perp);
+    para = rot_about_axis(twotheta, perp, para);
+    if (v3_eq(is_pos1(n), is_pos2(n))) {
+        rf3 dp = v3_scale(pen, path);
+        set_pos(n, v3_add(n_pos(n), dp));
+        n->N_TOF += v3_len(dp) / v3_len(n_vel(n));
+    } else {
+        set_pos(n, v3_add(is_pos1(n), v3_scale(pen, path)));
+        n->N_TOF += n->is[7] - n->is[3];
+    }
+    set_vel(n, v3_scale(vel, v3_norm(para)));
+    n->N_P *= (1 - rm_exp(-(sum + sigma_abs) * pen)) * sum / (sum + sigma_abs);
+    s_done(n, ci);
+}
+
+/* randnorm: monochromator.cl:10-20 (FP64: cos(2.0f*M_PI*u2) in double) */
+static float mono_randnorm(on_t *n)
+{
+    float u1 = ornd(n), u2 = ornd(n);
+    return (float)((double)rm_sqrt(-2.0f * rm_log(u1)) * cos(2.0f * RC_M_PI_D * (double)u2));
+}
+
+/* GAUSS macro: monochromator.cl:5-8 (FP64: sqrt(2.0f*M_PI)) */
+static double mono_gauss(float x, float mean, float rms)
+{
+    float e = rm_exp(-((x) - (mean)) * ((x) - (mean)) / (2.0f * (rms) * (rms)));
+    return (double)e / (sqrt(2.0f * RC_M_PI_D) * (double)rms);
+}
+
+/* scat/monochromator.cl:22-197 */
+static void s_mono(const octx *cx, on_t *n, const ramp_part_desc *s, uint32_t ci)
+{
+    const float *p = s->p;
+    const float *gx = tab(cx, s, MOT_GX), *gw = tab(cx, s, MOT_GW);
+    uint32_t nh = s->ni[MOI_NH], nv = s->ni[MOI_NV], glen = s->ni[MOI_GLEN];
+    set_pos(n, is_pos2(n));
+    n->N_TOF += n->is[7];
+    /* FP64: MIN2RAD*mosaic / sqrt(8.0f*log(2.0f)) and 2.0f*M_PI/d */
+    float l8 = rm_sqrt(8.0f * rm_log(2.0f));
+    float mos_rms_y = (float)(RC_MIN2RAD_D * (double)p[MO_MOS_H] / (double)l8);
+    float mos_rms_z = (float)(RC_MIN2RAD_D * (double)p[MO_MOS_V] / (double)l8);
+    float mos_rms_max = mos_rms_y > mos_rms_z ? mos_rms_y : mos_rms_z;
+    float Q = (float)((double)2.0f * RC_M_PI_D / (double)p[MO_D]);
+    float sw = p[MO_SLAB_W], sh = p[MO_SLAB_H], gap = p[MO_GAP];
+    float zmax = ((nh * (sw + gap)) - gap) / 2.0f, zmin = -zmax;
+    float ymax = ((nv * (sh + gap)) - gap) / 2.0f, ymin = -ymax;
+    if (n->N_Z > zmin && n->N_Z < zmax && n->N_Y > ymin && n->N_Y < ymax) {
+        float col = rm_ceil((n->N_Z - zmin) / (sw + gap));
+        float row = rm_ceil((n->N_Y - ymin) / (sh + gap));
+        float tilt_h = p[MO_RAD_H] > 0.0f
+                           ? rm_asin(((float)col - (float)(nh + 1) / 2.0f) * (sw + gap) / p[MO_RAD_H])
+                           : 0.0f;
+        float tilt_v = p[MO_RAD_V] > 0.0f
+                           ? -rm_asin(((float)row - (float)(nv + 1) / 2.0f) * (sh + gap) / p[MO_RAD_V])
+                           : 0.0f;
+        float cz = zmin + (col - 0.5f) * (sw + gap) - gap / 2.0f;
+        float cy = ymin + (row - 0.5f) * (sh + gap) - gap / 2.0f;
+        rf3 slab_pos = rf3_make(0.0f, cy, cz);
+        rmat3 SR = rmat3_euler(0.0f, tilt_h, tilt_v);
+        set_vel(n, rmat3_derotate(&SR, n_vel(n)));
+        set_pos(n, v3_sub(n_pos(n), slab_pos));
+        set_pos(n, rmat3_derotate(&SR, n_pos(n)));
+        if (rm_fabs(n->N_Z) <= sw / 2.0f && rm_fabs(n->N_Y) <= sh / 2.0f) {
+            rf3 ki = v3_scale(RC_V2K, n_vel(n));
+            float ratio = -2.0f * ki.x / Q;
+            float Q_order = rm_floor(ratio + 0.5f);
+            if (Q_order == 0.0f) Q_order = ratio < 0.0f ? -1.0f : 1.0f;
+            if (Q_order < 0.0f) Q_order = -Q_order;
+            rf3 ku = v3_norm(ki);
+            if (Q_order > 2.0f * v3_len(ki) / Q) Q_order -= 1.0f;
+            float q0 = Q_order * Q;
+            float q0x = ratio < 0.0f ? -q0 : q0;
+            float theta = rm_asin(q0 / (2.0f * v3_len(ki)));
+            float delta = rm_asin(rm_fabs(ku.x)) - theta;
+            float l12 = v2_len(rf2_make(ki.y, ki.z));
+            float p_reflect = rm_fabs(p[MO_R0]) *
+                              rm_exp(-ki.z * ki.z / (l12 * l12) * delta * delta /
+                                     (2.0f * mos_rms_y * mos_rms_y)) *
+                              rm_exp(-ki.y * ki.y / (l12 * l12) * delta * delta /
+                                     (2.0f * mos_rms_z * mos_rms_z));
+            if (ornd(n) <= p_reflect) {
+                float cos_2theta = rm_cos(2.0f * theta);
+                float k_sin_2theta = v3_len(ki) * rm_sin(2.0f * theta);
+                rf3 b = v3_norm(v3_cross(ki, rf3_make(q0x, 0.0f, 0.0f)));
+                b = v3_mulv(b, k_sin_2theta);
+                rf3 a = v3_cross(b, ku);
+                float total = 0.0f, w = 0.0f;
+                float mos_sample = mos_rms_max / rm_cos(theta);
+                rf3 c1 = v3_mulv(ki, (cos_2theta - 1.0f));
+                rf3 qv;
+                for (uint32_t i = 0; i < 100; i++) {
+                    w = 5.0f * mos_sample;
+                    float cphi = rm_cos(w), sphi = rm_sin(w);
+                    qv = v3_add(v3_add(c1, v3_scale(cphi, a)), v3_scale(sphi, b));
+                    qv.y = qv.y / mos_rms_z;
+                    qv.z = qv.z / mos_rms_y;
+                    if (qv.z * qv.z + qv.y * qv.y < (25.0f / (2.0f / 3.0f)) * (qv.x * qv.x)) break;
+                    mos_sample *= (2.0f / 3.0f);
+                }
+                for (uint32_t i = 0; i < glen; i++) {
+                    float phi = w * gx[i];
+                    float cphi = rm_cos(phi), sphi = rm_sin(phi);
+                    qv = v3_add(v3_add(c1, v3_scale(cphi, a)), v3_scale(sphi, b));
+                    p_reflect = (float)(mono_gauss((qv.z / qv.x), 0.0f, mos_rms_y) *
+                                        mono_gauss((qv.y / qv.x), 0.0f, mos_rms_z));
+                    total += gw[i] * p_reflect;
+                }
+                total *= w;
+                float phi = mos_sample * mono_randnorm(n);
+                float cphi = rm_cos(phi), sphi = rm_sin(phi);
+                qv = v3_add(v3_add(c1, v3_scale(cphi, a)), v3_scale(sphi, b));
+                p_reflect = (float)(mono_gauss((qv.z / qv.x), 0.0f, mos_rms_y) *
+                                    mono_gauss((qv.y / qv.x), 0.0f, mos_rms_z));
+                set_vel(n, v3_scale(RC_K2V, v3_add(ki, qv)));
+                p_reflect = (float)((double)p_reflect /
+                                    ((double)total * mono_gauss(phi, 0.0f, mos_sample)));
+                if (p_reflect > 1.0f) p_reflect = 1.0f;
+                n->N_P *= p_reflect;
+            }
+        }
+        set_pos(n, rmat3_rotate(&SR, n_pos(n)));
+        set_pos(n, v3_add(n_pos(n), slab_pos));
+        set_vel(n, rmat3_rotate(&SR, n_vel(n)));
+    }
+    s_done(n, ci);
+}
+
+/* wavelength of detector1D.cl:56 / detector2D.cl:99 (FP64: 2*M_PI / float) */
+static inline float det_wavelength(rf3 vel)
+{
+    return (float)(2 * RC_M_PI_D / (double)(RC_V2K * v3_len(vel)));
+}
+
+/* scat/detector1D.cl:20-77 */
+static void s_det1d(octx *cx, on_t *n, const ramp_component_desc *c)
+{
+    const ramp_part_desc *s = &c->scat;
+    float mn = s->p[D1_MIN], st = s->p[D1_STEP], mx = s->p[D1_MAX];
+    uint32_t var = s->ni[D1I_VAR];
+    float v = 0.0f;
+    if (var == 0) v = RC_VS2E * rm_pow(v3_len(n_vel(n)), 2.0f);
+    else if (var == 1) v = rm_degrees(rm_atan2(n->is[4], n->is[6]));
+    else if (var == 2) v = (1.0e6f) * (n->N_TOF + n->is[7]);
+    else if (var == 3) v = det_wavelength(n_vel(n));
+    if (mn <= v && v <= mx) {
+        uint32_t idx = (uint32_t)rm_floor((v - mn) / st);
+        hist_add(cx, c->det, idx, n->N_P);
+    }
+    if (s->ni[D1I_RESTORE] == 0) {
+        set_pos(n, is_pos2(n));
+        n->N_TOF += n->is[7];
+        n->N_DIE = 1.0f;
+    }
+    s_done(n, c->comp_idx);
+}
+
+/* axis variables: detector2D.cl:64-163 (axis 1 divX/divY use y2, :93,96) */
+static float det2d_var(const on_t *n, uint32_t var, int axis)
+{
+    float x2 = n->is[4], y2 = n->is[5], z2 = n->is[6];
+    switch (var) {
+    case 0: return x2;
+    case 1: return y2;
+    case 2: return rm_degrees(rm_atan2(x2, z2));
+    case 3: return rm_degrees(rm_atan2(y2, rm_sqrt(x2 * x2 + z2 * z2)));
+    case 4: return rm_sign(y2) * rm_degrees(rm_acos(x2 / rm_sqrt(x2 * x2 + y2 * y2)));
+    case 5: return (1.0e6f) * (n->N_TOF + n->is[7]);
+    case 6: return rm_degrees(rm_atan2(n->N_VX, axis == 1 ? y2 : n->N_VZ));
+    case 7: return rm_degrees(rm_atan2(n->N_VY, axis == 1 ? y2 : n->N_VZ));
+    case 8: return det_wavelength(n_vel(n));
+    case 9: return RC_VS2E * rm_pow(v3_len(n_vel(n)), 2.0f);
+    case 10:
+        return rm_sign(x2) * rm_degrees(rm_atan2(rm_sqrt(x2 * x2 + y2 * y2), z2));
+    default: return 0.0f;
+    }
+}
+
+/* find_idx: detector2D.cl:18-28 (round to nearest) */
+static int det2d_idx(float v, float mn, float st, float mx)
+{
+    if (mn <= v && v <= mx) return (int)rm_round((v - mn) / st);
+    return -1;
+}
+
+/* scat/detector2D.cl:31-186 */
+static void s_det2d(octx *cx, on_t *n, const ramp_component_desc *c)
+{
+    const ramp_part_desc *s = &c->scat;
+    const float *p = s->p;
+    float v1 = det2d_var(n, s->ni[D2I_VAR1], 1);
+    int a1 = det2d_idx(v1, p[D2_MIN1], p[D2_STEP1], p[D2_MAX1]);
+    float v2 = det2d_var(n, s->ni[D2I_VAR2], 2);
+    int a2 = det2d_idx(v2, p[D2_MIN2], p[D2_STEP2], p[D2_MAX2]);
+    if (!((a1 == -1) || (a2 == -1))) {
+        int flat = a1 * (int)s->ni[D2I_N2] + a2;
+        hist_add(cx, c->det, (uint64_t)(uint32_t)flat, n->N_P);
+    }
+    if (s->ni[D2I_RESTORE] == 0) {
+        set_pos(n, is_pos2(n));
+        n->N_TOF += n->is[7];
+        n->N_DIE = 1.0f;
+    }
+    s_done(n, c->comp_idx);
+}
+
+/* scat/counter.cl:18-49 */
+static void s_counter(octx *cx, on_t *n, const ramp_component_desc *c)
+{
+    hist_add(cx, c->det, 0, n->N_P);
+    n->N_SE = 1.0f;
+    s_done(n, c->comp_idx);
+}
+
+/* path through the sample along the current velocity: the isotropic/powder
+ * convention (pos1==pos2 means the neutron is inside already) */
+static rf3 sample_path(const on_t *n)
+{
+    if (v3_eq(is_pos1(n), is_pos2(n))) return v3_sub(is_pos2(n), n_pos(n));
+    return v3_sub(is_pos2(n), is_pos1(n));
+}
+
+/* SIncoherent (no reference counterpart): forced single scattering.
+ * L = |path|, Sigma = Sigma_s + Sigma_a*lambda/lambda_ref (1/cm, x100 -> 1/m),
+ * p_int = 1 - exp(-Sigma L); weight *= p_int * Sigma_s/Sigma; depth from the
+ * truncated exponential; isotropic direction (cos theta = 1 - 2u, phi =
+ * 2 pi u), elastic.  Draws: depth, cos theta, phi. */
+static void s_incoherent(on_t *n, const ramp_part_desc *s, uint32_t ci)
+{
+    const float *p = s->p;
+    rf3 path = sample_path(n);
+    float L = v3_len(path);
+    float v = v3_len(n_vel(n));
+    float lam = 6.28318530717958647692f / (RC_V2K * v);
+    float sig_s = p[IN_SIGMA_S] * 100.0f;
+    float sig_a = p[IN_SIGMA_A] * (lam / p[IN_LREF]) * 100.0f;
+    float sig = sig_s + sig_a;
+    float p_int = 1.0f - rm_exp(-sig * L);
+    n->N_P *= p_int * (sig_s / sig);
+    float depth = -rm_log(1.0f - ornd(n) * p_int) / sig;
+    float frac = depth / L;
+    rf3 start = v3_eq(is_pos1(n), is_pos2(n)) ? n_pos(n) : is_pos1(n);
+    float t_start = v3_eq(is_pos1(n), is_pos2(n)) ? 0.0f : n->is[3];
+    set_pos(n, v3_add(start, v3_scale(frac, path)));
+    n->N_TOF += t_start + depth / v;
+    float ct = 1.0f - 2.0f * ornd(n);
+    float ph = 6.28318530717958647692f * ornd(n);
+    float st = rm_sqrt(rm_fmax(0.0f, 1.0f - ct * ct));
+    set_vel(n, rf3_make(v * (st * rm_cos(ph)), v * (st * rm_sin(ph)), v * ct));
+    s_done(n, ci);
+}
+
+/* SAbsorber (no reference counterpart): Beer-Lambert transmission along the
+ * segment pos1 -> pos2; weight *= exp(-Sigma_a L); the neutron moves to pos2. */
+static void s_absorber(on_t *n, const ramp_part_desc *s, uint32_t ci)
+{
+    rf3 path = sample_path(n);
+    float L = v3_len(path);
+    n->N_P *= rm_exp(-(s->p[AB_SIGMA_A] * 100.0f) * L);
+    set_pos(n, is_pos2(n));
+    n->N_TOF += n->is[7];
+    s_done(n, ci);
+}
+
+/* SDetectorTubes (no reference counterpart): tube from the scattering angle
+ * atan2(x2, z2) in degrees, pixel from y2, TOF bin from (tof + t2) in us;
+ * floor binning over [min, max), flat index (tube*npix + pix)*ntof + tbin. */
+static void s_detpix(octx *cx, on_t *n, const ramp_component_desc *c)
+{
+    const ramp_part_desc *s = &c->scat;
+    const float *p = s->p;
+    uint32_t nt = s->ni[DTI_NTUBE], np = s->ni[DTI_NPIX], nf = s->ni[DTI_NTOF];
+    float th = rm_degrees(rm_atan2(n->is[4], n->is[6]));
+    float y = n->is[5];
+    float tof = (1.0e6f) * (n->N_TOF + n->is[7]);
+    float ft = (th - p[DT_THMIN]) / (p[DT_THMAX] - p[DT_THMIN]) * (float)nt;
+    float fp = (y - p[DT_YMIN]) / (p[DT_YMAX] - p[DT_YMIN]) * (float)np;
+    float ff = (tof - p[DT_TMIN]) / (p[DT_TMAX] - p[DT_TMIN]) * (float)nf;
+    if (ft >= 0.0f && ft < (float)nt && fp >= 0.0f && fp < (float)np && ff >= 0.0f &&
+        ff < (float)nf) {
+        uint64_t it = (uint64_t)ft, ip = (uint64_t)fp, iff = (uint64_t)ff;
+        hist_add(cx, c->det, (it * np + ip) * nf + iff, n->N_P);
+    }
+    if (s->ni[DTI_RESTORE] == 0) {
+        set_pos(n, is_pos2(n));
+        n->N_TOF += n->is[7];
+        n->N_DIE = 1.0f;
+    }
+    s_done(n, c->comp_idx);
+}
+
+static void o_scatter(octx *cx, on_t *n, const ramp_component_desc *c)
+{
+    if (n->iidx != c->comp_idx) return;
+    if (n->N_DIE > 0.0f) return;
+    const ramp_part_desc *s = &c->scat;
+    switch (s->type) {
+    case RAMP_S_ARM: s_arm(n, c->comp_idx); break;
+    case RAMP_S_BEAMSTOP: s_beamstop(n, c->comp_idx); break;
+    case RAMP_S_CHOPPER: s_chopper(n, s, c->comp_idx); break;
+    case RAMP_S_GUIDE: s_guide(n, s, c->comp_idx); break;
+    case RAMP_S_ISOTROPIC: s_isotropic(cx, n, s, c->comp_idx); break;
+    case RAMP_S_POWDER1: s_powder1(n, s, c->comp_idx); break;
+    case RAMP_S_POWDERN: s_powderN(cx, n, s, c->comp_idx); break;
+    case RAMP_S_MONO: s_mono(cx, n, s, c->comp_idx); break;
+    case RAMP_S_DET1D: s_det1d(cx, n, c); break;
+    case RAMP_S_DET2D: s_det2d(cx, n, c); break;
+    case RAMP_S_COUNTER: s_counter(cx, n, c); break;
+    case RAMP_S_INCOHERENT: s_incoherent(n, s, c->comp_idx); break;
+    case RAMP_S_ABSORBER: s_absorber(n, s, c->comp_idx); break;
+    case RAMP_S_DETPIX: s_detpix(cx, n, c); break;
+    default: break;
+    }
+}
+
+/* ======================= host loop ====================================== */
+
+/* ExecutionBlock.execute: mcramp.py:118-220 */
+static void o_block(octx *cx, on_t *n, const ramp_block_desc *b)
+{
+    const ramp_component_desc *cs = cx->d->comps + b->first;
+    if (b->linear) {
+        for (uint32_t k = 0; k < b->count; k++) {
+            const ramp_component_desc *c = cs + k;
+            o_transform(n, c);
+            o_geom(cx, n, c);
+            o_terminate(n, c->restore);
+            o_scatter(cx, n, c);
+            o_untransform(n, c);
+        }
+    } else {
+        for (uint32_t ev = 0; ev < b->multi; ev++) {
+            for (uint32_t k = 0; k < b->count; k++) {
+                o_transform(n, cs + k);
+                o_geom(cx, n, cs + k);
+                o_untransform(n, cs + k);
+            }
+            for (uint32_t k = 0; k < b->count; k++) {
+                o_transform(n, cs + k);
+                o_scatter(cx, n, cs + k);
+                o_untransform(n, cs + k);
+            }
+        }
+    }
+}
+
+static void o_source(octx *cx, on_t *n, int num_sim, int seed)
+{
+    const ramp_part_desc *s = &cx->d->source;
+    switch (s->type) {
+    case RAMP_SRC_FLAT: src_flat(n, s->p, num_sim, seed); break;
+    case RAMP_SRC_CONT: src_cont(n, s->p, num_sim, seed); break;
+    case RAMP_SRC_ISIS: src_isis(cx, n, s, num_sim, seed); break;
+    case RAMP_SRC_LAMBDA: src_lambda(n, s->p, num_sim, seed); break;
+    case RAMP_SRC_PARALLEL: src_parallel(n, s->p, num_sim, seed); break;
+    default: break;
+    }
+}
+
+/* Instrument.execute: mcramp.py:403-455 (chunk loop; buffers zeroed per
+ * execute; the rng counter and iidx carry over between chunks per slot) */
+int oracle_run(const ramp_plan_desc *desc, uint64_t n_hist, uint64_t chunk, const uint32_t *seeds,
+               uint32_t n_seeds, double **hist_w, double **hist_w2, float *neutrons16,
+               float *isect8, uint32_t *iidx, int n_threads, uint64_t *n_oob_out)
+{
+    if (!desc || !seeds) {
+        snprintf(g_err, sizeof g_err, "null descriptor or seeds");
+        return RAMP_E_INVALID;
+    }
+    if (chunk == 0 || chunk > n_hist) chunk = n_hist;
+    if (n_hist == 0) return 0;
+    uint64_t n_chunks = (n_hist + chunk - 1) / chunk;
+    if (n_seeds < n_chunks) {
+        snprintf(g_err, sizeof g_err, "need %llu seeds, got %u", (unsigned long long)n_chunks,
+                 n_seeds);
+        return RAMP_E_INVALID;
+    }
+    if (chunk > 0x7fffffffull) {
+        snprintf(g_err, sizeof g_err, "chunk too large");
+        return RAMP_E_RANGE;
+    }
+    on_t *slots = (on_t *)calloc(chunk, sizeof(on_t));
+    if (!slots) {
+        snprintf(g_err, sizeof g_err, "out of memory");
+        return RAMP_E_NOMEM;
+    }
+    octx cx;
+    cx.d = desc;
+    cx.hw = hist_w;
+    cx.hw2 = hist_w2;
+    cx.oob = 0;
+#ifdef _OPENMP
+    if (n_threads > 0) omp_set_num_threads(n_threads);
+#else
+    (void)n_threads;
+#endif
+    uint64_t remaining = n_hist, torun = 0;
+    for (uint64_t ck = 0; remaining > 0; ck++) {
+        torun = remaining > chunk ? chunk : remaining;
+        remaining -= torun;
+        int seed = (int)seeds[ck];
+        int64_t nt = (int64_t)torun;
+#pragma omp parallel for schedule(dynamic, 1024)
+        for (int64_t i = 0; i < nt; i++) {
+            on_t *n = &slots[i];
+            n->tid = (uint32_t)i;
+            o_source(&cx, n, (int)torun, seed);
+            for (uint32_t b = 0; b < desc->n_blocks; b++) o_block(&cx, n, &desc->blocks[b]);
+        }
+    }
+    if (neutrons16 || isect8 || iidx) {
+        for (uint64_t i = 0; i < torun; i++) {
+            if (neutrons16) memcpy(neutrons16 + 16 * i, slots[i].s, 16 * sizeof(float));
+            if (isect8) memcpy(isect8 + 8 * i, slots[i].is, 8 * sizeof(float));
+            if (iidx) iidx[i] = slots[i].iidx;
+        }
+    }
+    if (n_oob_out) *n_oob_out = cx.oob;
+    free(slots);
+    return 0;
+}
+
+/* Test hook: the full 4-word Threefry-4x32-20 block of the shared RNG
+ * header, compared with the reference's Random123 output by the tests. */
+void oracle_threefry4x32(const uint32_t *in, const uint32_t *key, uint32_t *out)
+{
+    rr_threefry4x32_20(in, key, out);
+}

@@ -1,0 +1,949 @@
+"""Component classes of the drop-in: same names and keyword arguments as the
+reference's wrappers (mcramp/moderator, mcramp/geom, mcramp/scat), resolved
+by name from the instrument JSON exactly as mcramp.py:58-92 resolves them.
+
+A component here does not launch anything: it converts its arguments the
+way the reference wrapper does (np.float32 / np.uint32, the same
+pre-computations) and writes them into a POD part descriptor (include/
+ramp.h, slots in csrc/rslots.h).  The Instrument fuses all components into
+one plan executed by libramp.so.
+
+Classes marked "no reference counterpart" implement configuration elements
+of the benchmark workloads that the reference cannot express.
+"""
+import os
+import re
+
+import numpy as np
+
+from . import _capi as capi
+from . import _slots as S
+
+
+class _Part:
+    kind = None
+    type_code = 0
+
+    def _init_part(self):
+        self.p = np.zeros(capi.RAMP_NP, dtype=np.float32)
+        self.ni = np.zeros(capi.RAMP_NI, dtype=np.uint32)
+        self.tables = [None] * capi.RAMP_NT
+
+    def fill(self, desc, pool):
+        """Write this part into a capi.PartDesc; tables are appended to pool."""
+        desc.type = self.type_code
+        for i in range(capi.RAMP_NP):
+            desc.p[i] = float(self.p[i])
+        for i in range(capi.RAMP_NI):
+            desc.ni[i] = int(self.ni[i])
+        for k, t in enumerate(self.tables):
+            if t is None:
+                desc.tab_off[k] = 0
+                desc.tab_len[k] = 0
+            else:
+                arr = np.ascontiguousarray(t, dtype=np.float32).ravel()
+                desc.tab_off[k] = pool.append(arr)
+                desc.tab_len[k] = arr.size
+
+    def lines(self):
+        return [[0], [0], [0]]
+
+
+class TablePool:
+    """Concatenated float32 tables of a plan (aligned to 64 floats)."""
+
+    def __init__(self):
+        self.parts = []
+        self.size = 0
+
+    def append(self, arr):
+        pad = (-self.size) % 64
+        if pad:
+            self.parts.append(np.zeros(pad, dtype=np.float32))
+            self.size += pad
+        off = self.size
+        self.parts.append(arr)
+        self.size += arr.size
+        return off
+
+    def array(self):
+        if not self.parts:
+            return np.zeros(64, dtype=np.float32)
+        return np.concatenate(self.parts).astype(np.float32)
+
+
+# ============================ sources ====================================
+
+
+class _Source(_Part):
+    kind = "source"
+    draws_per_history = 7
+
+    def _geom_args(self, mod_dim, target_dim, target_dist, E_min, E_max):
+        # flat_mod.py:14-18: float2 / float32 conversions
+        self.mod_dim = np.array(mod_dim[:2], dtype=np.float32)
+        self.target_dim = np.array(target_dim[:2], dtype=np.float32)
+        self.target_dist = np.float32(target_dist)
+        self.E_min = np.float32(E_min)
+        self.E_max = np.float32(E_max)
+        self.p[S.SP_MOD_W], self.p[S.SP_MOD_H] = self.mod_dim
+        self.p[S.SP_TGT_W], self.p[S.SP_TGT_H] = self.target_dim
+        self.p[S.SP_TGT_D] = self.target_dist
+        self.p[S.SP_EMIN] = self.E_min
+        self.p[S.SP_EMAX] = self.E_max
+
+    def weight_bound(self, num_sim):
+        """Upper bound of one history's weight (deterministic histograms)."""
+        return 1.0 / num_sim
+
+
+class MFlat(_Source):
+    """Flat energy spectrum source (mcramp/moderator/flat_mod.py:10-32)."""
+    type_code = capi.RAMP_SRC_FLAT
+
+    def __init__(self, ctx=None, spec_file="Let_Base.mcstas", mod_dim=(0.0, 0.0),
+                 target_dim=(0.0, 0.0), target_dist=0.0, E_min=0.0, E_max=0.0):
+        self._init_part()
+        self._geom_args(mod_dim, target_dim, target_dist, E_min, E_max)
+
+
+def _maxwell_ref64(E, T):
+    """float64 value of maxwell_energy_distn (cont_mod.cl:4-9) for bounds."""
+    kB = 1.38066e-23
+    return 2.0 * np.sqrt(E) / np.sqrt(np.pi) * (1.0 / (kB * T)) ** 1.5 * np.exp(-E / (kB * T))
+
+
+class MCont(_Source):
+    """Sum of three Maxwellians (mcramp/moderator/cont_mod.py:10-70).
+
+    Extension (no reference counterpart): tail_join > 0 replaces the first
+    Maxwellian above E_j = tail_join*k_B*T1 by an epithermal 1/E tail joined
+    continuously at E_j."""
+    type_code = capi.RAMP_SRC_CONT
+
+    def __init__(self, ctx=None, spec_file="Let_Base.mcstas", mod_dim=(0.0, 0.0),
+                 target_dim=(0.0, 0.0), target_dist=0.0, E_min=0.0, E_max=0.0, T1=0.0, I1=0.0,
+                 T2=0.0, I2=0.0, T3=0.0, I3=0.0, tail_join=0.0):
+        self._init_part()
+        self._geom_args(mod_dim, target_dim, target_dist, E_min, E_max)
+        self.T1, self.I1 = np.float32(T1), np.float32(I1)
+        self.T2, self.I2 = np.float32(T2), np.float32(I2)
+        self.T3, self.I3 = np.float32(T3), np.float32(I3)
+        self.str_area = np.float32(1)
+        self.p[S.SP_T1], self.p[S.SP_I1] = self.T1, self.I1
+        self.p[S.SP_T2], self.p[S.SP_I2] = self.T2, self.I2
+        self.p[S.SP_T3], self.p[S.SP_I3] = self.T3, self.I3
+        self.p[S.SP_STR_AREA] = self.str_area
+        self.p[S.SP_TAIL_JOIN] = np.float32(tail_join)
+
+    def weight_bound(self, num_sim):
+        # each term is I * M(E)/M(kT) and M(E)/M(kT) <= sqrt(1/2) e^(1/2) < 1.17
+        tot = 0.0
+        for I in (self.I1, self.I2, self.I3):
+            tot += abs(float(I)) * 1.2
+        return tot / num_sim
+
+
+class MISIS(_Source):
+    """Tabulated ISIS moderator spectrum (mcramp/moderator/isis_mod.py:10-185).
+    load_spectrum follows the reference's regex parser of the McStas
+    moderator file (isis_mod.py:64-166)."""
+    type_code = capi.RAMP_SRC_ISIS
+    draws_per_history = 8
+
+    def __init__(self, ctx=None, spec_file="Let_Base.mcstas", mod_dim=(0.0, 0.0),
+                 target_dim=(0.0, 0.0), target_dist=0.0, E_min=0.0, E_max=0.0):
+        self._init_part()
+        self._geom_args(mod_dim, target_dim, target_dist, E_min, E_max)
+        self.load_spectrum(spec_file)
+        self.calc_str_area(mod_dim, target_dim, target_dist)
+        self.p[S.SP_ISIS_TOTAL] = self.total
+        self.p[S.SP_ISIS_STR_AREA] = self.str_area
+        self.p[S.SP_ISIS_TOFFSET] = self.time_offset
+        self.ni[S.SI_ISIS_NT] = self.num_time_bins
+        self.ni[S.SI_ISIS_NE] = self.num_ener_bins
+        self.tables[S.ST_ISIS_FLUX] = self.flux
+        self.tables[S.ST_ISIS_TBINS] = self.t_bins
+        self.tables[S.ST_ISIS_EBINS] = self.e_bins
+        self.tables[S.ST_ISIS_EINT] = self.e_int
+
+    @staticmethod
+    def calc_fraction(lower, upper, minimum, maximum):
+        bin_range = upper - lower
+        frac = (upper - minimum) / bin_range if (minimum > lower) else 1.0
+        frac -= (upper - maximum) / bin_range if (maximum < upper) else 0.0
+        return frac
+
+    def calc_str_area(self, mod_dim, target_dim, target_dist):
+        # isis_mod.py:43-62
+        n_steps = 50
+        A = 0.0
+        tx = np.linspace(0, target_dim[0] / 2.0, num=n_steps)
+        ty = np.linspace(0, target_dim[1] / 2.0, num=n_steps)
+        TX, TY = np.meshgrid(tx, ty)
+        mx_ = np.linspace(-mod_dim[0] / 2.0, mod_dim[0] / 2.0, num=n_steps)
+        my_ = np.linspace(-mod_dim[1] / 2.0, mod_dim[1] / 2.0, num=n_steps)
+        MX, MY = np.meshgrid(mx_, my_)
+        for mx in MX:
+            for my in MY:
+                A += np.sum(1 / (np.power(-1.0 * TX + mx, 2.0) + np.power(-1.0 * TY + my, 2.0)
+                                 + target_dist ** 2.0))
+        A *= (mod_dim[0] * mod_dim[1]) / (n_steps ** 4.0)
+        A *= target_dim[0] * target_dim[1] * 10000
+        self.str_area = np.float32(A)
+
+    def load_spectrum(self, spec_file):
+        # isis_mod.py:64-166
+        with open(spec_file, "r") as mod_file:
+            lines = mod_file.read()
+            pattern = r"(?:\d*\.\d*[e][+\-]?\d+ \d*\.\d*[e][+\-]?\d+ \(\d*\.\d*[e][+\-]?\d+\) )"
+            result = re.findall(pattern, lines)
+            mod_file.seek(0)
+            line = ""
+            offset_linenum = 0
+            while "TimeOffset" not in line:
+                line = mod_file.readline()
+                offset_linenum += 1
+            self.time_offset = np.float32(line.split()[-1])
+            time_linenum = offset_linenum
+            while "time" not in line:
+                line = mod_file.readline()
+                time_linenum += 1
+            total_linenum = time_linenum
+            while "total" not in line:
+                line = mod_file.readline()
+                total_linenum += 1
+            mod_file.seek(0)
+            e_uppers, e_lowers = [], []
+            for line in mod_file:
+                if "energy bin" in line:
+                    bounds = re.findall(r"(?:\d*\.\d*[e][+\-]?\d+)", line)
+                    e_uppers.append(float(bounds[1]) * 1e9)
+                    e_lowers.append(float(bounds[0]) * 1e9)
+        num_time_bins = total_linenum - time_linenum - 1
+        data = np.array([r.strip().replace("(", "").replace(")", "") for r in result])
+        data = np.reshape(data, (int(np.size(data) / num_time_bins), num_time_bins))
+        e_min_idx = np.searchsorted(e_lowers, self.E_min)
+        e_max_idx = np.searchsorted(e_lowers, self.E_max)
+        if self.E_min == e_lowers[0]:
+            e_min_idx += 1
+        e_bins = [self.E_min] + list(e_lowers[e_min_idx:e_max_idx]) + [self.E_max]
+        e_bins = np.array(e_bins)
+        fractions = [self.calc_fraction(e_lowers[i], e_uppers[i], self.E_min, self.E_max)
+                     for i in range(e_min_idx - 1, e_max_idx)]
+        t_bins = np.array([l.split(" ")[0] for l in data[0]]).astype(float) / 1e8
+        i_vals = np.zeros(data.shape)
+        for i in range(data.shape[0]):
+            i_vals[i] = np.array([l.split(" ")[1] for l in data[i]]).astype(float)
+        crop = i_vals[e_min_idx - 1:e_max_idx, :]
+        fr = np.repeat(np.array([[f] for f in fractions]), crop.shape[1], axis=1)
+        crop = np.multiply(crop, fr)
+        flux = np.cumsum(crop.flatten()).astype(np.float32)
+        e_int = np.insert((flux.reshape(crop.shape)[:, -1]), 0, 0.0).astype(np.float32)
+        self.num_time_bins = np.int32(num_time_bins)
+        self.num_ener_bins = np.int32(len(e_bins))
+        self.flux = flux
+        self.t_bins = t_bins.astype(np.float32)
+        self.e_bins = e_bins.astype(np.float32)
+        self.e_int = e_int
+        self.total = np.float32(np.sum(crop))
+
+    def weight_bound(self, num_sim):
+        return float(self.str_area) * float(self.total) * 6.2415093e12 / num_sim
+
+
+class MLambda(_Source):
+    """Wavelength-uniform source with weight proportional to 1/lambda,
+    normalised to unit expected total weight.  No reference counterpart:
+    MFlat samples energy, not wavelength.  Draw order follows MFlat."""
+    type_code = capi.RAMP_SRC_LAMBDA
+
+    def __init__(self, ctx=None, mod_dim=(0.0, 0.0), target_dim=(0.0, 0.0), target_dist=0.0,
+                 lambda_min=1.0, lambda_max=10.0):
+        self._init_part()
+        self._geom_args(mod_dim, target_dim, target_dist, 0.0, 0.0)
+        lmin, lmax = float(lambda_min), float(lambda_max)
+        if not (0 < lmin < lmax):
+            raise ValueError("MLambda needs 0 < lambda_min < lambda_max")
+        self.p[S.SP_LMIN] = np.float32(lmin)
+        self.p[S.SP_LMAX] = np.float32(lmax)
+        self.p[S.SP_LNORM] = np.float32((lmax - lmin) / np.log(lmax / lmin))
+        self._wmax = float(self.p[S.SP_LNORM]) / lmin
+
+    def weight_bound(self, num_sim):
+        return self._wmax * 1.001 / num_sim
+
+
+class MParallel(_Source):
+    """Parallel beam along +z over a width x height face, wavelength uniform.
+    No reference counterpart (MFlat always aims at a target window)."""
+    type_code = capi.RAMP_SRC_PARALLEL
+    draws_per_history = 5
+
+    def __init__(self, ctx=None, width=0.0, height=0.0, lambda_min=1.0, lambda_max=10.0):
+        self._init_part()
+        self.p[S.SP_MOD_W] = np.float32(width)
+        self.p[S.SP_MOD_H] = np.float32(height)
+        self.p[S.SP_LMIN] = np.float32(lambda_min)
+        self.p[S.SP_LMAX] = np.float32(lambda_max)
+
+
+# ============================ geometry ===================================
+
+
+class _Geom(_Part):
+    kind = "geom"
+
+    def __init__(self):
+        self._init_part()
+
+
+class GArm(_Geom):
+    """geom/arm.py:7-37"""
+    type_code = capi.RAMP_G_ARM
+
+    def __init__(self, idx=0, ctx=None):
+        super().__init__()
+        self.idx = np.uint32(idx)
+
+
+class GPlane(_Geom):
+    """geom/plane.py:9-77"""
+    type_code = capi.RAMP_G_PLANE
+
+    def __init__(self, width=0, height=0, idx=0, orientation="xy", ctx=None):
+        super().__init__()
+        orientations = {"xy": 0, "yz": 1}
+        self.orientation_str = orientation
+        self.orientation = np.uint32(orientations[orientation])
+        self.width = np.float32(width)
+        self.height = np.float32(height)
+        self.idx = np.uint32(idx)
+        self.p[S.GP_W], self.p[S.GP_H] = self.width, self.height
+        self.ni[S.GI_ORIENT] = self.orientation
+
+    def lines(self):
+        w, h = self.width / 2.0, self.height / 2.0
+        x = [-w, w, w, -w, -w]
+        y = [h, h, -h, -h, h]
+        z = [0.0] * 5
+        return [x, y, z] if self.orientation_str == "xy" else [z, y, x]
+
+
+class GBox(_Geom):
+    """geom/box.py:9-82"""
+    type_code = capi.RAMP_G_BOX
+
+    def __init__(self, width=0, height=0, depth=0, idx=0, ctx=None):
+        super().__init__()
+        self.width, self.height, self.depth = np.float32(width), np.float32(height), np.float32(depth)
+        self.idx = np.uint32(idx)
+        self.p[S.GP_W], self.p[S.GP_H], self.p[S.GP_D] = self.width, self.height, self.depth
+
+
+class GSphere(_Geom):
+    """geom/sphere.py:9-83"""
+    type_code = capi.RAMP_G_SPHERE
+
+    def __init__(self, radius=0, idx=0, ctx=None):
+        super().__init__()
+        self.radius = np.float32(radius)
+        self.idx = np.uint32(idx)
+        self.p[S.GP_R] = self.radius
+
+
+class GSphereInt(GSphere):
+    """geom/sphere_int.py:9-81"""
+    type_code = capi.RAMP_G_SPHERE_INT
+
+
+class GCylinderExt(_Geom):
+    """geom/cylinder_ext.py:10-97"""
+    type_code = capi.RAMP_G_CYLINDER
+
+    def __init__(self, radius=0, height=0, idx=0, ctx=None):
+        super().__init__()
+        self.radius, self.height = np.float32(radius), np.float32(height)
+        self.idx = idx
+        self.p[S.GP_R], self.p[S.GP_HEIGHT] = self.radius, self.height
+
+
+class GBanana(_Geom):
+    """geom/banana.py:10-87"""
+    type_code = capi.RAMP_G_BANANA
+
+    def __init__(self, radius=0, height=0, mintheta=0, maxtheta=0, idx=0, ctx=None):
+        super().__init__()
+        self.radius, self.height = np.float32(radius), np.float32(height)
+        self.mintheta, self.maxtheta = np.float32(mintheta), np.float32(maxtheta)
+        self.idx = idx
+        self.p[S.GP_R], self.p[S.GP_HEIGHT] = self.radius, self.height
+        self.p[S.GP_MINTH], self.p[S.GP_MAXTH] = self.mintheta, self.maxtheta
+
+
+def read_stl(filename):
+    """Triangles of an ASCII or binary STL file as float32 (n, 9), the
+    layout of numpy-stl's Mesh.points used by geom/mesh.py:68-73."""
+    with open(filename, "rb") as f:
+        raw = f.read()
+    head = raw[:5].lower()
+    if head == b"solid" and b"facet" in raw[:1024]:
+        vals = re.findall(rb"vertex\s+(\S+)\s+(\S+)\s+(\S+)", raw)
+        pts = np.array(vals, dtype=np.float64).astype(np.float32).reshape(-1, 9)
+        return pts
+    n = int(np.frombuffer(raw[80:84], dtype="<u4")[0])
+    rec = np.dtype([("n", "<f4", 3), ("v", "<f4", 9), ("attr", "<u2")])
+    arr = np.frombuffer(raw[84:84 + n * 50], dtype=rec, count=n)
+    return np.ascontiguousarray(arr["v"], dtype=np.float32)
+
+
+def write_stl_binary(filename, points):
+    pts = np.asarray(points, dtype=np.float32).reshape(-1, 9)
+    rec = np.zeros(pts.shape[0], dtype=np.dtype([("n", "<f4", 3), ("v", "<f4", 9),
+                                                  ("attr", "<u2")]))
+    rec["v"] = pts
+    with open(filename, "wb") as f:
+        f.write(b"\0" * 80)
+        f.write(np.uint32(pts.shape[0]).tobytes())
+        f.write(rec.tobytes())
+
+
+class GMesh(_Geom):
+    """geom/mesh.py:11-99 (triangle mesh from an STL file).  The reference
+    traces every triangle; libramp builds a BVH on the GPU with identical
+    nearest-hit and crossing-count results."""
+    type_code = capi.RAMP_G_MESH
+
+    def __init__(self, filename="", interior=False, idx=0, ctx=None, points=None):
+        super().__init__()
+        self.idx = np.uint32(idx)
+        pts = read_stl(filename) if points is None else np.asarray(points, np.float32).reshape(-1, 9)
+        self.points = pts
+        allv = pts.reshape(-1, 3)
+        lo, hi = allv.min(axis=0), allv.max(axis=0)
+        self.p[S.GP_XMIN], self.p[S.GP_XMAX] = lo[0], hi[0]
+        self.p[S.GP_YMIN], self.p[S.GP_YMAX] = lo[1], hi[1]
+        self.p[S.GP_ZMIN], self.p[S.GP_ZMAX] = lo[2], hi[2]
+        self.num_tri = np.uint32(pts.shape[0])
+        self.interior = np.uint32(1 if interior else 0)
+        self.ni[S.GI_NTRI] = self.num_tri
+        self.ni[S.GI_INTERIOR] = self.interior
+        self.tables[S.GT_POINTS] = pts.ravel()
+
+
+# ============================ scattering =================================
+
+
+class _Scat(_Part):
+    kind = "scat"
+    n_bins = 0  # histogram bins per moment (0: no histogram)
+
+    def __init__(self):
+        self._init_part()
+        self._w = None
+        self._w2 = None
+
+    def _set_hist(self, w, w2):
+        self._w, self._w2 = w, w2
+
+    def data_reduce(self, queue=None):
+        return 0
+
+    def data(self, queue=None):
+        return 0
+
+    def plot(self, queue=None):
+        return 0
+
+    def save(self, queue=None):
+        return 0
+
+
+class SArm(_Scat):
+    """scat/arm.py:7-38"""
+    type_code = capi.RAMP_S_ARM
+
+    def __init__(self, idx=0, ctx=None, **kwargs):
+        super().__init__()
+        self.idx = np.uint32(idx)
+
+
+class SBeamstop(_Scat):
+    """scat/beamstop.py:8-40"""
+    type_code = capi.RAMP_S_BEAMSTOP
+
+    def __init__(self, idx=0, ctx=None, filename=None, **kwargs):
+        super().__init__()
+        self.idx = np.uint32(idx)
+
+
+class SChopper(_Scat):
+    """scat/chopper.py:10-70 (freq is an angular frequency in rad/s)"""
+    type_code = capi.RAMP_S_CHOPPER
+
+    def __init__(self, slit_width=0.0, radius=0.0, freq=0.0, n_slits=0, phase=0.0, jitter=0.0,
+                 idx=0, ctx=0, **kwargs):
+        super().__init__()
+        self.slit_width = np.float32(slit_width)
+        self.radius = np.float32(radius)
+        self.freq = np.float32(freq)
+        self.n_slits = np.uint32(n_slits)
+        self.phase = np.float32(phase)
+        self.jitter = np.float32(jitter)
+        self.idx = np.uint32(idx)
+        self.p[S.CP_SLIT_W], self.p[S.CP_RADIUS] = self.slit_width, self.radius
+        self.p[S.CP_FREQ], self.p[S.CP_PHASE], self.p[S.CP_JITTER] = self.freq, self.phase, self.jitter
+        self.ni[S.CI_NSLITS] = self.n_slits
+
+
+class SGuide(_Scat):
+    """scat/guide.py:10-123 (float build; CONFIG_USE_DOUBLE is not offered)"""
+    type_code = capi.RAMP_S_GUIDE
+
+    def __init__(self, w1=0, h1=0, w2=0, h2=0, l=0, R0=0, Qc=0, alpha=0, m=1, W=0, idx=0, ctx=0,
+                 max_bounces=50, **kwargs):
+        super().__init__()
+        vals = [w1, h1, w2, h2, l, R0, Qc, alpha, m, W]
+        (self.w1, self.h1, self.w2, self.h2, self.l, self.R0, self.Qc, self.alpha, self.m,
+         self.W) = [np.float32(v) for v in vals]
+        self.idx = np.uint32(idx)
+        self.max_bounces = np.uint32(max_bounces)
+        for slot, v in zip((S.GU_W1, S.GU_H1, S.GU_W2, S.GU_H2, S.GU_L, S.GU_R0, S.GU_QC,
+                            S.GU_ALPHA, S.GU_M, S.GU_W), vals):
+            self.p[slot] = np.float32(v)
+        self.ni[S.GUI_MAXB] = self.max_bounces
+
+
+def _simps_avg(y, x, axis=-1):
+    """Composite Simpson integration with the even-count rule of SciPy's
+    former simps(even='avg') (the average of the first-interval and
+    last-interval trapezoid corrections), written out here because simps
+    no longer exists in SciPy; used by the S(Q,w) loader (isotropic.py:
+    201-211).  Parity with the reference's SciPy is unpinned."""
+    y = np.asarray(y, dtype=np.float64)
+    x = np.asarray(x, dtype=np.float64)
+    y = np.moveaxis(y, axis, -1)
+    n = y.shape[-1]
+
+    def simpson_odd(yy, xx):
+        m = yy.shape[-1]
+        if m == 1:
+            return np.zeros(yy.shape[:-1])
+        h = np.diff(xx)
+        tot = np.zeros(yy.shape[:-1])
+        for i in range(0, m - 2, 2):
+            h0, h1 = h[i], h[i + 1]
+            hs = h0 + h1
+            tot = tot + hs / 6.0 * (yy[..., i] * (2 - h1 / h0) + yy[..., i + 1] * hs * hs / (h0 * h1)
+                                    + yy[..., i + 2] * (2 - h0 / h1))
+        return tot
+
+    if n < 2:
+        return np.zeros(y.shape[:-1])
+    if n % 2 == 1:
+        return simpson_odd(y, x)
+    first = simpson_odd(y[..., :-1], x[:-1]) + 0.5 * (x[-1] - x[-2]) * (y[..., -1] + y[..., -2])
+    last = simpson_odd(y[..., 1:], x[1:]) + 0.5 * (x[1] - x[0]) * (y[..., 0] + y[..., 1])
+    return 0.5 * (first + last)
+
+
+def _cdf_from_pd(x, pd):
+    # isotropic.py:210-212
+    un = np.array([_simps_avg(pd[:i], x[:i]) for i in range(1, len(x) + 1)], dtype=np.float32)
+    return un / un[-1]
+
+
+class SIsotropic(_Scat):
+    """S(Q,w) isotropic scatterer (scat/isotropic.py:12-212).  _LoadSQW
+    follows the reference text; its integrations use _simps_avg."""
+    type_code = capi.RAMP_S_ISOTROPIC
+
+    def __init__(self, fn_coh="", fn_inc="", fn_mag="", temperature=0, transmit=1, idx=0,
+                 ctx=None, **kwargs):
+        super().__init__()
+        self.temperature = np.float32(temperature)
+        self.transmit = np.uint32(transmit)
+        if fn_coh == "" and fn_inc == "" and fn_mag == "":
+            raise ValueError("Invalid filename combination")
+        self.idx = np.uint32(idx)
+        for ch, fn in enumerate((fn_coh, fn_inc, fn_mag)):
+            if fn == "":
+                # dummy tables of the reference (isotropic.py:70-81)
+                q = w = pw = pq = np.zeros(1, np.float32)
+                nq = nw = 0
+                rho = sa = ss = np.float32(0)
+            else:
+                q, w, rho, sa, ss, pw, pq, _ = self._LoadSQW(fn)
+                pq = pq.flatten()
+                nq, nw = len(q), len(w)
+            self.p[S.slot("IS_RHO", ch)] = np.float32(rho)
+            self.p[S.slot("IS_SABS", ch)] = np.float32(sa)
+            self.p[S.slot("IS_SSCAT", ch)] = np.float32(ss)
+            self.ni[S.slot("ISI_NQ", ch)] = nq
+            self.ni[S.slot("ISI_NW", ch)] = nw
+            self.tables[ch] = np.concatenate([np.asarray(a, np.float32).ravel()
+                                              for a in (q, w, pw, pq)])
+        self.p[S.IS_TEMP] = self.temperature
+        self.ni[S.ISI_TRANSMIT] = self.transmit
+
+    @staticmethod
+    def _LoadSQW(fn):
+        header = {}
+        flag = 0
+        with open(fn, "r") as fin:
+            for line in fin:
+                d = re.findall(r"#\s+([A-za-z_]+)\s+([-+]?\d*\.\d+|\d+)", line)
+                if d:
+                    header[d[0][0]] = d[0][1]
+                if not line[0] == "#":
+                    if flag == 0:
+                        q = np.array(line.split(), dtype=np.float32)[:-1]
+                        flag += 1
+                        continue
+                    elif flag == 1:
+                        w = np.array(line.split(), dtype=np.float32)
+                        flag += 1
+                        continue
+                    else:
+                        break
+            sqw = np.genfromtxt(fin.readlines()).astype(np.float32) + \
+                np.finfo(float).eps.astype(np.float32)
+        sigma_abs = np.float32(header["sigma_abs"])
+        sigma_coh = np.float32(header["sigma_coh"])
+        rho = np.float32(header["V_rho"])
+        sigma_scat = sigma_coh
+        pw = _simps_avg(q * sqw.T, q, axis=1)
+        pw /= _simps_avg(pw, w)
+        pq = np.array([q * sqw[:, i] / _simps_avg(sqw.T, w, axis=0) for i in range(0, len(w))])
+        pw_cdf = _cdf_from_pd(w, pw)
+        pq_cdf = np.array([_cdf_from_pd(q, pq[j, :]) for j in range(0, len(w))], dtype=np.float32)
+        return (q, w, rho, sigma_abs, sigma_scat, pw_cdf, pq_cdf, sqw)
+
+
+def _powder_sigma(d_spacing, pack, vc, sigma_abs, multiplicity, DW, F2):
+    # powderN.py:59-64 / powder1.py:57-63
+    q = np.array((2 * np.pi) / np.array(d_spacing), dtype=np.float32)
+    numer = np.array(4 * np.pi ** 3.0 * pack * np.array(multiplicity) * np.array(F2) * 100 * DW,
+                     dtype=np.float32)
+    denom = vc ** 2.0 * q
+    return q, numer / denom, np.float32(pack * sigma_abs / vc * 100)
+
+
+class SPowder1(_Scat):
+    """Single Bragg line (scat/powder1.py:9-81)"""
+    type_code = capi.RAMP_S_POWDER1
+
+    def __init__(self, d_spacing=0.0, pack=0.0, vc=0.0, sigma_abs=0.0, multiplicity=0, DW=0.0,
+                 F2=0.0, d_phi=180.0, idx=0, ctx=None, **kwargs):
+        super().__init__()
+        q = (2 * np.pi) / d_spacing
+        numer = 4 * np.pi ** 3.0 * pack * multiplicity * F2 * 100 * DW
+        denom = vc ** 2.0 * q
+        self.sigma_scat_v2 = np.float32(numer / denom)
+        self.sigma_abs_v = np.float32(pack * sigma_abs / vc * 100)
+        self.q = np.float32(q)
+        self.idx = np.uint32(idx)
+        self.d_phi = np.float32(d_phi)
+        self.p[S.P1_SSCAT_V2], self.p[S.P1_SABS_V] = self.sigma_scat_v2, self.sigma_abs_v
+        self.p[S.P1_Q], self.p[S.P1_DPHI] = self.q, self.d_phi
+
+
+class SPowderN(_Scat):
+    """N Bragg lines (scat/powderN.py:9-91).
+
+    Extension: U (Debye-Waller, A^2) multiplies each F2_i by exp(-U Q_i^2),
+    Q_i = 2 pi/d_i (no reference counterpart; U = 0 leaves the reference
+    behaviour)."""
+    type_code = capi.RAMP_S_POWDERN
+
+    def __init__(self, d_spacing=[0.0], pack=0.0, vc=0.0, sigma_abs=0.0, multiplicity=[0], DW=0.0,
+                 F2=[0.0], d_phi=180.0, transmit=1, idx=0, ctx=None, U=0.0, **kwargs):
+        super().__init__()
+        F2 = np.asarray(F2, dtype=np.float64)
+        if U:
+            F2 = F2 * np.exp(-float(U) * ((2 * np.pi) / np.asarray(d_spacing, np.float64)) ** 2)
+        q, ssv2, sav = _powder_sigma(d_spacing, pack, vc, sigma_abs, multiplicity, DW, F2)
+        self.q_arr = q
+        self.sigma_scat_v2 = ssv2.astype(np.float32)
+        self.sigma_abs_v = sav
+        self.idx = np.uint32(idx)
+        self.d_phi = np.float32(d_phi)
+        self.num_lines = np.uint32(len(d_spacing))
+        self.transmit = np.uint32(transmit)
+        self.p[S.PN_SABS_V], self.p[S.PN_DPHI] = self.sigma_abs_v, self.d_phi
+        self.ni[S.PNI_NLINES], self.ni[S.PNI_TRANSMIT] = self.num_lines, self.transmit
+        self.tables[S.PNT_SSCAT_V2] = self.sigma_scat_v2
+        self.tables[S.PNT_Q] = self.q_arr
+
+
+_GAUSS_X = np.array([-0.987992518020485, -0.937273392400706, -0.848206583410427,
+                     -0.724417731360170, -0.570972172608539, -0.394151347077563,
+                     -0.201194093997435, 0, 0.201194093997435, 0.394151347077563,
+                     0.570972172608539, 0.724417731360170, 0.848206583410427,
+                     0.937273392400706, 0.987992518020485]).astype(np.float32)
+_GAUSS_W = np.array([0.030753241996117, 0.070366047488108, 0.107159220467172,
+                     0.139570677926154, 0.166269205816994, 0.186161000115562,
+                     0.198431485327111, 0.202578241925561, 0.198431485327111,
+                     0.186161000115562, 0.166269205816994, 0.139570677926154,
+                     0.107159220467172, 0.070366047488108, 0.030753241996117]).astype(np.float32)
+
+
+class SMonochromator(_Scat):
+    """Curved mosaic crystal (scat/monochromator.py:10-111).  The 15-point
+    Gauss-Legendre nodes/weights are the standard quadrature table."""
+    type_code = capi.RAMP_S_MONO
+
+    def __init__(self, slab_width=0.0, slab_height=0.0, gap=0.0, n_horizontal=1, n_vertical=1,
+                 mosaic_horizontal=0.0, mosaic_vertical=0.0, r0=1.0, d_spacing=0.0,
+                 radius_vertical=0.0, radius_horizontal=0.0, idx=0, ctx=0, **kwargs):
+        super().__init__()
+        self.idx = np.uint32(idx)
+        vals = (slab_width, slab_height, gap, mosaic_horizontal, mosaic_vertical, r0, d_spacing,
+                radius_vertical, radius_horizontal)
+        for slot, v in zip((S.MO_SLAB_W, S.MO_SLAB_H, S.MO_GAP, S.MO_MOS_H, S.MO_MOS_V, S.MO_R0,
+                            S.MO_D, S.MO_RAD_V, S.MO_RAD_H), vals):
+            self.p[slot] = np.float32(v)
+        self.ni[S.MOI_NH] = np.uint32(n_horizontal)
+        self.ni[S.MOI_NV] = np.uint32(n_vertical)
+        self.ni[S.MOI_GLEN] = np.uint32(len(_GAUSS_X))
+        self.tables[S.MOT_GX] = _GAUSS_X
+        self.tables[S.MOT_GW] = _GAUSS_W
+
+
+class _Hist(_Scat):
+    def _moments(self):
+        if self._w is None:
+            return (np.zeros(self.n_bins, np.float32), np.zeros(self.n_bins, np.float32))
+        return self._w.astype(np.float32), self._w2.astype(np.float32)
+
+
+class SDetector1D(_Hist):
+    """One-axis monitor (scat/detector1D.py:11-138)"""
+    type_code = capi.RAMP_S_DET1D
+
+    def __init__(self, binning=(0, 0, 0), logscale=False, restore_neutron=False, idx=0,
+                 var="energy", ctx=None, filename=None, **kwargs):
+        super().__init__()
+        var_dict = {"energy": 0, "theta": 1, "tof": 2, "wavelength": 3}
+        self.var_label_dict = {0: "Energy [meV]", 1: "Theta [deg]", 2: "Time-of-flight [us]",
+                               3: "Wavelength [Ang]"}
+        self.var = np.uint32(var_dict[var])
+        self.binning = np.array(binning[:3], dtype=np.float32)
+        self.idx = idx
+        self.logscale = logscale
+        self.num_bins = np.ceil((binning[2] - binning[0]) / binning[1]).astype(np.uint32)
+        self.n_bins = int(self.num_bins)
+        self.restore = np.uint32(1 if restore_neutron else 0)
+        self.filename = filename
+        self.axis = np.linspace(self.binning[0], self.binning[2], num=self.num_bins)
+        self.p[S.D1_MIN], self.p[S.D1_STEP], self.p[S.D1_MAX] = self.binning
+        self.ni[S.D1I_VAR], self.ni[S.D1I_NBINS], self.ni[S.D1I_RESTORE] = (
+            self.var, self.num_bins, self.restore)
+
+    def data(self, queue=None):
+        h, e = self._moments()
+        h[np.isnan(h)] = 0
+        e[np.isnan(e)] = 0
+        self.histo, self.histo_err = h, e
+        return (self.axis, h, e)
+
+    def get_histo(self):
+        return self.data()
+
+    def save(self, queue=None):
+        axis, h, e = self.data()
+        if self.filename:
+            np.save(self.filename + "X", axis)
+            np.save(self.filename + "Y", h)
+            np.save(self.filename + "E", e)
+
+    def plot(self, queue=None):
+        import matplotlib.pyplot as plt
+        axis, h, e = self.data()
+        plt.figure()
+        plt.errorbar(axis, np.log(h + 1e-7) if self.logscale else h,
+                     0.434 * np.sqrt(e) / np.log(h + 1e-7) if self.logscale else np.sqrt(e),
+                     capsize=8)
+        plt.ylabel("Intensity")
+        plt.xlabel(self.var_label_dict[int(self.var)])
+        plt.tight_layout()
+
+
+class SDetector2D(_Hist):
+    """Two-axis monitor (scat/detector2D.py:11-207)"""
+    type_code = capi.RAMP_S_DET2D
+
+    var_dict = {"x": 0, "y": 1, "theta": 2, "alpha": 3, "phi": 4, "tof": 5, "divX": 6, "divY": 7,
+                "wavelength": 8, "energy": 9, "angle": 10}
+
+    def __init__(self, shape="", axis1_binning=(0, 0, 0), axis2_binning=(0, 0, 0),
+                 axis1_var="theta", axis2_var="tof", restore_neutron=False, idx=0, ctx=None,
+                 filename=None, logscale=False, **kwargs):
+        super().__init__()
+        self.axis1_var = np.uint32(self.var_dict[axis1_var])
+        self.axis2_var = np.uint32(self.var_dict[axis2_var])
+        self.axis1_binning = np.array(axis1_binning[:3], dtype=np.float32)
+        self.axis2_binning = np.array(axis2_binning[:3], dtype=np.float32)
+        self.idx = np.uint32(idx)
+        self.restore_neutron = np.uint32(1 if restore_neutron else 0)
+        self.filename = filename
+        self.logscale = logscale
+        self.axis1_num_bins = np.uint32(np.ceil((axis1_binning[2] - axis1_binning[0]) /
+                                                axis1_binning[1]))
+        self.axis2_num_bins = np.uint32(np.ceil((axis2_binning[2] - axis2_binning[0]) /
+                                                axis2_binning[1]))
+        self.num_bins = np.uint32(self.axis1_num_bins * self.axis2_num_bins)
+        self.n_bins = int(self.axis1_num_bins) * int(self.axis2_num_bins)
+        x = np.linspace(self.axis1_binning[0], self.axis1_binning[2], num=self.axis1_num_bins)
+        y = np.linspace(self.axis2_binning[0], self.axis2_binning[2], num=self.axis2_num_bins)
+        self.X, self.Y = np.meshgrid(x, y)
+        self.p[S.D2_MIN1], self.p[S.D2_STEP1], self.p[S.D2_MAX1] = self.axis1_binning
+        self.p[S.D2_MIN2], self.p[S.D2_STEP2], self.p[S.D2_MAX2] = self.axis2_binning
+        self.ni[S.D2I_N1], self.ni[S.D2I_N2] = self.axis1_num_bins, self.axis2_num_bins
+        self.ni[S.D2I_VAR1], self.ni[S.D2I_VAR2] = self.axis1_var, self.axis2_var
+        self.ni[S.D2I_RESTORE] = self.restore_neutron
+
+    def data(self, queue=None):
+        h, e = self._moments()
+        shape = (int(self.axis1_num_bins), int(self.axis2_num_bins))
+        self.Z = h.reshape(shape).T
+        self.E = e.reshape(shape).T
+        return (self.X, self.Y, self.Z, self.E)
+
+    def get_histo(self):
+        return self.data()
+
+    def save(self, queue=None):
+        X, Y, Z, E = self.data()
+        if self.filename:
+            np.save(self.filename + "X", X)
+            np.save(self.filename + "Y", Y)
+            np.save(self.filename + "Z", Z)
+            np.save(self.filename + "E", E)
+
+    def plot(self, queue=None):
+        import matplotlib.pyplot as plt
+        X, Y, Z, _ = self.data()
+        plt.figure()
+        plt.pcolormesh(X, Y, np.log(Z + 1e-7) if self.logscale else Z, cmap="jet",
+                       shading="auto")
+        plt.colorbar()
+        plt.tight_layout()
+
+
+class SCounter(_Hist):
+    """Sum of detected weights (scat/counter.py:11-78)"""
+    type_code = capi.RAMP_S_COUNTER
+    n_bins = 1
+
+    def __init__(self, idx=0, ctx=None, filename=None, **kwargs):
+        super().__init__()
+        self.idx = idx
+        self.filename = filename
+
+    def data(self, queue=None):
+        h, _ = self._moments()
+        return h[0]
+
+    def save(self, queue=None):
+        if self.filename:
+            np.save(self.filename, self.data())
+
+
+class SIncoherent(_Scat):
+    """Constant-cross-section incoherent sample with forced single
+    scattering and weight correction.  No reference counterpart (SIsotropic
+    needs an S(Q,w) file and scales sigma_s by 1/(2k^2)).
+
+    sigma_s: macroscopic scattering cross-section (1/cm);
+    sigma_a: macroscopic absorption cross-section at lambda_ref (1/cm),
+    scaled linearly with wavelength."""
+    type_code = capi.RAMP_S_INCOHERENT
+
+    def __init__(self, sigma_s=0.0, sigma_a=0.0, lambda_ref=1.8, idx=0, ctx=None, **kwargs):
+        super().__init__()
+        self.idx = np.uint32(idx)
+        self.p[S.IN_SIGMA_S] = np.float32(sigma_s)
+        self.p[S.IN_SIGMA_A] = np.float32(sigma_a)
+        self.p[S.IN_LREF] = np.float32(lambda_ref)
+
+
+class SAbsorber(_Scat):
+    """Pure Beer-Lambert absorber (weight *= exp(-sigma_a L)).  No reference
+    counterpart."""
+    type_code = capi.RAMP_S_ABSORBER
+
+    def __init__(self, sigma_a=0.0, idx=0, ctx=None, **kwargs):
+        super().__init__()
+        self.idx = np.uint32(idx)
+        self.p[S.AB_SIGMA_A] = np.float32(sigma_a)
+
+
+class SDetectorTubes(_Hist):
+    """Position-sensitive tube bank: n_tubes along the scattering angle
+    (degrees, atan2(x, z) of the hit), n_pixels along y, n_tof TOF bins
+    (microseconds).  data() returns the (n_tubes, n_pixels, n_tof) weight
+    and weight^2 arrays.  No reference counterpart."""
+    type_code = capi.RAMP_S_DETPIX
+
+    def __init__(self, theta_range=(-40.0, 140.0), y_range=(-2.0, 2.0), tof_range=(10000.0, 60000.0),
+                 n_tubes=768, n_pixels=128, n_tof=1000, restore_neutron=False, idx=0, ctx=None,
+                 filename=None, **kwargs):
+        super().__init__()
+        self.idx = np.uint32(idx)
+        self.shape = (int(n_tubes), int(n_pixels), int(n_tof))
+        self.n_bins = int(n_tubes) * int(n_pixels) * int(n_tof)
+        self.filename = filename
+        self.p[S.DT_THMIN], self.p[S.DT_THMAX] = np.float32(theta_range[0]), np.float32(theta_range[1])
+        self.p[S.DT_YMIN], self.p[S.DT_YMAX] = np.float32(y_range[0]), np.float32(y_range[1])
+        self.p[S.DT_TMIN], self.p[S.DT_TMAX] = np.float32(tof_range[0]), np.float32(tof_range[1])
+        self.ni[S.DTI_NTUBE], self.ni[S.DTI_NPIX], self.ni[S.DTI_NTOF] = self.shape
+        self.ni[S.DTI_RESTORE] = 1 if restore_neutron else 0
+
+    def data(self, queue=None):
+        h, e = self._moments()
+        return (h.reshape(self.shape), e.reshape(self.shape))
+
+    def save(self, queue=None):
+        if self.filename:
+            h, e = self.data()
+            np.save(self.filename + "Z", h)
+            np.save(self.filename + "E", e)
+
+
+class _OutOfScope:
+    reason = ""
+
+    def __init__(self, *a, **k):
+        raise NotImplementedError(f"{type(self).__name__}: {self.reason}")
+
+
+class SPolariser(_OutOfScope):
+    reason = "polarisation components are not part of the MI355X neutron path"
+
+
+class SAnalyser(_OutOfScope):
+    reason = "polarisation components are not part of the MI355X neutron path"
+
+
+class SCollimatorLin(_OutOfScope):
+    reason = "the reference kernel does not compile (collimator_lin.cl:30)"
+
+
+class SRescal(_OutOfScope):
+    reason = "host-side resolution event list, not part of the MI355X neutron path"
+
+
+class SMCPLIn(_OutOfScope):
+    reason = "MCPL particle-file input is not implemented"
+
+
+class SMCPLOut(_OutOfScope):
+    reason = "MCPL particle-file output is not implemented"
+
+
+SOURCES = ("MFlat", "MCont", "MISIS", "MLambda", "MParallel")
+GEOMS = ("GArm", "GPlane", "GBox", "GSphere", "GSphereInt", "GCylinderExt", "GBanana", "GMesh")
+SCATS = ("SArm", "SBeamstop", "SChopper", "SGuide", "SIsotropic", "SPowder1", "SPowderN",
+         "SMonochromator", "SDetector1D", "SDetector2D", "SCounter", "SIncoherent", "SAbsorber",
+         "SDetectorTubes")

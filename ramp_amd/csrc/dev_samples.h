@@ -1,0 +1,378 @@
+/* dev_samples.h - sample and crystal components (isotropic S(Q,w), powder,
+ * mosaic monochromator) of the fused segment kernels.  Included by
+ * dev_components.h. */
+#ifndef RAMP_DEV_SAMPLES_H
+#define RAMP_DEV_SAMPLES_H
+
+/* sample_distn (scat/isotropic.cl:5-73) over one packed channel table
+ * [q(nq) | w(nw) | pw_cdf(nw) | pq_cdf(nw*nq)] */
+DEV void iso_sample(NS &n, const float *base, uint32_t nq, uint32_t nw, float &omega, float &Q)
+{
+    const float *q = base, *w = base + nq, *pw_cdf = base + nq + nw, *pq_cdf = base + nq + 2 * nw;
+    float u = drw(n), v = drw(n);
+    float mindiff = 1.0f;
+    uint32_t w_index = 0;
+    omega = 0.0f;
+    Q = 0.0f;
+    for (uint32_t i = 0; i < nw; i++) {
+        if (pw_cdf[i] == 1.0f) break;
+        float dd = rm_fabs(pw_cdf[i] - v);
+        if (dd < mindiff) {
+            mindiff = dd;
+            float wprev = (i > 0 ? w[i - 1] : 0.0f);
+            omega = wprev + drw(n) * (w[i] - wprev);
+            w_index = i;
+        }
+    }
+    mindiff = 1.0f;
+    for (uint32_t i = 0; i < nq; i++) {
+        float cdf = pq_cdf[w_index * nq + i];
+        if (cdf == 1.0f) break;
+        float dd = rm_fabs(cdf - u);
+        if (dd < mindiff) {
+            mindiff = dd;
+            float qprev = (i > 0 ? q[i - 1] : 0.0f);
+            Q = qprev + drw(n) * (q[i] - qprev);
+        }
+    }
+}
+
+/* scat/isotropic.cl:75-323.  d[0]/d[1]: 2*pi split; d[2]: kB as float
+ * (detailed balance is an FP32 approximation of the reference's double
+ * exp; exact for elastic channels, omega = 0) */
+template <bool FULL>
+DEV void s_isotropic(NS &n, const DComp &c)
+{
+    const DPart &s = c.s;
+    rf3 path = sample_path(n);
+    float vlen = v3_len(nvel(n));
+    float ki = RC_V2K * vlen;
+    rf3 normvel = v3_norm(nvel(n));
+    float cs = s.p[IS_SSCAT(0)], is_ = s.p[IS_SSCAT(1)], ms = s.p[IS_SSCAT(2)];
+    rdf twok2 = df_prod(2.0f * ki, ki);
+    float sigma_s = df_rdiv((cs + is_) + ms, twok2);
+    rdf sa = df_prod((s.p[IS_SABS(0)] + s.p[IS_SABS(1)]) + s.p[IS_SABS(2)], 2200.0f);
+    float sigma_a = df_div(sa, vlen);
+    float sigma_tot = sigma_a + sigma_s;
+    float mu = (((s.p[IS_RHO(0)] + s.p[IS_RHO(1)]) + s.p[IS_RHO(2)]) * sigma_tot) * 100.0f;
+    float plen = v3_len(path);
+    float p_trans = rm_exp(-mu * plen);
+    if (drw(n) < p_trans) {
+        n.w *= p_trans;
+        setpos(n, v3_add(ipos2(n), v3_scale(0.0001f, v3_norm(path))));
+        n.t += n.is[7];
+        if (s.ni[ISI_TRANSMIT] == 0) n.dead = true;
+        n.sc = (float)c.comp_idx;
+        n.iidx = 0;
+        isect_reset(n);
+        return;
+    }
+    n.w *= (1 - p_trans) * sigma_s / sigma_tot;
+    float eta = drw(n);
+    float lg = rm_log(1 - eta * (1 - rm_exp(-mu * plen)));
+    /* (-1./mu)*lg in double */
+    float inv = 1.0f / mu;
+    rdf im; im.hi = inv; im.lo = rm_fma(-inv, mu, 1.0f) / mu;
+    float path_length = -df_mul_f(im, lg);
+    float u = drw(n);
+    u *= sigma_s;
+    uint32_t flag = 0;
+    /* thresholds cs/(2ki^2) and (cs+is)/(2ki^2) in double */
+    float q1h = cs / twok2.hi;
+    rdf th1; th1.hi = q1h; th1.lo = (rm_fma(-q1h, twok2.hi, cs) - q1h * twok2.lo) / twok2.hi;
+    float csis = cs + is_;
+    float q2h = csis / twok2.hi;
+    rdf th2; th2.hi = q2h; th2.lo = (rm_fma(-q2h, twok2.hi, csis) - q2h * twok2.lo) / twok2.hi;
+    if (df_lt(u, th1)) flag = 1;
+    else if (df_gt(u, th1) && df_lt(u, th2)) flag = 2;
+    else if (df_gt(u, th2)) flag = 3;
+    float omega = 0.0f, Q = 0.0f;
+    if (flag >= 1) {
+        int ch = (int)flag - 1;
+        iso_sample(n, s.tab[ch], s.ni[ISI_NQ(ch)], s.ni[ISI_NW(ch)], omega, Q);
+    }
+    if (ki * ki < (RC_E2KS * omega)) {
+        n.dead = true;
+        n.iidx = 0;
+        isect_reset(n);
+        return;
+    }
+    float kf;
+    float db = 0.5f * rm_exp(-omega / (s.d[2] * s.p[IS_TEMP]));
+    if (drw(n) > db) {
+        kf = rm_sqrt(ki * ki - RC_E2KS * omega);
+        n.se = omega;
+    } else {
+        kf = rm_sqrt(ki * ki + RC_E2KS * omega);
+        n.se = -omega;
+    }
+    float arg = (ki * ki + kf * kf - Q * Q) / (2 * ki * kf);
+    if (rm_fabs(arg) > 1) {
+        n.dead = true;
+        n.iidx = 0;
+        isect_reset(n);
+        return;
+    }
+    float theta = rm_acos(arg);
+    rf3 oldvel = nvel(n);
+    float z = -(n.vx + n.vy) / (n.vz);
+    rf3 perp = v3_norm(rf3_make(1.0f, 1.0f, z));
+    rdf tp; tp.hi = s.d[0]; tp.lo = s.d[1];
+    float alpha = df_mulc(drw(n), tp);
+    perp = rot_about_axis(alpha, normvel, perp);
+    normvel = rot_about_axis(theta, perp, normvel);
+    float TOF = path_length / vlen;
+    n.sd = Q;
+    if (v3_eq(ipos1(n), ipos2(n))) {
+        setpos(n, v3_scale(path_length, path));
+        n.t += TOF;
+    } else {
+        setpos(n, v3_add(ipos1(n), v3_scale(path_length, path)));
+        n.t += n.is[3] + TOF;
+    }
+    setvel(n, v3_mulv(v3_mulv(normvel, kf), RC_K2V));
+    if (FULL) {
+        rf3 q_norm = v3_norm(v3_mulv(v3_sub(oldvel, nvel(n)), RC_V2K));
+        if (flag == 2) {
+            n.px = n.px * (-1.0f / 3.0f);
+            n.py = n.py * (-1.0f / 3.0f);
+            n.pz = n.pz * (-1.0f / 3.0f);
+        } else if (flag == 3) {
+            float dp = v3_dot(rf3_make(n.px, n.py, n.pz), q_norm);
+            n.px = -q_norm.x * dp;
+            n.py = -q_norm.y * dp;
+            n.pz = -q_norm.z * dp;
+        }
+    }
+    n.sc = (float)c.comp_idx;
+    n.iidx = 0;
+    isect_reset(n);
+}
+
+/* azimuth of the Debye-Scherrer cone (powder1.cl:68-79, powderN.cl:117-130):
+ * d[0]/d[1] span = d_phi*M_PI/180.0f, d[2]/d[3] M_PI, d[4]/d[5] M_PI/2.0f,
+ * d[6]/d[7] 2.0f*M_PI (double constants, split) */
+DEV float powder_phi(NS &n, const DPart &s, float d_phi)
+{
+    if (d_phi != 0.0f) {
+        rdf span; span.hi = s.d[0]; span.lo = s.d[1];
+        rdf pi; pi.hi = s.d[2]; pi.lo = s.d[3];
+        rdf pio2; pio2.hi = s.d[4]; pio2.lo = s.d[5];
+        float ph = df_mulc(2.0f * drw(n) - 1.0f, span);
+        if (drw(n) > 0.5f) ph = df_addc(ph, pi);
+        return df_addc(ph, pio2);
+    }
+    rdf tp; tp.hi = s.d[6]; tp.lo = s.d[7];
+    return df_mulc(drw(n), tp);
+}
+
+DEV void powder_dirs(rf3 vel, rf3 &para, rf3 &perp)
+{
+    para = v3_norm(vel);
+    perp = v3_cross(para, rf3_make(0.0f, 1.0f, 0.0f));
+    if (v3_len(perp) < 1e-3f) perp = v3_norm(v3_cross(para, rf3_make(1.0f, 0.0f, 0.0f)));
+    else perp = v3_norm(perp);
+}
+
+/* sigma_abs_v * 2200.0 / vel with a double constant (powder1.cl:37) */
+DEV float powder_sabs(float sav, float vel) { return df_div(df_prod(sav, 2200.0f), vel); }
+
+/* scat/powder1.cl:5-101 */
+DEV void s_powder1(NS &n, const DComp &c)
+{
+    const DPart &s = c.s;
+    n.t += n.is[3];
+    rf3 path = v3_sub(ipos2(n), ipos1(n));
+    float full = v3_len(path);
+    float vel = v3_len(nvel(n));
+    float sigma_scat = s.p[P1_SSCAT_V2] / (vel * vel * RC_V2K * RC_V2K);
+    float sigma_abs = powder_sabs(s.p[P1_SABS_V], vel);
+    float p_scat = sigma_scat / (sigma_abs + sigma_scat);
+    float p_inter = 1 - rm_exp(-(sigma_scat + sigma_abs) * full);
+    float arg = s.p[P1_Q] * RC_K2V / (2.0f * vel);
+    if (arg > 1.0f) {
+        s_done(n, c.comp_idx);
+        return;
+    }
+    float twotheta = 2.0f * rm_asin(arg);
+    n.w *= p_scat;
+    float pen = -rm_log(1.0f - drw(n) * (1.0f - p_inter)) / (sigma_abs + sigma_scat);
+    rf3 para, perp;
+    powder_dirs(nvel(n), para, perp);
+    float phi = powder_phi(n, s, s.p[P1_DPHI]);
+    perp = rot_about_axis(phi, para, perp);
+    para = rot_about_axis(twotheta, perp, para);
+    setpos(n, v3_add(ipos1(n), v3_scale(pen, path)));
+    setvel(n, v3_scale(vel, v3_norm(para)));
+    n.t += n.is[7] - n.is[3];
+    n.w *= full * sigma_scat * rm_exp(-(sigma_scat + sigma_abs) * full);
+    s_done(n, c.comp_idx);
+}
+
+/* scat/powderN.cl:5-157 (line scan bounded by the line count) */
+DEV void s_powderN(NS &n, const DComp &c)
+{
+    const DPart &s = c.s;
+    const float *ssv2 = s.tab[PNT_SSCAT_V2], *qs = s.tab[PNT_Q];
+    uint32_t nl = s.ni[PNI_NLINES];
+    rf3 path = sample_path(n);
+    float full = v3_len(path);
+    float vel = v3_len(nvel(n));
+    float sigma_abs = powder_sabs(s.p[PN_SABS_V], vel);
+    float vv = vel * vel * RC_V2K * RC_V2K;
+    float sum = 0.0f;
+    for (uint32_t i = 0; i < nl; i++) sum += ssv2[i] / vv;
+    float p_inter = 1 - rm_exp(-(sum + sigma_abs) * full);
+    if (drw(n) > p_inter) {
+        n.w *= 1 - p_inter;
+        setpos(n, v3_add(ipos2(n), v3_scale(0.0001f, v3_norm(path))));
+        n.t += n.is[7];
+        if (s.ni[PNI_TRANSMIT] == 0) n.dead = true;
+        n.sc = (float)c.comp_idx;
+        n.iidx = 0;
+        isect_reset(n);
+        return;
+    }
+    uint32_t attempts = 0;
+    float q_chosen = 0.0f, arg;
+    while (true) {
+        float u = drw(n), v = 0.0f;
+        uint32_t i = 0;
+        while (v < u) {
+            if (i >= nl) break;
+            float sg = ssv2[i] / vv;
+            q_chosen = qs[i];
+            v += sg / sum;
+            i++;
+        }
+        arg = q_chosen * RC_K2V / (2.0f * vel);
+        if (arg < 1.0f) break;
+        else if (arg > 1.0f && attempts > 2 * nl) {
+            n.dead = true;
+            n.iidx = 0;
+            isect_reset(n);
+            return;
+        }
+        attempts++;
+    }
+    float twotheta = 2.0f * rm_asin(arg);
+    float pen = -rm_log(1.0f - drw(n) * (1.0f - p_inter)) / (sigma_abs + sum);
+    rf3 para, perp;
+    powder_dirs(nvel(n), para, perp);
+    float phi = powder_phi(n, s, s.p[PN_DPHI]);
+    perp = rot_about_axis(phi, para, perp);
+    para = rot_about_axis(twotheta, perp, para);
+    if (v3_eq(ipos1(n), ipos2(n))) {
+        rf3 dp = v3_scale(pen, path);
+        setpos(n, v3_add(npos(n), dp));
+        n.t += v3_len(dp) / v3_len(nvel(n));
+    } else {
+        setpos(n, v3_add(ipos1(n), v3_scale(pen, path)));
+        n.t += n.is[7] - n.is[3];
+    }
+    setvel(n, v3_scale(vel, v3_norm(para)));
+    n.w *= (1 - rm_exp(-(sum + sigma_abs) * pen)) * sum / (sum + sigma_abs);
+    s_done(n, c.comp_idx);
+}
+
+/* GAUSS of monochromator.cl:5-8 as an FP32 approximation; d[8] holds
+ * sqrt(2*M_PI) rounded to float */
+DEV float mono_gauss(float x, float rms, float sq2pi)
+{
+    return rm_exp(-(x) * (x) / (2.0f * (rms) * (rms))) / (sq2pi * rms);
+}
+
+/* scat/monochromator.cl:22-197.  d[0] mos_rms_y, d[1] mos_rms_z,
+ * d[2] mos_rms_max, d[3] Q, d[4] zmax, d[5] ymax, d[8] sqrt(2 pi),
+ * d[9]/d[10] 2*pi split.  Box-Muller's cos and the GAUSS weights are FP32
+ * approximations of the reference's double evaluation. */
+DEV void s_mono(NS &n, const DComp &c)
+{
+    const DPart &s = c.s;
+    const float *gx = s.tab[MOT_GX], *gw = s.tab[MOT_GW];
+    uint32_t nh = s.ni[MOI_NH], nv = s.ni[MOI_NV], glen = s.ni[MOI_GLEN];
+    setpos(n, ipos2(n));
+    n.t += n.is[7];
+    float mry = s.d[0], mrz = s.d[1], mrmax = s.d[2], Q = s.d[3];
+    float sw = s.p[MO_SLAB_W], sh = s.p[MO_SLAB_H], gap = s.p[MO_GAP];
+    float zmax = s.d[4], zmin = -zmax, ymax = s.d[5], ymin = -ymax;
+    if (n.z > zmin && n.z < zmax && n.y > ymin && n.y < ymax) {
+        float col = rm_ceil((n.z - zmin) / (sw + gap));
+        float row = rm_ceil((n.y - ymin) / (sh + gap));
+        float tilt_h = s.p[MO_RAD_H] > 0.0f
+                           ? rm_asin(((float)col - (float)(nh + 1) / 2.0f) * (sw + gap) / s.p[MO_RAD_H])
+                           : 0.0f;
+        float tilt_v = s.p[MO_RAD_V] > 0.0f
+                           ? -rm_asin(((float)row - (float)(nv + 1) / 2.0f) * (sh + gap) / s.p[MO_RAD_V])
+                           : 0.0f;
+        float cz = zmin + (col - 0.5f) * (sw + gap) - gap / 2.0f;
+        float cy = ymin + (row - 0.5f) * (sh + gap) - gap / 2.0f;
+        rmat3 SR = rmat3_euler(0.0f, tilt_h, tilt_v);
+        setvel(n, rmat3_derotate(&SR, nvel(n)));
+        setpos(n, rf3_make(n.x - 0.0f, n.y - cy, n.z - cz));
+        setpos(n, rmat3_derotate(&SR, npos(n)));
+        if (rm_fabs(n.z) <= sw / 2.0f && rm_fabs(n.y) <= sh / 2.0f) {
+            rf3 ki = v3_scale(RC_V2K, nvel(n));
+            float ratio = -2.0f * ki.x / Q;
+            float Q_order = rm_floor(ratio + 0.5f);
+            if (Q_order == 0.0f) Q_order = ratio < 0.0f ? -1.0f : 1.0f;
+            if (Q_order < 0.0f) Q_order = -Q_order;
+            rf3 ku = v3_norm(ki);
+            float kl = v3_len(ki);
+            if (Q_order > 2.0f * kl / Q) Q_order -= 1.0f;
+            float q0 = Q_order * Q;
+            float q0x = ratio < 0.0f ? -q0 : q0;
+            float theta = rm_asin(q0 / (2.0f * kl));
+            float delta = rm_asin(rm_fabs(ku.x)) - theta;
+            float l12 = v2_len(rf2_make(ki.y, ki.z));
+            float p_reflect = rm_fabs(s.p[MO_R0]) *
+                              rm_exp(-ki.z * ki.z / (l12 * l12) * delta * delta / (2.0f * mry * mry)) *
+                              rm_exp(-ki.y * ki.y / (l12 * l12) * delta * delta / (2.0f * mrz * mrz));
+            if (drw(n) <= p_reflect) {
+                float cos_2theta = rm_cos(2.0f * theta);
+                float k_sin_2theta = kl * rm_sin(2.0f * theta);
+                rf3 b = v3_mulv(v3_norm(v3_cross(ki, rf3_make(q0x, 0.0f, 0.0f))), k_sin_2theta);
+                rf3 a = v3_cross(b, ku);
+                float total = 0.0f, w = 0.0f;
+                float mos_sample = mrmax / rm_cos(theta);
+                rf3 c1 = v3_mulv(ki, (cos_2theta - 1.0f));
+                rf3 qv;
+                for (uint32_t i = 0; i < 100; i++) {
+                    w = 5.0f * mos_sample;
+                    float cphi = rm_cos(w), sphi = rm_sin(w);
+                    qv = v3_add(v3_add(c1, v3_scale(cphi, a)), v3_scale(sphi, b));
+                    qv.y = qv.y / mrz;
+                    qv.z = qv.z / mry;
+                    if (qv.z * qv.z + qv.y * qv.y < (25.0f / (2.0f / 3.0f)) * (qv.x * qv.x)) break;
+                    mos_sample *= (2.0f / 3.0f);
+                }
+                float sq2pi = s.d[8];
+                for (uint32_t i = 0; i < glen; i++) {
+                    float phi = w * gx[i];
+                    float cphi = rm_cos(phi), sphi = rm_sin(phi);
+                    qv = v3_add(v3_add(c1, v3_scale(cphi, a)), v3_scale(sphi, b));
+                    float pr = mono_gauss(qv.z / qv.x, mry, sq2pi) * mono_gauss(qv.y / qv.x, mrz, sq2pi);
+                    total += gw[i] * pr;
+                }
+                total *= w;
+                float u1 = drw(n), u2 = drw(n);
+                rdf tp; tp.hi = s.d[9]; tp.lo = s.d[10];
+                float rn = rm_sqrt(-2.0f * rm_log(u1)) * rm_cos(df_mulc(u2, tp));
+                float phi = mos_sample * rn;
+                float cphi = rm_cos(phi), sphi = rm_sin(phi);
+                qv = v3_add(v3_add(c1, v3_scale(cphi, a)), v3_scale(sphi, b));
+                p_reflect = mono_gauss(qv.z / qv.x, mry, sq2pi) * mono_gauss(qv.y / qv.x, mrz, sq2pi);
+                setvel(n, v3_scale(RC_K2V, v3_add(ki, qv)));
+                p_reflect = p_reflect / (total * mono_gauss(phi, mos_sample, sq2pi));
+                if (p_reflect > 1.0f) p_reflect = 1.0f;
+                n.w *= p_reflect;
+            }
+        }
+        setpos(n, rmat3_rotate(&SR, npos(n)));
+        setpos(n, rf3_make(n.x + 0.0f, n.y + cy, n.z + cz));
+        setvel(n, rmat3_rotate(&SR, nvel(n)));
+    }
+    s_done(n, c.comp_idx);
+}
+
+#endif

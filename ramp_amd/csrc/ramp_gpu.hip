@@ -1,0 +1,1193 @@
+/* ramp_gpu.hip - libramp.so: fused, stream-compacted HIP segment kernels
+ * for the RAMP neutron path on gfx950, and the C-ABI of include/ramp.h.
+ *
+ * Execution model
+ *   The instrument is a list of steps (one per linear-block component, one
+ *   per non-linear block).  A run splits every chunk's histories over a
+ *   few segment launches.  Segment 0 generates neutrons from the source
+ *   (the Threefry key is the chunk-local index) and carries each through
+ *   its steps in registers; at the end of a segment the surviving
+ *   neutrons are appended to a structure-of-arrays queue with one
+ *   wave-wide ballot and one atomic per wave, so the next segment runs
+ *   dense waves.  Segment boundaries are placed where a pilot run shows
+ *   the live fraction dropping (after choppers, slits).  Detector events
+ *   are float atomics (reference-like, order dependent) or int64 fixed
+ *   point atomics (order independent, bit-reproducible).
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ramp.h"
+#include "dev_components.h"
+
+/* ======================================================================== */
+/* error handling                                                           */
+/* ======================================================================== */
+static thread_local std::string g_err;
+
+static int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+static int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(x)                                                                     \
+    do {                                                                              \
+        hipError_t e_ = (x);                                                          \
+        if (e_ != hipSuccess)                                                         \
+            return fail(RAMP_E_DEVICE, "%s failed: %s", #x, hipGetErrorString(e_));   \
+    } while (0)
+
+extern "C" const char *ramp_last_error(void) { return g_err.c_str(); }
+extern "C" const char *ramp_version(void) { return "ramp_amd 0.1.0 gfx950"; }
+
+/* ======================================================================== */
+/* segment kernel                                                           */
+/* ======================================================================== */
+struct SegArgs {
+    const DComp *__restrict__ comps;
+    const DStep *__restrict__ steps;
+    const DPart *__restrict__ src;
+    const DDet *__restrict__ dets;
+    unsigned long long *cnt;
+    int s0, s1;
+    int last;
+    int pilot;
+    int hist_mode;
+    int carry;
+    int keep;
+    int num_sim;
+    int seed;
+    uint64_t n;                 /* items of segment 0 */
+    const uint32_t *in_count;   /* items of later segments */
+    uint32_t *out_count;
+    DQueue in, out;
+    float *slot_ctr;
+    uint32_t *slot_iidx;
+    float *st16, *st8;
+    uint32_t *stii;
+};
+
+template <bool FULL>
+DEV void run_steps(NS &n, const SegArgs &a, const HistCtx &hc)
+{
+    for (int k = a.s0; k < a.s1; k++) {
+        if (!a.pilot && !__any(!n.dead)) break;
+        const DStep st = a.steps[k];
+        if (st.kind == 0) {
+            const DComp &c = a.comps[st.first];
+            tr_in<FULL>(n, c);
+            geom_dispatch(n, c);
+            terminate(n, c.restore);
+            scat_dispatch<FULL>(n, c, hc);
+            tr_out<FULL>(n, c);
+        } else {
+            for (uint32_t ev = 0; ev < st.multi; ev++) {
+                for (uint32_t j = 0; j < st.count; j++) {
+                    const DComp &c = a.comps[st.first + j];
+                    tr_in<FULL>(n, c);
+                    geom_dispatch(n, c);
+                    tr_out<FULL>(n, c);
+                }
+                for (uint32_t j = 0; j < st.count; j++) {
+                    const DComp &c = a.comps[st.first + j];
+                    tr_in<FULL>(n, c);
+                    scat_dispatch<FULL>(n, c, hc);
+                    tr_out<FULL>(n, c);
+                }
+            }
+        }
+        if (a.pilot) {
+            unsigned long long m = __ballot(!n.dead);
+            if (__lane_id() == 0 && m) atomicAdd(&a.cnt[CNT_NSTEP0 + k], (unsigned long long)__popcll(m));
+        }
+    }
+}
+
+template <bool FULL>
+DEV void finalize(const NS &n, const SegArgs &a)
+{
+    if (a.carry) {
+        a.slot_ctr[n.key] = n.ctr;
+        a.slot_iidx[n.key] = n.iidx;
+    }
+    if (FULL && a.keep) {
+        float *o = a.st16 + 16 * (size_t)n.key;
+        o[0] = n.x; o[1] = n.y; o[2] = n.z; o[3] = n.vx; o[4] = n.vy; o[5] = n.vz;
+        o[6] = n.px; o[7] = n.py; o[8] = n.pz; o[9] = n.w; o[10] = n.t; o[11] = n.ctr;
+        o[12] = n.sc; o[13] = n.sd; o[14] = n.se; o[15] = n.dead ? 1.0f : 0.0f;
+        float *q = a.st8 + 8 * (size_t)n.key;
+        for (int i = 0; i < 8; i++) q[i] = n.is[i];
+        a.stii[n.key] = n.iidx;
+    }
+}
+
+/* status word of the queue: iidx in bits 0..23, bit 24: intersection is all
+ * zeros (GBox miss with restore) instead of the sentinel */
+#define ST_ZERO_ISECT (1u << 24)
+
+template <bool FULL>
+DEV void queue_load(NS &n, const DQueue &q, uint64_t i)
+{
+    n.x = q.x[i]; n.y = q.y[i]; n.z = q.z[i];
+    n.vx = q.vx[i]; n.vy = q.vy[i]; n.vz = q.vz[i];
+    n.w = q.w[i]; n.t = q.t[i]; n.ctr = q.ctr[i];
+    n.key = q.key[i];
+    uint32_t st = q.status[i];
+    n.iidx = st & 0x00ffffffu;
+    n.dead = false;
+    if (st & ST_ZERO_ISECT) {
+        for (int k = 0; k < 8; k++) n.is[k] = 0.0f;
+    } else {
+        isect_reset(n);
+    }
+    if (FULL) {
+        n.px = q.px[i]; n.py = q.py[i]; n.pz = q.pz[i];
+        n.sc = q.sc[i]; n.sd = q.sd[i]; n.se = q.se[i];
+    } else {
+        n.px = n.py = n.pz = n.sc = n.sd = n.se = 0.0f;
+    }
+}
+
+template <bool FULL>
+DEV void queue_store(const NS &n, const DQueue &q, uint64_t i)
+{
+    q.x[i] = n.x; q.y[i] = n.y; q.z[i] = n.z;
+    q.vx[i] = n.vx; q.vy[i] = n.vy; q.vz[i] = n.vz;
+    q.w[i] = n.w; q.t[i] = n.t; q.ctr[i] = n.ctr;
+    q.key[i] = n.key;
+    bool zero = (n.is[3] == 0.0f && n.is[7] == 0.0f);
+    q.status[i] = (n.iidx & 0x00ffffffu) | (zero ? ST_ZERO_ISECT : 0u);
+    if (FULL) {
+        q.px[i] = n.px; q.py[i] = n.py; q.pz[i] = n.pz;
+        q.sc[i] = n.sc; q.sd[i] = n.sd; q.se[i] = n.se;
+    }
+}
+
+template <bool FULL, bool FIRST>
+__global__ __launch_bounds__(256) void k_segment(SegArgs a)
+{
+    HistCtx hc;
+    hc.dets = a.dets;
+    hc.cnt = a.cnt;
+    hc.mode = a.hist_mode;
+    hc.enabled = !a.pilot;
+    uint64_t n_items = FIRST ? a.n : (uint64_t)(*a.in_count);
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n_items; base += stride) {
+        uint64_t i = base + threadIdx.x;
+        bool valid = i < n_items;
+        NS n;
+        if (valid) {
+            if (FIRST) {
+                float sb_old = a.carry ? a.slot_ctr[i] : 0.0f;
+                n.iidx = a.carry ? a.slot_iidx[i] : 0u;
+                n.key = (uint32_t)i;
+                src_generate<FULL>(n, *a.src, nullptr, a.num_sim, a.seed, sb_old);
+            } else {
+                queue_load<FULL>(n, a.in, i);
+            }
+        } else {
+            n.dead = true;
+            n.key = 0;
+            n.iidx = 0;
+        }
+        run_steps<FULL>(n, a, hc);
+        if (a.pilot) continue;
+        bool live = valid && !n.dead && !a.last;
+        if (!a.last) {
+            unsigned long long m = __ballot(live);
+            uint32_t cnt = (uint32_t)__popcll(m);
+            uint32_t off = 0;
+            if (__lane_id() == 0 && cnt) off = atomicAdd(a.out_count, cnt);
+            off = __shfl(off, 0);
+            if (live) {
+                unsigned long long below = m & ((1ull << __lane_id()) - 1ull);
+                queue_store<FULL>(n, a.out, (uint64_t)off + (uint64_t)__popcll(below));
+            }
+        }
+        if (valid && !live) finalize<FULL>(n, a);
+    }
+}
+
+/* ======================================================================== */
+/* small kernels                                                            */
+/* ======================================================================== */
+__global__ void k_fill_u64(unsigned long long *p, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0ull;
+}
+
+__global__ void k_fill_f32(float *p, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0.0f;
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+__device__ static float math_eval(int fn, float x, float y)
+{
+    switch (fn) {
+    case RAMP_FN_SIN: return rm_sin(x);
+    case RAMP_FN_COS: return rm_cos(x);
+    case RAMP_FN_EXP: return rm_exp(x);
+    case RAMP_FN_LOG: return rm_log(x);
+    case RAMP_FN_POW: return rm_pow(x, y);
+    case RAMP_FN_TANH: return rm_tanh(x);
+    case RAMP_FN_ATAN2: return rm_atan2(x, y);
+    case RAMP_FN_ASIN: return rm_asin(x);
+    case RAMP_FN_ACOS: return rm_acos(x);
+    case RAMP_FN_FMOD: return rm_fmod(x, y);
+    case RAMP_FN_ROUND: return rm_round(x);
+    case RAMP_FN_SQRT: return rm_sqrt(x);
+    case RAMP_FN_DIV: return x / y;
+    default: return 0.0f;
+    }
+}
+
+static float math_eval_host(int fn, float x, float y)
+{
+    switch (fn) {
+    case RAMP_FN_SIN: return rm_sin(x);
+    case RAMP_FN_COS: return rm_cos(x);
+    case RAMP_FN_EXP: return rm_exp(x);
+    case RAMP_FN_LOG: return rm_log(x);
+    case RAMP_FN_POW: return rm_pow(x, y);
+    case RAMP_FN_TANH: return rm_tanh(x);
+    case RAMP_FN_ATAN2: return rm_atan2(x, y);
+    case RAMP_FN_ASIN: return rm_asin(x);
+    case RAMP_FN_ACOS: return rm_acos(x);
+    case RAMP_FN_FMOD: return rm_fmod(x, y);
+    case RAMP_FN_ROUND: return rm_round(x);
+    case RAMP_FN_SQRT: return rm_sqrt(x);
+    case RAMP_FN_DIV: return x / y;
+    default: return 0.0f;
+    }
+}
+
+__global__ void k_math(int fn, const float *x, const float *y, float *out, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        out[i] = math_eval(fn, x[i], y ? y[i] : 0.0f);
+}
+
+__global__ void k_rng(const uint32_t *tid, const float *ctr, float *out, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        float c = ctr[i];
+        out[i] = rr_draw(&c, tid[i]);
+    }
+}
+
+/* ======================================================================== */
+/* host: context and plan                                                   */
+/* ======================================================================== */
+struct ramp_ctx {
+    int device;
+    int n_cu;
+    hipStream_t stream;
+    hipEvent_t ev[4];
+};
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+static int dev_alloc(DevBuf &b, size_t bytes)
+{
+    if (b.bytes >= bytes && b.p) return 0;
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+    if (bytes == 0) return 0;
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) return fail(RAMP_E_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    b.bytes = bytes;
+    return 0;
+}
+
+static void dev_free(DevBuf &b)
+{
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
+struct QueueBufs {
+    DevBuf f[15];   /* x y z vx vy vz w t ctr px py pz sc sd se */
+    DevBuf key, status;
+};
+
+struct ramp_plan {
+    ramp_ctx *ctx = nullptr;
+    std::vector<DComp> hcomps;
+    std::vector<DStep> hsteps;
+    DPart hsrc;
+    std::vector<DDet> hdets;
+    std::vector<ramp_det_desc> det_desc;
+    DevBuf comps, steps, src, dets, tables, cnt, count;
+    std::vector<DevBuf> hw, hw2, hq, hq2;
+    std::vector<DevBuf> mesh_nodes, mesh_order;
+    QueueBufs q[2];
+    uint64_t qcap = 0;
+    bool qfull = false;
+    DevBuf slot_ctr, slot_iidx, st16, st8, stii;
+    uint64_t st_n = 0;
+    bool det_scale_set = false;
+    bool has_atomic = false, has_det = false;
+    std::vector<int> bounds;  /* segment boundaries (step indices) */
+    bool bounds_set = false;
+    ramp_run_stats stats;
+};
+
+extern "C" int ramp_ctx_create(int device, ramp_ctx **out)
+{
+    if (!out) return fail(RAMP_E_INVALID, "null output pointer");
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(RAMP_E_INVALID, "device %d of %d", device, ndev);
+    HIPCHK(hipSetDevice(device));
+    ramp_ctx *c = new ramp_ctx();
+    c->device = device;
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    c->n_cu = prop.multiProcessorCount;
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (int i = 0; i < 4; i++) HIPCHK(hipEventCreate(&c->ev[i]));
+    *out = c;
+    return 0;
+}
+
+extern "C" int ramp_ctx_destroy(ramp_ctx *ctx)
+{
+    if (!ctx) return 0;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (int i = 0; i < 4; i++) (void)hipEventDestroy(ctx->ev[i]);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return 0;
+}
+
+/* ---- parameter-only expressions of the reference, evaluated once -------- */
+static float maxwell_ref_host(float E, float T)
+{
+    /* cont_mod.cl:4-9 with its double promotion (2.0f*rootE/sqrt(M_PI)) */
+    float rootE = rm_sqrt(E);
+    float overKT = 1.0f / (RC_K_B * T);
+    double r = (double)(2.0f * rootE) / sqrt(RC_M_PI_D);
+    r = r * (double)rm_pow(overKT, (float)(3.0f / 2.0f));
+    r = r * (double)rm_exp(-E * overKT);
+    return (float)r;
+}
+
+static void set_df(float *d, int i, double v)
+{
+    rdf s = df_split(v);
+    d[i] = s.hi;
+    d[i + 1] = s.lo;
+}
+
+/* smallest float threshold T with: refl < T  <=>  (double)refl < 1e-12 */
+static float thresh_lt_double(double c)
+{
+    float f = (float)c;
+    if ((double)f < c) f = nextafterf(f, INFINITY);
+    return f;
+}
+
+static void derive_source(DPart &s)
+{
+    std::memset(s.d, 0, sizeof s.d);
+    set_df(s.d, 30, 2 * RC_M_PI_D);
+    if (s.type == RAMP_SRC_CONT) {
+        float T[3] = {s.p[SP_T1], s.p[SP_T2], s.p[SP_T3]};
+        for (int i = 0; i < 3; i++) {
+            float overKT = 1.0f / (RC_K_B * T[i]);
+            s.d[i] = overKT;
+            s.d[3 + i] = rm_pow(overKT, (float)(3.0f / 2.0f));
+            s.d[6 + i] = maxwell_ref_host(RC_K_B * T[i], T[i]);
+        }
+        float EjJ = s.p[SP_TAIL_JOIN] * (RC_K_B * T[0]);
+        s.d[9] = EjJ;
+        s.d[10] = s.p[SP_TAIL_JOIN] > 0.0f ? maxwell_ref_host(EjJ, T[0]) : 0.0f;
+        set_df(s.d, 11, 1.0 / sqrt(RC_M_PI_D));
+    }
+}
+
+static void derive_geom(DPart &g)
+{
+    std::memset(g.d, 0, sizeof g.d);
+    switch (g.type) {
+    case RAMP_G_PLANE:
+        g.d[0] = g.p[GP_W] / 2.0f;
+        g.d[1] = g.p[GP_H] / 2.0f;
+        break;
+    case RAMP_G_BOX:
+        g.d[0] = -g.p[GP_W] / 2.0f; g.d[1] = -g.p[GP_H] / 2.0f; g.d[2] = -g.p[GP_D] / 2.0f;
+        g.d[3] = g.p[GP_W] / 2.0f; g.d[4] = g.p[GP_H] / 2.0f; g.d[5] = g.p[GP_D] / 2.0f;
+        break;
+    case RAMP_G_BANANA: {
+        float mx = g.p[GP_MAXTH];
+        /* (mintheta < theta2) is 0 or 1, then compared with maxtheta */
+        g.d[0] = (mx > 1.0f) ? 0.0f : (mx <= 0.0f ? 1.0f : 2.0f);
+        break;
+    }
+    default: break;
+    }
+}
+
+static void derive_scat(DPart &s)
+{
+    std::memset(s.d, 0, sizeof s.d);
+    switch (s.type) {
+    case RAMP_S_CHOPPER:
+        s.d[0] = (float)((double)2.0f * RC_M_PI_D / (double)(float)s.ni[CI_NSLITS]);
+        break;
+    case RAMP_S_GUIDE: {
+        float w1 = s.p[GU_W1], h1 = s.p[GU_H1], w2 = s.p[GU_W2], h2 = s.p[GU_H2], l = s.p[GU_L];
+        float ww = (float)(0.5 * (double)(w2 - w1)), hh = (float)(0.5 * (double)(h2 - h1));
+        s.d[0] = ww;
+        s.d[1] = hh;
+        s.d[2] = (float)(0.5 * (double)w1);
+        s.d[3] = (float)(0.5 * (double)h1);
+        float m = s.p[GU_M], W = s.p[GU_W], alpha = s.p[GU_ALPHA], beta = 0.0f;
+        if (W == 0.0 && alpha == 0.0) {
+            m = (float)((double)m * 0.9853 + 0.1978);
+            W = (float)(-0.0002 * (double)m + 0.0022);
+            alpha = (float)(0.2304 * (double)m + 5.0944);
+            beta = (float)(-7.6251 * (double)m + 68.1137);
+            if (m <= 3) { alpha = m; beta = 0; }
+        }
+        s.d[4] = m; s.d[5] = W; s.d[6] = alpha; s.d[7] = beta;
+        s.d[8] = thresh_lt_double(1e-12);
+        s.d[9] = RC_V2K * -2.0f;
+        float nv = l * l + ww * ww, nh = l * l + hh * hh;
+        s.d[10] = rm_sqrt(nv); s.d[11] = rm_sqrt(nh);
+        s.d[12] = nv; s.d[13] = nh;
+        float Qc = s.p[GU_QC];
+        float m2 = m;
+        bool zero = (m <= 0 || Qc <= 0 || s.p[GU_R0] <= 0);
+        if (m2 < 1) { Qc *= m2; m2 = 1; }
+        s.d[14] = Qc; s.d[15] = m2; s.d[16] = zero ? 1.0f : 0.0f;
+        break;
+    }
+    case RAMP_S_ISOTROPIC:
+        set_df(s.d, 0, 2 * RC_M_PI_D);
+        s.d[2] = (float)RC_KB_D;
+        break;
+    case RAMP_S_POWDER1:
+    case RAMP_S_POWDERN: {
+        float d_phi = s.type == RAMP_S_POWDER1 ? s.p[P1_DPHI] : s.p[PN_DPHI];
+        set_df(s.d, 0, (double)d_phi * RC_M_PI_D / 180.0f);
+        set_df(s.d, 2, RC_M_PI_D);
+        set_df(s.d, 4, RC_M_PI_D / 2.0f);
+        set_df(s.d, 6, 2.0f * RC_M_PI_D);
+        break;
+    }
+    case RAMP_S_MONO: {
+        float l8 = rm_sqrt(8.0f * rm_log(2.0f));
+        float mry = (float)(RC_MIN2RAD_D * (double)s.p[MO_MOS_H] / (double)l8);
+        float mrz = (float)(RC_MIN2RAD_D * (double)s.p[MO_MOS_V] / (double)l8);
+        s.d[0] = mry; s.d[1] = mrz; s.d[2] = mry > mrz ? mry : mrz;
+        s.d[3] = (float)((double)2.0f * RC_M_PI_D / (double)s.p[MO_D]);
+        float sw = s.p[MO_SLAB_W], sh = s.p[MO_SLAB_H], gap = s.p[MO_GAP];
+        s.d[4] = (((float)s.ni[MOI_NH] * (sw + gap)) - gap) / 2.0f;
+        s.d[5] = (((float)s.ni[MOI_NV] * (sh + gap)) - gap) / 2.0f;
+        s.d[8] = (float)sqrt(2.0f * RC_M_PI_D);
+        set_df(s.d, 9, 2.0f * RC_M_PI_D);
+        break;
+    }
+    case RAMP_S_DET1D:
+    case RAMP_S_DET2D:
+        set_df(s.d, 0, 2 * RC_M_PI_D);
+        break;
+    default: break;
+    }
+}
+
+/* ---- BVH over the triangles of a mesh (host build; median split) -------- */
+struct BvhBuild {
+    const float *pts;
+    std::vector<BvhNode> nodes;
+    std::vector<uint32_t> order;
+    std::vector<float> cen;
+};
+
+static void tri_bounds(const float *P, float lo[3], float hi[3])
+{
+    for (int a = 0; a < 3; a++) {
+        lo[a] = std::min(P[a], std::min(P[3 + a], P[6 + a]));
+        hi[a] = std::max(P[a], std::max(P[3 + a], P[6 + a]));
+    }
+}
+
+static int bvh_rec(BvhBuild &b, uint32_t first, uint32_t count, int depth)
+{
+    int idx = (int)b.nodes.size();
+    b.nodes.push_back(BvhNode());
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t k = first; k < first + count; k++) {
+        uint32_t t = b.order[k];
+        float tl[3], th[3];
+        tri_bounds(b.pts + 9 * (size_t)t, tl, th);
+        for (int a = 0; a < 3; a++) {
+            lo[a] = std::min(lo[a], tl[a]);
+            hi[a] = std::max(hi[a], th[a]);
+            clo[a] = std::min(clo[a], b.cen[3 * t + a]);
+            chi[a] = std::max(chi[a], b.cen[3 * t + a]);
+        }
+    }
+    BvhNode nd;
+    std::memset(&nd, 0, sizeof nd);
+    for (int a = 0; a < 3; a++) {
+        float ext = hi[a] - lo[a];
+        float mag = std::max(std::fabs(lo[a]), std::fabs(hi[a]));
+        float margin = 1e-4f * ext + 1e-5f * mag + 1e-9f;
+        nd.lo[a] = lo[a] - margin;
+        nd.hi[a] = hi[a] + margin;
+    }
+    if (count <= 4 || depth > 48) {
+        nd.left = -1;
+        nd.right = (int32_t)first;
+        nd.count = (int32_t)count;
+        b.nodes[idx] = nd;
+        return idx;
+    }
+    int ax = 0;
+    for (int a = 1; a < 3; a++)
+        if (chi[a] - clo[a] > chi[ax] - clo[ax]) ax = a;
+    uint32_t mid = first + count / 2;
+    std::nth_element(b.order.begin() + first, b.order.begin() + mid, b.order.begin() + first + count,
+                     [&](uint32_t x, uint32_t y) { return b.cen[3 * x + ax] < b.cen[3 * y + ax]; });
+    int l = bvh_rec(b, first, mid - first, depth + 1);
+    int r = bvh_rec(b, mid, first + count - mid, depth + 1);
+    nd.left = l;
+    nd.right = r;
+    nd.count = 0;
+    b.nodes[idx] = nd;
+    return idx;
+}
+
+static int build_mesh(ramp_plan *p, DPart &g, const float *pts, uint32_t ntri)
+{
+    BvhBuild b;
+    b.pts = pts;
+    b.order.resize(ntri);
+    b.cen.resize(3 * (size_t)ntri);
+    for (uint32_t t = 0; t < ntri; t++) {
+        b.order[t] = t;
+        for (int a = 0; a < 3; a++)
+            b.cen[3 * t + a] = (pts[9 * t + a] + pts[9 * t + 3 + a] + pts[9 * t + 6 + a]) / 3.0f;
+    }
+    if (ntri == 0) {
+        g.aux = nullptr;
+        g.aux2 = nullptr;
+        return 0;
+    }
+    bvh_rec(b, 0, ntri, 0);
+    DevBuf nb, ob;
+    int rc = dev_alloc(nb, b.nodes.size() * sizeof(BvhNode));
+    if (rc) return rc;
+    rc = dev_alloc(ob, b.order.size() * sizeof(uint32_t));
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(nb.p, b.nodes.data(), b.nodes.size() * sizeof(BvhNode), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ob.p, b.order.data(), b.order.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    g.aux = nb.p;
+    g.aux2 = ob.p;
+    p->mesh_nodes.push_back(nb);
+    p->mesh_order.push_back(ob);
+    return 0;
+}
+
+static int copy_part(DPart &d, const ramp_part_desc &s, const float *dtab, uint64_t n_tables)
+{
+    std::memset(&d, 0, sizeof d);
+    d.type = s.type;
+    for (int i = 0; i < 8; i++) d.ni[i] = s.ni[i];
+    for (int i = 0; i < 48; i++) d.p[i] = s.p[i];
+    for (int k = 0; k < RAMP_NT; k++) {
+        if (s.tab_len[k] == 0) continue;
+        if (s.tab_off[k] + s.tab_len[k] > n_tables)
+            return fail(RAMP_E_INVALID, "table %d out of range", k);
+        d.tab[k] = dtab + s.tab_off[k];
+        d.tab_len[k] = s.tab_len[k];
+    }
+    return 0;
+}
+
+static void rotation_matrix(const float rot[3], float R[9])
+{
+    rmat3 M = rmat3_euler(rot[0], rot[1], rot[2]);
+    for (int i = 0; i < 9; i++) R[i] = M.m[i];
+}
+
+static int validate_plan(const ramp_plan_desc *d)
+{
+    if (!d) return fail(RAMP_E_INVALID, "null plan descriptor");
+    if (d->n_blocks == 0 || !d->blocks) return fail(RAMP_E_INVALID, "plan has no blocks");
+    if (d->n_comps && !d->comps) return fail(RAMP_E_INVALID, "null component array");
+    if (d->n_dets && !d->dets) return fail(RAMP_E_INVALID, "null detector array");
+    if (d->source.type < RAMP_SRC_FLAT || d->source.type > RAMP_SRC_PARALLEL)
+        return fail(RAMP_E_INVALID, "unknown source type %u", d->source.type);
+    if (d->source.type == RAMP_SRC_ISIS) {
+        const ramp_part_desc &s = d->source;
+        uint64_t nt = s.ni[SI_ISIS_NT], ne = s.ni[SI_ISIS_NE];
+        if (nt < 2 || ne < 2 || s.tab_len[ST_ISIS_FLUX] < (ne - 1) * nt || s.tab_len[ST_ISIS_TBINS] < nt ||
+            s.tab_len[ST_ISIS_EBINS] < ne || s.tab_len[ST_ISIS_EINT] < ne)
+            return fail(RAMP_E_INVALID, "ISIS source tables inconsistent with their sizes");
+    }
+    for (uint32_t b = 0; b < d->n_blocks; b++) {
+        const ramp_block_desc &bl = d->blocks[b];
+        if ((uint64_t)bl.first + bl.count > d->n_comps)
+            return fail(RAMP_E_INVALID, "block %u out of range", b);
+        if (!bl.linear && bl.multi > 100000) return fail(RAMP_E_RANGE, "block %u: multi too large", b);
+    }
+    for (uint32_t k = 0; k < d->n_comps; k++) {
+        const ramp_component_desc &c = d->comps[k];
+        if (c.geom.type < RAMP_G_ARM || c.geom.type > RAMP_G_MESH)
+            return fail(RAMP_E_INVALID, "component %u: unknown geometry type %u", k, c.geom.type);
+        if (c.scat.type < RAMP_S_ARM || c.scat.type > RAMP_S_DETPIX)
+            return fail(RAMP_E_INVALID, "component %u: unknown scatter type %u", k, c.scat.type);
+        if (c.comp_idx == 0 || c.comp_idx > 0x00ffffffu)
+            return fail(RAMP_E_INVALID, "component %u: index %u out of range", k, c.comp_idx);
+        bool hist = c.scat.type == RAMP_S_DET1D || c.scat.type == RAMP_S_DET2D ||
+                    c.scat.type == RAMP_S_COUNTER || c.scat.type == RAMP_S_DETPIX;
+        if (hist && (c.det < 0 || (uint32_t)c.det >= d->n_dets))
+            return fail(RAMP_E_INVALID, "component %u: detector slot %d invalid", k, c.det);
+        for (int i = 0; i < 3; i++)
+            if (!std::isfinite(c.pos[i]) || !std::isfinite(c.rot[i]))
+                return fail(RAMP_E_INVALID, "component %u: non-finite position/rotation", k);
+        if (c.scat.type == RAMP_S_CHOPPER && c.scat.ni[CI_NSLITS] == 0)
+            return fail(RAMP_E_INVALID, "component %u: chopper with 0 slits", k);
+        if (c.scat.type == RAMP_S_MONO && c.scat.p[MO_D] == 0.0f)
+            return fail(RAMP_E_INVALID, "component %u: monochromator d-spacing 0", k);
+        if (c.geom.type == RAMP_G_MESH && c.geom.ni[GI_NTRI] * 9ull > c.geom.tab_len[GT_POINTS])
+            return fail(RAMP_E_INVALID, "component %u: mesh table shorter than 9*n_tri", k);
+        const ramp_part_desc &s = c.scat;
+        if (s.type == RAMP_S_POWDERN &&
+            (s.tab_len[PNT_SSCAT_V2] < s.ni[PNI_NLINES] || s.tab_len[PNT_Q] < s.ni[PNI_NLINES]))
+            return fail(RAMP_E_INVALID, "component %u: powder tables shorter than the line count", k);
+        if (s.type == RAMP_S_MONO &&
+            (s.tab_len[MOT_GX] < s.ni[MOI_GLEN] || s.tab_len[MOT_GW] < s.ni[MOI_GLEN]))
+            return fail(RAMP_E_INVALID, "component %u: quadrature tables too short", k);
+        if (s.type == RAMP_S_ISOTROPIC)
+            for (int ch = 0; ch < 3; ch++) {
+                uint64_t nq = s.ni[ISI_NQ(ch)], nw = s.ni[ISI_NW(ch)];
+                if (nq * nw && s.tab_len[ch] < nq + 2 * nw + nw * nq)
+                    return fail(RAMP_E_INVALID, "component %u: S(Q,w) table %d too short", k, ch);
+            }
+        if (s.type == RAMP_S_DETPIX && (uint64_t)s.ni[DTI_NTUBE] * s.ni[DTI_NPIX] * s.ni[DTI_NTOF] >
+                                           (c.det >= 0 ? d->dets[c.det].n_bins : 0))
+            return fail(RAMP_E_INVALID, "component %u: tube bank larger than its histogram", k);
+    }
+    for (uint32_t k = 0; k < d->n_dets; k++)
+        if (d->dets[k].n_bins == 0 || d->dets[k].n_bins > (1ull << 33))
+            return fail(RAMP_E_RANGE, "detector %u: %llu bins", k, (unsigned long long)d->dets[k].n_bins);
+    return 0;
+}
+
+extern "C" int ramp_plan_destroy(ramp_plan *p);
+
+extern "C" int ramp_plan_create(ramp_ctx *ctx, const ramp_plan_desc *d, ramp_plan **out)
+{
+    int rc = validate_plan(d);
+    if (rc) return rc;
+    if (!ctx || !out) return fail(RAMP_E_INVALID, "null context or output");
+    HIPCHK(hipSetDevice(ctx->device));
+    ramp_plan *p = new ramp_plan();
+    p->ctx = ctx;
+    std::memset(&p->stats, 0, sizeof p->stats);
+#define PCHK(x) do { int r_ = (x); if (r_) { ramp_plan_destroy(p); return r_; } } while (0)
+    uint64_t ntab = d->n_tables ? d->n_tables : 1;
+    PCHK(dev_alloc(p->tables, ntab * sizeof(float)));
+    if (d->n_tables && d->tables)
+        HIPCHK(hipMemcpy(p->tables.p, d->tables, d->n_tables * sizeof(float), hipMemcpyHostToDevice));
+    const float *dtab = (const float *)p->tables.p;
+    PCHK(copy_part(p->hsrc, d->source, dtab, d->n_tables));
+    derive_source(p->hsrc);
+    p->hcomps.resize(d->n_comps);
+    for (uint32_t k = 0; k < d->n_comps; k++) {
+        const ramp_component_desc &c = d->comps[k];
+        DComp &dc = p->hcomps[k];
+        std::memset(&dc, 0, sizeof dc);
+        PCHK(copy_part(dc.g, c.geom, dtab, d->n_tables));
+        PCHK(copy_part(dc.s, c.scat, dtab, d->n_tables));
+        derive_geom(dc.g);
+        derive_scat(dc.s);
+        if (c.geom.type == RAMP_G_MESH)
+            PCHK(build_mesh(p, dc.g, d->tables + c.geom.tab_off[GT_POINTS], c.geom.ni[GI_NTRI]));
+        dc.comp_idx = c.comp_idx;
+        dc.restore = c.restore;
+        dc.det = c.det;
+        for (int i = 0; i < 3; i++) dc.pos[i] = c.pos[i];
+        rotation_matrix(c.rot, dc.R);
+    }
+    /* steps: blocks in order, starting from block 0 (mcramp.py:452-453) */
+    for (uint32_t b = 0; b < d->n_blocks; b++) {
+        const ramp_block_desc &bl = d->blocks[b];
+        if (bl.count == 0) continue;
+        if (bl.linear) {
+            for (uint32_t k = 0; k < bl.count; k++) {
+                DStep s = {0u, bl.first + k, 1u, 1u};
+                p->hsteps.push_back(s);
+            }
+        } else {
+            DStep s = {1u, bl.first, bl.count, bl.multi};
+            p->hsteps.push_back(s);
+        }
+    }
+    if (p->hsteps.size() > CNT_N - CNT_NSTEP0)
+        PCHK(fail(RAMP_E_RANGE, "too many steps (%zu)", p->hsteps.size()));
+    p->det_desc.assign(d->dets, d->dets + d->n_dets);
+    p->hdets.resize(d->n_dets);
+    p->hw.resize(d->n_dets);
+    p->hw2.resize(d->n_dets);
+    p->hq.resize(d->n_dets);
+    p->hq2.resize(d->n_dets);
+    for (uint32_t k = 0; k < d->n_dets; k++) {
+        std::memset(&p->hdets[k], 0, sizeof(DDet));
+        p->hdets[k].n = d->dets[k].n_bins;
+    }
+    PCHK(dev_alloc(p->comps, std::max<size_t>(1, p->hcomps.size()) * sizeof(DComp)));
+    PCHK(dev_alloc(p->steps, std::max<size_t>(1, p->hsteps.size()) * sizeof(DStep)));
+    PCHK(dev_alloc(p->src, sizeof(DPart)));
+    PCHK(dev_alloc(p->dets, std::max<size_t>(1, p->hdets.size()) * sizeof(DDet)));
+    PCHK(dev_alloc(p->cnt, CNT_N * sizeof(unsigned long long)));
+    PCHK(dev_alloc(p->count, 64 * sizeof(uint32_t)));
+    if (!p->hcomps.empty())
+        HIPCHK(hipMemcpy(p->comps.p, p->hcomps.data(), p->hcomps.size() * sizeof(DComp), hipMemcpyHostToDevice));
+    if (!p->hsteps.empty())
+        HIPCHK(hipMemcpy(p->steps.p, p->hsteps.data(), p->hsteps.size() * sizeof(DStep), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(p->src.p, &p->hsrc, sizeof(DPart), hipMemcpyHostToDevice));
+#undef PCHK
+    *out = p;
+    return 0;
+}
+
+extern "C" int ramp_plan_destroy(ramp_plan *p)
+{
+    if (!p) return 0;
+    if (p->ctx) {
+        (void)hipSetDevice(p->ctx->device);
+        (void)hipStreamSynchronize(p->ctx->stream);
+    }
+    DevBuf *single[] = {&p->comps, &p->steps, &p->src, &p->dets, &p->tables, &p->cnt, &p->count,
+                        &p->slot_ctr, &p->slot_iidx, &p->st16, &p->st8, &p->stii};
+    for (DevBuf *b : single) dev_free(*b);
+    for (auto *v : {&p->hw, &p->hw2, &p->hq, &p->hq2, &p->mesh_nodes, &p->mesh_order})
+        for (auto &b : *v) dev_free(b);
+    for (int i = 0; i < 2; i++) {
+        for (auto &b : p->q[i].f) dev_free(b);
+        dev_free(p->q[i].key);
+        dev_free(p->q[i].status);
+    }
+    delete p;
+    return 0;
+}
+
+/* ---- run ---------------------------------------------------------------- */
+static void launch_fill_u64(hipStream_t s, void *p, uint64_t n)
+{
+    if (!n) return;
+    unsigned g = (unsigned)std::min<uint64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill_u64, dim3(g), dim3(256), 0, s, (unsigned long long *)p, n);
+}
+static void launch_fill_f32(hipStream_t s, void *p, uint64_t n)
+{
+    if (!n) return;
+    unsigned g = (unsigned)std::min<uint64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill_f32, dim3(g), dim3(256), 0, s, (float *)p, n);
+}
+static void launch_fill_u32(hipStream_t s, void *p, uint64_t n)
+{
+    if (!n) return;
+    unsigned g = (unsigned)std::min<uint64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill_u32, dim3(g), dim3(256), 0, s, (uint32_t *)p, n);
+}
+
+static int ensure_hist(ramp_plan *p, int mode)
+{
+    hipStream_t s = p->ctx->stream;
+    for (size_t k = 0; k < p->hdets.size(); k++) {
+        uint64_t n = p->hdets[k].n;
+        if (mode == RAMP_HIST_ATOMIC && !p->hw[k].p) {
+            int rc = dev_alloc(p->hw[k], n * sizeof(float));
+            if (!rc) rc = dev_alloc(p->hw2[k], n * sizeof(float));
+            if (rc) return rc;
+            launch_fill_f32(s, p->hw[k].p, n);
+            launch_fill_f32(s, p->hw2[k].p, n);
+            p->hdets[k].w = (float *)p->hw[k].p;
+            p->hdets[k].w2 = (float *)p->hw2[k].p;
+        }
+        if (mode == RAMP_HIST_DETERMINISTIC && !p->hq[k].p) {
+            int rc = dev_alloc(p->hq[k], n * sizeof(unsigned long long));
+            if (!rc) rc = dev_alloc(p->hq2[k], n * sizeof(unsigned long long));
+            if (rc) return rc;
+            launch_fill_u64(s, p->hq[k].p, n);
+            launch_fill_u64(s, p->hq2[k].p, n);
+            p->hdets[k].qw = (unsigned long long *)p->hq[k].p;
+            p->hdets[k].qw2 = (unsigned long long *)p->hq2[k].p;
+        }
+    }
+    if (mode == RAMP_HIST_ATOMIC) p->has_atomic = true;
+    else p->has_det = true;
+    return 0;
+}
+
+static int ensure_queues(ramp_plan *p, uint64_t cap, bool full)
+{
+    if (cap <= p->qcap && (!full || p->qfull)) return 0;
+    uint64_t c = std::max(cap, p->qcap);
+    int nf = full ? 15 : 9;
+    for (int i = 0; i < 2; i++) {
+        for (int f = 0; f < nf; f++) {
+            int rc = dev_alloc(p->q[i].f[f], c * sizeof(float));
+            if (rc) return rc;
+        }
+        int rc = dev_alloc(p->q[i].key, c * sizeof(uint32_t));
+        if (!rc) rc = dev_alloc(p->q[i].status, c * sizeof(uint32_t));
+        if (rc) return rc;
+    }
+    p->qcap = c;
+    p->qfull = p->qfull || full;
+    return 0;
+}
+
+static DQueue queue_view(ramp_plan *p, int i)
+{
+    DQueue q;
+    float **f[15] = {&q.x, &q.y, &q.z, &q.vx, &q.vy, &q.vz, &q.w, &q.t, &q.ctr,
+                     &q.px, &q.py, &q.pz, &q.sc, &q.sd, &q.se};
+    for (int k = 0; k < 15; k++) *f[k] = (float *)p->q[i].f[k].p;
+    q.key = (uint32_t *)p->q[i].key.p;
+    q.status = (uint32_t *)p->q[i].status.p;
+    return q;
+}
+
+template <bool FULL>
+static void launch_segment(const SegArgs &a, bool first, unsigned grid, unsigned wg, hipStream_t s)
+{
+    if (first) hipLaunchKernelGGL((k_segment<FULL, true>), dim3(grid), dim3(wg), 0, s, a);
+    else hipLaunchKernelGGL((k_segment<FULL, false>), dim3(grid), dim3(wg), 0, s, a);
+}
+
+/* choose segment boundaries from a pilot pass: a boundary follows every step
+ * after which the live fraction fell below half of the segment's start */
+static int choose_bounds(ramp_plan *p, const SegArgs &base, uint64_t chunk, unsigned wg, int max_seg)
+{
+    int ns = (int)p->hsteps.size();
+    p->bounds.clear();
+    p->bounds.push_back(0);
+    if (ns > 1 && max_seg != 1) {
+        hipStream_t s = p->ctx->stream;
+        uint64_t pn = std::min<uint64_t>(chunk, 65536);
+        SegArgs a = base;
+        a.s0 = 0;
+        a.s1 = ns;
+        a.pilot = 1;
+        a.last = 1;
+        a.carry = 0;
+        a.keep = 0;
+        a.n = pn;
+        a.num_sim = (int)pn;
+        launch_fill_u64(s, p->cnt.p, CNT_N);
+        unsigned grid = (unsigned)std::max<uint64_t>(1, (pn + wg - 1) / wg);
+        launch_segment<false>(a, true, grid, wg, s);
+        HIPCHK(hipGetLastError());
+        std::vector<unsigned long long> cnt(CNT_N);
+        HIPCHK(hipMemcpyAsync(cnt.data(), p->cnt.p, CNT_N * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        double start = (double)pn;
+        int lim = max_seg > 0 ? max_seg : 16;
+        for (int k = 0; k < ns - 1 && (int)p->bounds.size() < lim; k++) {
+            double alive = (double)cnt[CNT_NSTEP0 + k];
+            if (alive < 0.5 * start) {
+                p->bounds.push_back(k + 1);
+                start = std::max(alive, 1.0);
+            }
+        }
+    }
+    p->bounds.push_back(ns);
+    p->bounds_set = true;
+    return 0;
+}
+
+extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uint32_t *seeds,
+                        uint32_t n_seeds, const ramp_run_opts *opts)
+{
+    if (!p) return fail(RAMP_E_INVALID, "null plan");
+    ramp_run_opts o;
+    std::memset(&o, 0, sizeof o);
+    if (opts) o = *opts;
+    if (o.hist_mode != RAMP_HIST_ATOMIC && o.hist_mode != RAMP_HIST_DETERMINISTIC)
+        return fail(RAMP_E_INVALID, "unknown histogram mode %u", o.hist_mode);
+    if (n_hist == 0) return 0;
+    if (chunk == 0 || chunk > n_hist) chunk = n_hist;
+    if (chunk > 0x7fffffffull) return fail(RAMP_E_RANGE, "chunk %llu exceeds 2^31-1", (unsigned long long)chunk);
+    uint64_t n_chunks = (n_hist + chunk - 1) / chunk;
+    if (!seeds || n_seeds < n_chunks)
+        return fail(RAMP_E_INVALID, "need %llu seeds, got %u", (unsigned long long)n_chunks, n_seeds);
+    for (uint64_t c = 0; c < n_chunks; c++)
+        if (seeds[c] > 0x7fffffffu) return fail(RAMP_E_INVALID, "seed %u out of range", seeds[c]);
+    ramp_ctx *ctx = p->ctx;
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    unsigned wg = o.wg ? o.wg : 256;
+    if (wg != 64 && wg != 128 && wg != 256) return fail(RAMP_E_INVALID, "workgroup size %u", wg);
+    bool full = o.keep_state != 0;
+    int rc = ensure_hist(p, (int)o.hist_mode);
+    if (rc) return rc;
+    if (o.hist_mode == RAMP_HIST_DETERMINISTIC && !p->det_scale_set) {
+        double wb = o.weight_bound > 0 ? o.weight_bound : 1.0 / (double)chunk;
+        double tot = wb * (double)n_hist * 16.0;
+        double tot2 = wb * wb * (double)n_hist * 16.0;
+        int sh = (int)std::floor(std::log2(std::ldexp(1.0, 61) / tot));
+        int sh2 = (int)std::floor(std::log2(std::ldexp(1.0, 61) / tot2));
+        sh = std::max(-120, std::min(120, sh));
+        sh2 = std::max(-120, std::min(120, sh2));
+        for (auto &d : p->hdets) { d.sh_w = sh; d.sh_w2 = sh2; }
+        p->det_scale_set = true;
+    }
+    if (!p->hdets.empty())
+        HIPCHK(hipMemcpyAsync(p->dets.p, p->hdets.data(), p->hdets.size() * sizeof(DDet), hipMemcpyHostToDevice, s));
+    bool carry = n_chunks > 1;
+    if (carry) {
+        rc = dev_alloc(p->slot_ctr, chunk * sizeof(float));
+        if (!rc) rc = dev_alloc(p->slot_iidx, chunk * sizeof(uint32_t));
+        if (rc) return rc;
+        launch_fill_f32(s, p->slot_ctr.p, chunk);
+        launch_fill_u32(s, p->slot_iidx.p, chunk);
+    }
+    if (full) {
+        rc = dev_alloc(p->st16, chunk * 16 * sizeof(float));
+        if (!rc) rc = dev_alloc(p->st8, chunk * 8 * sizeof(float));
+        if (!rc) rc = dev_alloc(p->stii, chunk * sizeof(uint32_t));
+        if (rc) return rc;
+    }
+    SegArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.comps = (const DComp *)p->comps.p;
+    a.steps = (const DStep *)p->steps.p;
+    a.src = (const DPart *)p->src.p;
+    a.dets = (const DDet *)p->dets.p;
+    a.cnt = (unsigned long long *)p->cnt.p;
+    a.hist_mode = (int)o.hist_mode;
+    a.carry = carry ? 1 : 0;
+    a.keep = full ? 1 : 0;
+    a.slot_ctr = (float *)p->slot_ctr.p;
+    a.slot_iidx = (uint32_t *)p->slot_iidx.p;
+    a.st16 = (float *)p->st16.p;
+    a.st8 = (float *)p->st8.p;
+    a.stii = (uint32_t *)p->stii.p;
+    if (!p->bounds_set || (o.max_segments && (int)p->bounds.size() - 1 > (int)o.max_segments)) {
+        rc = choose_bounds(p, a, chunk, wg, (int)o.max_segments);
+        if (rc) return rc;
+    }
+    int nseg = (int)p->bounds.size() - 1;
+    if (nseg > 1) {
+        rc = ensure_queues(p, chunk, full);
+        if (rc) return rc;
+    }
+    launch_fill_u64(s, p->cnt.p, CNT_N);
+    HIPCHK(hipEventRecord(ctx->ev[0], s));
+    unsigned grid_cap = (unsigned)ctx->n_cu * 16u * (256u / wg);
+    uint32_t *dcount = (uint32_t *)p->count.p;
+    uint64_t segs = 0;
+    for (uint64_t ck = 0; ck < n_chunks; ck++) {
+        uint64_t torun = std::min(chunk, n_hist - ck * chunk);
+        a.num_sim = (int)torun;
+        a.seed = (int)seeds[ck];
+        a.n = torun;
+        launch_fill_u32(s, dcount, 64);
+        for (int sg = 0; sg < nseg; sg++) {
+            a.s0 = p->bounds[sg];
+            a.s1 = p->bounds[sg + 1];
+            a.last = (sg == nseg - 1) ? 1 : 0;
+            a.pilot = 0;
+            a.in_count = dcount + (sg % 2 == 1 ? 0 : 1) * 16;
+            a.out_count = dcount + (sg % 2 == 0 ? 0 : 1) * 16;
+            if (sg >= 1) a.in = queue_view(p, (sg - 1) % 2);
+            a.out = queue_view(p, sg % 2);
+            unsigned grid;
+            if (sg == 0) grid = (unsigned)std::min<uint64_t>((torun + wg - 1) / wg, grid_cap);
+            else grid = grid_cap;
+            if (sg == 0 && ck == 0) HIPCHK(hipEventRecord(ctx->ev[2], s));
+            if (full) launch_segment<true>(a, sg == 0, grid, wg, s);
+            else launch_segment<false>(a, sg == 0, grid, wg, s);
+            if (sg == 0 && ck == 0) HIPCHK(hipEventRecord(ctx->ev[3], s));
+            HIPCHK(hipGetLastError());
+            segs++;
+            /* the next writer of this queue's counter must start from 0 */
+            if (sg + 1 < nseg) {
+                uint32_t *next_out = dcount + (sg % 2 == 0 ? 1 : 0) * 16;
+                launch_fill_u32(s, next_out, 1);
+            }
+        }
+    }
+    HIPCHK(hipEventRecord(ctx->ev[1], s));
+    HIPCHK(hipEventSynchronize(ctx->ev[1]));
+    std::vector<unsigned long long> cnt(CNT_N);
+    HIPCHK(hipMemcpy(cnt.data(), p->cnt.p, CNT_N * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    float ms = 0, ms0 = 0;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+    HIPCHK(hipEventElapsedTime(&ms0, ctx->ev[2], ctx->ev[3]));
+    p->stats.n_hist = n_hist;
+    p->stats.n_chunks = n_chunks;
+    p->stats.n_events = cnt[CNT_EVENTS];
+    p->stats.n_oob = cnt[CNT_OOB];
+    p->stats.n_det_overflow = cnt[CNT_OVERFLOW];
+    p->stats.n_segments = segs;
+    p->stats.ms_total = ms;
+    p->stats.ms_source_segment = ms0;
+    p->st_n = full ? std::min(chunk, n_hist - (n_chunks - 1) * chunk) : 0;
+    return 0;
+}
+
+extern "C" int ramp_hist_read(ramp_plan *p, uint32_t det, double *w, double *w2, uint64_t n)
+{
+    if (!p || !w || !w2) return fail(RAMP_E_INVALID, "null argument");
+    if (det >= p->hdets.size()) return fail(RAMP_E_INVALID, "detector %u of %zu", det, p->hdets.size());
+    const DDet &d = p->hdets[det];
+    if (n != d.n) return fail(RAMP_E_INVALID, "detector %u has %llu bins, asked %llu", det,
+                              (unsigned long long)d.n, (unsigned long long)n);
+    HIPCHK(hipSetDevice(p->ctx->device));
+    HIPCHK(hipStreamSynchronize(p->ctx->stream));
+    std::fill(w, w + n, 0.0);
+    std::fill(w2, w2 + n, 0.0);
+    if (p->has_atomic && d.w) {
+        std::vector<float> a(n), b(n);
+        HIPCHK(hipMemcpy(a.data(), d.w, n * sizeof(float), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(b.data(), d.w2, n * sizeof(float), hipMemcpyDeviceToHost));
+        for (uint64_t i = 0; i < n; i++) { w[i] += a[i]; w2[i] += b[i]; }
+    }
+    if (p->has_det && d.qw) {
+        std::vector<long long> a(n), b(n);
+        HIPCHK(hipMemcpy(a.data(), d.qw, n * sizeof(long long), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(b.data(), d.qw2, n * sizeof(long long), hipMemcpyDeviceToHost));
+        for (uint64_t i = 0; i < n; i++) {
+            w[i] += std::ldexp((double)a[i], -d.sh_w);
+            w2[i] += std::ldexp((double)b[i], -d.sh_w2);
+        }
+    }
+    return 0;
+}
+
+extern "C" int ramp_hist_reset(ramp_plan *p)
+{
+    if (!p) return fail(RAMP_E_INVALID, "null plan");
+    HIPCHK(hipSetDevice(p->ctx->device));
+    hipStream_t s = p->ctx->stream;
+    for (size_t k = 0; k < p->hdets.size(); k++) {
+        if (p->hw[k].p) { launch_fill_f32(s, p->hw[k].p, p->hdets[k].n); launch_fill_f32(s, p->hw2[k].p, p->hdets[k].n); }
+        if (p->hq[k].p) { launch_fill_u64(s, p->hq[k].p, p->hdets[k].n); launch_fill_u64(s, p->hq2[k].p, p->hdets[k].n); }
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    return 0;
+}
+
+extern "C" int ramp_state_read(ramp_plan *p, float *n16, float *i8, uint32_t *ii, uint64_t n)
+{
+    if (!p) return fail(RAMP_E_INVALID, "null plan");
+    if (p->st_n == 0) return fail(RAMP_E_STATE, "no state kept: run with keep_state=1 first");
+    if (n > p->st_n) return fail(RAMP_E_RANGE, "asked %llu states, %llu kept", (unsigned long long)n,
+                                 (unsigned long long)p->st_n);
+    HIPCHK(hipSetDevice(p->ctx->device));
+    HIPCHK(hipStreamSynchronize(p->ctx->stream));
+    if (n16) HIPCHK(hipMemcpy(n16, p->st16.p, n * 16 * sizeof(float), hipMemcpyDeviceToHost));
+    if (i8) HIPCHK(hipMemcpy(i8, p->st8.p, n * 8 * sizeof(float), hipMemcpyDeviceToHost));
+    if (ii) HIPCHK(hipMemcpy(ii, p->stii.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int ramp_stats_read(const ramp_plan *p, ramp_run_stats *out)
+{
+    if (!p || !out) return fail(RAMP_E_INVALID, "null argument");
+    *out = p->stats;
+    return 0;
+}
+
+extern "C" int ramp_math_eval(ramp_ctx *ctx, int fn, const float *x, const float *y, float *out,
+                              uint64_t n, int on_device)
+{
+    if (!x || !out) return fail(RAMP_E_INVALID, "null argument");
+    if (fn < RAMP_FN_SIN || fn > RAMP_FN_DIV) return fail(RAMP_E_INVALID, "unknown function %d", fn);
+    if (!on_device) {
+        for (uint64_t i = 0; i < n; i++) out[i] = math_eval_host(fn, x[i], y ? y[i] : 0.0f);
+        return 0;
+    }
+    if (!ctx) return fail(RAMP_E_INVALID, "device evaluation needs a context");
+    HIPCHK(hipSetDevice(ctx->device));
+    DevBuf bx, by, bo;
+    int rc = dev_alloc(bx, n * 4 + 4);
+    if (!rc) rc = dev_alloc(by, n * 4 + 4);
+    if (!rc) rc = dev_alloc(bo, n * 4 + 4);
+    if (rc) { dev_free(bx); dev_free(by); dev_free(bo); return rc; }
+    (void)hipMemcpy(bx.p, x, n * 4, hipMemcpyHostToDevice);
+    if (y) (void)hipMemcpy(by.p, y, n * 4, hipMemcpyHostToDevice);
+    unsigned g = (unsigned)std::min<uint64_t>((n + 255) / 256 + 1, 4096);
+    hipLaunchKernelGGL(k_math, dim3(g), dim3(256), 0, ctx->stream, fn, (const float *)bx.p,
+                       y ? (const float *)by.p : nullptr, (float *)bo.p, n);
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) e = hipMemcpy(out, bo.p, n * 4, hipMemcpyDeviceToHost);
+    dev_free(bx); dev_free(by); dev_free(bo);
+    if (e != hipSuccess) return fail(RAMP_E_DEVICE, "math eval: %s", hipGetErrorString(e));
+    return 0;
+}
+
+extern "C" int ramp_rng_eval(ramp_ctx *ctx, const uint32_t *tid, const float *ctr, float *out,
+                             uint64_t n, int on_device)
+{
+    if (!tid || !ctr || !out) return fail(RAMP_E_INVALID, "null argument");
+    if (!on_device) {
+        for (uint64_t i = 0; i < n; i++) {
+            float c = ctr[i];
+            out[i] = rr_draw(&c, tid[i]);
+        }
+        return 0;
+    }
+    if (!ctx) return fail(RAMP_E_INVALID, "device evaluation needs a context");
+    HIPCHK(hipSetDevice(ctx->device));
+    DevBuf bt, bc, bo;
+    int rc = dev_alloc(bt, n * 4 + 4);
+    if (!rc) rc = dev_alloc(bc, n * 4 + 4);
+    if (!rc) rc = dev_alloc(bo, n * 4 + 4);
+    if (rc) { dev_free(bt); dev_free(bc); dev_free(bo); return rc; }
+    (void)hipMemcpy(bt.p, tid, n * 4, hipMemcpyHostToDevice);
+    (void)hipMemcpy(bc.p, ctr, n * 4, hipMemcpyHostToDevice);
+    unsigned g = (unsigned)std::min<uint64_t>((n + 255) / 256 + 1, 4096);
+    hipLaunchKernelGGL(k_rng, dim3(g), dim3(256), 0, ctx->stream, (const uint32_t *)bt.p,
+                       (const float *)bc.p, (float *)bo.p, n);
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) e = hipMemcpy(out, bo.p, n * 4, hipMemcpyDeviceToHost);
+    dev_free(bt); dev_free(bc); dev_free(bo);
+    if (e != hipSuccess) return fail(RAMP_E_DEVICE, "rng eval: %s", hipGetErrorString(e));
+    return 0;
+}

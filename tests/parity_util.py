@@ -1,0 +1,55 @@
+"""Helpers shared by the GPU parity tests: run an instrument on the GPU
+(libramp.so) and on the CPU oracle with the same seeds, and compare."""
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def instrument_cases():
+    ang = float(np.arcsin((9.045 / np.sqrt(5.0)) / (2 * 3.355)))
+    return {
+        "minimal": (os.path.join(ROOT, "ramp_amd", "instruments", "minimal.json"), {}),
+        "guide_chopper": (os.path.join(ROOT, "tests", "instruments", "guide_chopper.json"),
+                          {"v_foc": 841.3}),
+        "powder": (os.path.join(ROOT, "tests", "instruments", "powder.json"), {"Ei": 25.0}),
+        "tas": (os.path.join(ROOT, "tests", "instruments", "tas.json"), {"Mono_angle": ang}),
+    }
+
+
+def run_gpu(path, kw, n, seeds, chunk=None, hist_mode="deterministic", keep_state=True):
+    import ramp_amd
+    inst = ramp_amd.Instrument(path, hist_mode=hist_mode, chunk=chunk, **kw)
+    inst.execute(n, keep_state=keep_state, seeds=seeds)
+    st = inst.state(inst.last_chunk if n % inst.last_chunk == 0 else n % inst.last_chunk) \
+        if keep_state else None
+    hists = [inst.hist64(c.name) for c in inst._comp_list if getattr(c.scat, "n_bins", 0)]
+    stats = inst.stats()
+    inst.close()
+    return st, hists, stats
+
+
+def run_oracle(path, kw, n, seeds, chunk=0):
+    import ramp_amd
+    from oracle import pyoracle
+    return pyoracle.run(ramp_amd.Instrument(path, **kw), n, seeds, chunk=chunk, keep_state=True)
+
+
+def compare_states(g, o):
+    """Counts of differences between GPU and oracle final states."""
+    g16, g8, gi = g
+    o16, o8, oi = o
+    out = {}
+    out["dead"] = int((g16[:, 15] != o16[:, 15]).sum())
+    out["iidx"] = int((gi != oi).sum())
+    out["ctr"] = int((g16[:, 11] != o16[:, 11]).sum())
+    bits = (g16.view(np.uint32) != o16.view(np.uint32)) & ~(np.isnan(g16) & np.isnan(o16))
+    out["bitdiff_values"] = int(bits.sum())
+    out["bitdiff_rows"] = int(bits.any(axis=1).sum())
+    scale = np.maximum(np.nanmax(np.abs(o16), axis=0), 1e-30)
+    close = np.isclose(g16, o16, rtol=1e-5, atol=0) | (np.abs(g16 - o16) <= 1e-6 * scale) | \
+        (np.isnan(g16) & np.isnan(o16))
+    out["beyond_1e-5"] = int((~close).sum())
+    out["beyond_rows"] = int((~close).any(axis=1).sum())
+    return out
