@@ -144,6 +144,7 @@ typedef struct {
     uint64_t state_in;       /* neutrons read from compacted queues */
     uint64_t state_out;      /* neutrons written to compacted queues */
     uint64_t n_det_overflow; /* deterministic mode: events beyond the fixed-point range */
+    uint64_t state_out_first;/* neutrons the first (source) segment wrote to its queue */
     double ms_total;         /* device time of the last run (HIP events) */
     double ms_source_segment;/* device time of the first (source) segment of the last run */
 } ramp_run_stats;

@@ -107,6 +107,7 @@ class RunStats(C.Structure):
         ("state_in", C.c_uint64),
         ("state_out", C.c_uint64),
         ("n_det_overflow", C.c_uint64),
+        ("state_out_first", C.c_uint64),
         ("ms_total", C.c_double),
         ("ms_source_segment", C.c_double),
     ]

@@ -66,6 +66,9 @@ struct DQueue {
 #define CNT_EVENTS 0
 #define CNT_OOB 1
 #define CNT_OVERFLOW 2
+#define CNT_QOUT 3
+#define CNT_QOUT0 4
+#define CNT_QIN 5
 #define CNT_NSTEP0 8                     /* pilot: alive after step k at [8+k] */
 #define CNT_N 264
 

@@ -188,6 +188,7 @@ __global__ __launch_bounds__(256) void k_segment(SegArgs a)
     hc.enabled = !a.pilot;
     uint64_t n_items = FIRST ? a.n : (uint64_t)(*a.in_count);
     uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    if (!FIRST && !a.pilot && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&a.cnt[CNT_QIN], n_items);
     for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n_items; base += stride) {
         uint64_t i = base + threadIdx.x;
         bool valid = i < n_items;
@@ -213,7 +214,11 @@ __global__ __launch_bounds__(256) void k_segment(SegArgs a)
             unsigned long long m = __ballot(live);
             uint32_t cnt = (uint32_t)__popcll(m);
             uint32_t off = 0;
-            if (__lane_id() == 0 && cnt) off = atomicAdd(a.out_count, cnt);
+            if (__lane_id() == 0 && cnt) {
+                off = atomicAdd(a.out_count, cnt);
+                atomicAdd(&a.cnt[CNT_QOUT], (unsigned long long)cnt);
+                if (FIRST) atomicAdd(&a.cnt[CNT_QOUT0], (unsigned long long)cnt);
+            }
             off = __shfl(off, 0);
             if (live) {
                 unsigned long long below = m & ((1ull << __lane_id()) - 1ull);
@@ -1065,6 +1070,9 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
     p->stats.n_oob = cnt[CNT_OOB];
     p->stats.n_det_overflow = cnt[CNT_OVERFLOW];
     p->stats.n_segments = segs;
+    p->stats.state_out = cnt[CNT_QOUT];
+    p->stats.state_out_first = cnt[CNT_QOUT0];
+    p->stats.state_in = cnt[CNT_QIN];
     p->stats.ms_total = ms;
     p->stats.ms_source_segment = ms0;
     p->st_n = full ? std::min(chunk, n_hist - (n_chunks - 1) * chunk) : 0;

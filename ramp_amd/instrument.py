@@ -283,13 +283,16 @@ class Instrument:
         self.last_seeds = seeds
         self.last_chunk = chunk
         self.last_stats = self.stats()
-        self._pull_hists()
+        self._hist_stale = True
         if debug:
             n16, _, _ = self.state(min(chunk, N))
             live = n16[n16[:, 15] == 0.0]
             print(live)
 
     def _pull_hists(self):
+        if self._plan is None:
+            return
+        self._hist_stale = False
         lib = capi.load_library()
         for k, c in enumerate(self._comp_list):
             if k in self._det_of:
@@ -302,8 +305,13 @@ class Instrument:
                                               w2.ctypes.data_as(C.POINTER(C.c_double)), nb))
                 c.scat._set_hist(w, w2)
 
+    def _sync_hists(self):
+        if getattr(self, "_hist_stale", False):
+            self._pull_hists()
+
     def hist64(self, name):
         """(w, w^2) float64 sums of a detector component (extension)."""
+        self._sync_hists()
         for c in self._comp_list:
             if c.name == name:
                 return c.scat._w, c.scat._w2
@@ -335,15 +343,18 @@ class Instrument:
     # ---- results ------------------------------------------------------------
     def data(self):
         """{component name: data} (mcramp.py:281-293)."""
+        self._sync_hists()
         return {c.name: c.scat.data(self.queue) for c in self.kernel_refs}
 
     def plot(self):
         from matplotlib.pyplot import show
+        self._sync_hists()
         for c in self.kernel_refs:
             c.scat.plot(self.queue)
         show()
 
     def save(self):
+        self._sync_hists()
         for c in self.kernel_refs:
             c.scat.save(self.queue)
 
