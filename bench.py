@@ -39,6 +39,8 @@ def parse():
     ap.add_argument("--hist-mode", default="atomic", choices=["atomic", "deterministic"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="first run a device-to-device copy of this many MiB (HBM counter scale)")
     return ap.parse_args()
 
 
@@ -73,6 +75,13 @@ def main():
         sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
     except Exception:
         sync = lambda: None  # noqa: E731
+    if a.calibrate_copy:
+        import torch
+        src = torch.ones(a.calibrate_copy << 20, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     path = os.path.join(ROOT, "ramp_amd", "instruments", "let_like.json")
     N = int(a.histories)
     inst = ramp_amd.Instrument(path, hist_mode=a.hist_mode, chunk=N)

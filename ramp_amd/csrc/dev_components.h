@@ -25,6 +25,15 @@
 #include "../../include/ramp.h"
 
 #define DEV __device__ static inline __attribute__((always_inline))
+
+/* feature sets compiled into a segment-kernel instance: component types
+ * outside the set are compiled out, which keeps register use (occupancy)
+ * at what the instrument actually needs */
+#define FEAT_MESH 1u    /* GMesh */
+#define FEAT_SAMPLE 2u  /* SIsotropic, SPowder1, SPowderN */
+#define FEAT_MONO 4u    /* SMonochromator */
+#define FEAT_ISIS 8u    /* MISIS */
+#define FEAT_ALL 15u
 #define DEVNI __device__ static __attribute__((noinline))
 
 struct NS {
@@ -65,11 +74,28 @@ DEV rf3 mat_rot(const float *R, rf3 v)
                     (R[6] * v.x + R[7] * v.y) + R[8] * v.z);
 }
 
+/* For the matrix of zero Euler angles (1 on the diagonal, signed zeros
+ * elsewhere) every product of an off-diagonal entry is a signed zero, so
+ * (x*1 + y*0) + z*0 equals x exactly whenever x, y, z are finite and x is
+ * nonzero.  When all six components of a wave's neutrons satisfy this the
+ * matrix product is skipped; otherwise it is evaluated in full. */
+DEV bool fin_nz(float v) { return v != 0.0f && (v - v) == 0.0f; }
+DEV bool ident_ok(const NS &n, float x, float y, float z)
+{
+    return n.dead || (fin_nz(x) && fin_nz(y) && fin_nz(z) && fin_nz(n.vx) && fin_nz(n.vy) &&
+                      fin_nz(n.vz));
+}
+
 template <bool FULL>
 DEV void tr_in(NS &n, const DComp &c)
 {
-    if (n.dead) return;
     rf3 p = rf3_make(n.x - c.pos[0], n.y - c.pos[1], n.z - c.pos[2]);
+    if (!FULL && c.pad == 1u && __all(ident_ok(n, p.x, p.y, p.z))) {
+        if (n.dead) return;
+        setpos(n, p);
+        return;
+    }
+    if (n.dead) return;
     setpos(n, mat_derot(c.R, p));
     setvel(n, mat_derot(c.R, nvel(n)));
     if (FULL) {
@@ -81,6 +107,11 @@ DEV void tr_in(NS &n, const DComp &c)
 template <bool FULL>
 DEV void tr_out(NS &n, const DComp &c)
 {
+    if (!FULL && c.pad == 1u && __all(ident_ok(n, n.x, n.y, n.z))) {
+        if (n.dead) return;
+        n.x = n.x + c.pos[0]; n.y = n.y + c.pos[1]; n.z = n.z + c.pos[2];
+        return;
+    }
     if (n.dead) return;
     rf3 p = mat_rot(c.R, npos(n));
     n.x = p.x + c.pos[0]; n.y = p.y + c.pos[1]; n.z = p.z + c.pos[2];
@@ -95,7 +126,8 @@ DEV void tr_out(NS &n, const DComp &c)
 DEV void terminate(NS &n, uint32_t restore)
 {
     if (n.dead) return;
-    if (v3_len(ipos2(n)) == 0.0f && restore == 0) n.dead = true;
+    /* length(pos2) == 0 exactly when its squared norm is 0 */
+    if (v3_dot(ipos2(n), ipos2(n)) == 0.0f && restore == 0) n.dead = true;
 }
 
 /* ---------------- sources ---------------------------------------------- */
@@ -151,11 +183,9 @@ DEV float maxwell_dev(float E, float overKT, float pf, rdf rsqpi)
     return df_mul_f(c, ef);
 }
 
-template <bool FULL>
-DEV void src_generate(NS &n, const DPart &s, const float *tabs_dummy, int num_sim, int seed,
-                      float sb_old)
+template <bool FULL, uint32_t F>
+DEV void src_generate(NS &n, const DPart &s, int num_sim, int seed, float sb_old)
 {
-    (void)tabs_dummy;
     src_clear(n, seed, sb_old);
     switch (s.type) {
     case RAMP_SRC_FLAT: {
@@ -177,14 +207,16 @@ DEV void src_generate(NS &n, const DPart &s, const float *tabs_dummy, int num_si
         else
             term1 = s.p[SP_I1] * maxwell_dev(E_J, s.d[0], s.d[3], rsq) / s.d[6];
         float M = term1;
-        M += s.p[SP_I2] * maxwell_dev(E_J, s.d[1], s.d[4], rsq) / s.d[7];
-        M += s.p[SP_I3] * maxwell_dev(E_J, s.d[2], s.d[5], rsq) / s.d[8];
+        /* d[16]/d[17]: the term has I = 0 and a finite Maxwellian value, so
+         * adding it leaves M unchanged and it is not evaluated */
+        if (s.d[16] == 0.0f) M += s.p[SP_I2] * maxwell_dev(E_J, s.d[1], s.d[4], rsq) / s.d[7];
+        if (s.d[17] == 0.0f) M += s.p[SP_I3] * maxwell_dev(E_J, s.d[2], s.d[5], rsq) / s.d[8];
         n.w = M / (float)num_sim;
         src_posdir(n, s, RC_SE2V * rm_sqrt(E_val));
         src_pol<FULL>(n, s);
         break;
     }
-    case RAMP_SRC_ISIS: {
+    case RAMP_SRC_ISIS: if constexpr ((F & FEAT_ISIS) != 0) {
         /* isis_mod.cl:15-105 (scans bounded by the table, index clamped) */
         const float *flux = s.tab[ST_ISIS_FLUX], *tb = s.tab[ST_ISIS_TBINS];
         const float *eb = s.tab[ST_ISIS_EBINS], *Eint = s.tab[ST_ISIS_EINT];
@@ -226,7 +258,7 @@ DEV void src_generate(NS &n, const DPart &s, const float *tabs_dummy, int num_si
         n.w = s.p[SP_ISIS_STR_AREA] * s.p[SP_ISIS_TOTAL] * RC_ISIS_WSCALE / (float)num_sim;
         n.t = time_val;
         break;
-    }
+    } else { break; }
     case RAMP_SRC_LAMBDA: {
         float lam = s.p[SP_LMIN] + (s.p[SP_LMAX] - s.p[SP_LMIN]) * drw(n);
         n.w = (1.0f / lam) * s.p[SP_LNORM] / (float)num_sim;
@@ -375,6 +407,7 @@ DEV void g_cyl(NS &n, const DPart &g, uint32_t ci, bool banana)
 
 #include "dev_mesh.h"
 
+template <uint32_t F>
 DEV void geom_dispatch(NS &n, const DComp &c)
 {
     if (n.dead) return;
@@ -390,25 +423,29 @@ DEV void geom_dispatch(NS &n, const DComp &c)
     case RAMP_G_SPHERE_INT: g_sphere(n, g, c.comp_idx, true); break;
     case RAMP_G_CYLINDER: g_cyl(n, g, c.comp_idx, false); break;
     case RAMP_G_BANANA: g_cyl(n, g, c.comp_idx, true); break;
-    case RAMP_G_MESH: g_mesh(n, g, c.comp_idx); break;
+    case RAMP_G_MESH:
+        if constexpr ((F & FEAT_MESH) != 0) g_mesh(n, g, c.comp_idx);
+        break;
     default: break;
     }
 }
 
 /* ---------------- detectors -------------------------------------------- */
+/* per-lane event tallies are kept in registers and reduced once per wave
+ * at the end of a segment kernel (no same-address atomics on the hot path) */
 struct HistCtx {
     const DDet *dets;
-    unsigned long long *cnt;
     int mode;       /* RAMP_HIST_* */
     int enabled;    /* 0 in pilot runs */
+    uint32_t events, oob, overflow;
 };
 
-DEV void hist_add(const HistCtx &hc, int det, uint64_t idx, float w)
+DEV void hist_add(HistCtx &hc, int det, uint64_t idx, float w)
 {
     if (!hc.enabled) return;
     const DDet &d = hc.dets[det];
     if (idx >= d.n) {
-        atomicAdd(&hc.cnt[CNT_OOB], 1ull);
+        hc.oob++;
         return;
     }
     float w2 = w * w;
@@ -416,7 +453,7 @@ DEV void hist_add(const HistCtx &hc, int det, uint64_t idx, float w)
         float a = rm_rint(rm_ldexp(w, d.sh_w));
         float b = rm_rint(rm_ldexp(w2, d.sh_w2));
         if (!(rm_fabs(a) < 0x1p62f) || !(rm_fabs(b) < 0x1p62f)) {
-            atomicAdd(&hc.cnt[CNT_OVERFLOW], 1ull);
+            hc.overflow++;
             return;
         }
         atomicAdd(&d.qw[idx], (unsigned long long)(long long)a);
@@ -425,7 +462,7 @@ DEV void hist_add(const HistCtx &hc, int det, uint64_t idx, float w)
         unsafeAtomicAdd(&d.w[idx], w);
         unsafeAtomicAdd(&d.w2[idx], w2);
     }
-    atomicAdd(&hc.cnt[CNT_EVENTS], 1ull);
+    hc.events++;
 }
 
 /* wavelength variable: 2*M_PI/(V2K*|v|) with a double numerator */
@@ -455,7 +492,7 @@ DEV void det_finish(NS &n, uint32_t restore)
 }
 
 /* scat/detector1D.cl:20-77 */
-DEV void s_det1d(NS &n, const DComp &c, const HistCtx &hc)
+DEV void s_det1d(NS &n, const DComp &c, HistCtx &hc)
 {
     const DPart &s = c.s;
     float mn = s.p[D1_MIN], st = s.p[D1_STEP], mx = s.p[D1_MAX];
@@ -499,7 +536,7 @@ DEV int det2d_idx(float v, float mn, float st, float mx)
 }
 
 /* scat/detector2D.cl:31-186 */
-DEV void s_det2d(NS &n, const DComp &c, const HistCtx &hc)
+DEV void s_det2d(NS &n, const DComp &c, HistCtx &hc)
 {
     const DPart &s = c.s;
     float v1 = det2d_var(n, s, s.ni[D2I_VAR1], 1);
@@ -515,7 +552,7 @@ DEV void s_det2d(NS &n, const DComp &c, const HistCtx &hc)
 }
 
 /* tube x pixel x TOF bank (no reference counterpart) */
-DEV void s_detpix(NS &n, const DComp &c, const HistCtx &hc)
+DEV void s_detpix(NS &n, const DComp &c, HistCtx &hc)
 {
     const DPart &s = c.s;
     uint32_t nt = s.ni[DTI_NTUBE], np = s.ni[DTI_NPIX], nf = s.ni[DTI_NTOF];
@@ -534,7 +571,7 @@ DEV void s_detpix(NS &n, const DComp &c, const HistCtx &hc)
 }
 
 /* scat/counter.cl:18-49 */
-DEV void s_counter(NS &n, const DComp &c, const HistCtx &hc)
+DEV void s_counter(NS &n, const DComp &c, HistCtx &hc)
 {
     hist_add(hc, c.det, 0, n.w);
     n.se = 1.0f;
@@ -679,8 +716,8 @@ DEV void s_absorber(NS &n, const DComp &c)
 
 #include "dev_samples.h"
 
-template <bool FULL>
-DEV void scat_dispatch(NS &n, const DComp &c, const HistCtx &hc)
+template <bool FULL, uint32_t F>
+DEV void scat_dispatch(NS &n, const DComp &c, HistCtx &hc)
 {
     if (n.iidx != c.comp_idx) return;
     if (n.dead) return;
@@ -695,10 +732,18 @@ DEV void scat_dispatch(NS &n, const DComp &c, const HistCtx &hc)
         break;
     case RAMP_S_CHOPPER: s_chopper(n, c); break;
     case RAMP_S_GUIDE: s_guide(n, c); break;
-    case RAMP_S_ISOTROPIC: s_isotropic<FULL>(n, c); break;
-    case RAMP_S_POWDER1: s_powder1(n, c); break;
-    case RAMP_S_POWDERN: s_powderN(n, c); break;
-    case RAMP_S_MONO: s_mono(n, c); break;
+    case RAMP_S_ISOTROPIC:
+        if constexpr ((F & FEAT_SAMPLE) != 0) s_isotropic<FULL>(n, c);
+        break;
+    case RAMP_S_POWDER1:
+        if constexpr ((F & FEAT_SAMPLE) != 0) s_powder1(n, c);
+        break;
+    case RAMP_S_POWDERN:
+        if constexpr ((F & FEAT_SAMPLE) != 0) s_powderN(n, c);
+        break;
+    case RAMP_S_MONO:
+        if constexpr ((F & FEAT_MONO) != 0) s_mono(n, c);
+        break;
     case RAMP_S_DET1D: s_det1d(n, c, hc); break;
     case RAMP_S_DET2D: s_det2d(n, c, hc); break;
     case RAMP_S_COUNTER: s_counter(n, c, hc); break;

@@ -31,7 +31,7 @@ struct DComp {
     uint32_t comp_idx;
     uint32_t restore;
     int32_t det;
-    uint32_t pad;
+    uint32_t pad;            /* 1: zero Euler angles (identity up to signed zeros) */
     float pos[3];
     float R[9];
 };

@@ -72,9 +72,13 @@ struct SegArgs {
     int keep;
     int num_sim;
     int seed;
-    uint64_t n;                 /* items of segment 0 */
-    const uint32_t *in_count;   /* items of later segments */
-    uint32_t *out_count;
+    uint64_t n;                 /* items of the first segment */
+    /* input of later segments: wave regions of the previous segment */
+    const uint64_t *in_prefix;  /* [n_in + 1] exclusive prefix of region counts */
+    const uint64_t *in_meta;    /* [0] region stride, [1] number of regions */
+    /* output: one region per wave, stride written to out_meta[0] */
+    uint32_t *out_counts;
+    uint64_t *out_meta;
     DQueue in, out;
     float *slot_ctr;
     uint32_t *slot_iidx;
@@ -82,8 +86,8 @@ struct SegArgs {
     uint32_t *stii;
 };
 
-template <bool FULL>
-DEV void run_steps(NS &n, const SegArgs &a, const HistCtx &hc)
+template <bool FULL, uint32_t F>
+DEV void run_steps(NS &n, const SegArgs &a, HistCtx &hc)
 {
     for (int k = a.s0; k < a.s1; k++) {
         if (!a.pilot && !__any(!n.dead)) break;
@@ -91,22 +95,22 @@ DEV void run_steps(NS &n, const SegArgs &a, const HistCtx &hc)
         if (st.kind == 0) {
             const DComp &c = a.comps[st.first];
             tr_in<FULL>(n, c);
-            geom_dispatch(n, c);
+            geom_dispatch<F>(n, c);
             terminate(n, c.restore);
-            scat_dispatch<FULL>(n, c, hc);
+            scat_dispatch<FULL, F>(n, c, hc);
             tr_out<FULL>(n, c);
         } else {
             for (uint32_t ev = 0; ev < st.multi; ev++) {
                 for (uint32_t j = 0; j < st.count; j++) {
                     const DComp &c = a.comps[st.first + j];
                     tr_in<FULL>(n, c);
-                    geom_dispatch(n, c);
+                    geom_dispatch<F>(n, c);
                     tr_out<FULL>(n, c);
                 }
                 for (uint32_t j = 0; j < st.count; j++) {
                     const DComp &c = a.comps[st.first + j];
                     tr_in<FULL>(n, c);
-                    scat_dispatch<FULL>(n, c, hc);
+                    scat_dispatch<FULL, F>(n, c, hc);
                     tr_out<FULL>(n, c);
                 }
             }
@@ -178,54 +182,127 @@ DEV void queue_store(const NS &n, const DQueue &q, uint64_t i)
     }
 }
 
-template <bool FULL, bool FIRST>
+DEV uint32_t wave_sum(uint32_t v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+/* map a dense input index to its slot in the previous segment's wave
+ * regions: the region is the last one whose prefix is <= v */
+DEV uint64_t region_slot(const uint64_t *prefix, uint64_t nreg, uint64_t stride, uint64_t v)
+{
+    uint64_t lo = 0, hi = nreg;  /* prefix[lo] <= v < prefix[hi] */
+    while (hi - lo > 1) {
+        uint64_t mid = (lo + hi) >> 1;
+        if (prefix[mid] <= v) lo = mid;
+        else hi = mid;
+    }
+    return lo * stride + (v - prefix[lo]);
+}
+
+/* One segment.  Work is split statically: wave g of G handles the dense
+ * item range [g*per, (g+1)*per) and appends its survivors to its own region
+ * [g*per, ...) of the output queue, so compaction needs no atomics; the
+ * region counts are prefix-summed by k_scan before the next segment. */
+template <bool FULL, bool FIRST, uint32_t F>
 __global__ __launch_bounds__(256) void k_segment(SegArgs a)
 {
     HistCtx hc;
     hc.dets = a.dets;
-    hc.cnt = a.cnt;
     hc.mode = a.hist_mode;
     hc.enabled = !a.pilot;
-    uint64_t n_items = FIRST ? a.n : (uint64_t)(*a.in_count);
-    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    if (!FIRST && !a.pilot && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&a.cnt[CNT_QIN], n_items);
-    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n_items; base += stride) {
-        uint64_t i = base + threadIdx.x;
-        bool valid = i < n_items;
+    hc.events = hc.oob = hc.overflow = 0;
+    const uint32_t lane = __lane_id();
+    const uint64_t G = (uint64_t)gridDim.x * (blockDim.x >> 6);
+    const uint64_t g = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    uint64_t total, nreg = 0, stride = 0;
+    if (FIRST) {
+        total = a.n;
+    } else {
+        nreg = a.in_meta[1];
+        stride = a.in_meta[0];
+        total = a.in_prefix[nreg];
+    }
+    const uint64_t per = (total + G - 1) / G;
+    if (g == 0 && lane == 0 && !a.pilot) a.out_meta[0] = per;
+    const uint64_t beg = g * per;
+    const uint64_t end = beg + per < total ? beg + per : total;
+    uint32_t wcount = 0;
+    for (uint64_t base = beg; base < end; base += 64) {
+        uint64_t i = base + lane;
+        bool valid = i < end;
         NS n;
         if (valid) {
             if (FIRST) {
                 float sb_old = a.carry ? a.slot_ctr[i] : 0.0f;
                 n.iidx = a.carry ? a.slot_iidx[i] : 0u;
                 n.key = (uint32_t)i;
-                src_generate<FULL>(n, *a.src, nullptr, a.num_sim, a.seed, sb_old);
+                src_generate<FULL, F>(n, *a.src, a.num_sim, a.seed, sb_old);
             } else {
-                queue_load<FULL>(n, a.in, i);
+                queue_load<FULL>(n, a.in, region_slot(a.in_prefix, nreg, stride, i));
             }
         } else {
             n.dead = true;
             n.key = 0;
             n.iidx = 0;
         }
-        run_steps<FULL>(n, a, hc);
+        run_steps<FULL, F>(n, a, hc);
         if (a.pilot) continue;
         bool live = valid && !n.dead && !a.last;
         if (!a.last) {
             unsigned long long m = __ballot(live);
-            uint32_t cnt = (uint32_t)__popcll(m);
-            uint32_t off = 0;
-            if (__lane_id() == 0 && cnt) {
-                off = atomicAdd(a.out_count, cnt);
-                atomicAdd(&a.cnt[CNT_QOUT], (unsigned long long)cnt);
-                if (FIRST) atomicAdd(&a.cnt[CNT_QOUT0], (unsigned long long)cnt);
-            }
-            off = __shfl(off, 0);
             if (live) {
-                unsigned long long below = m & ((1ull << __lane_id()) - 1ull);
-                queue_store<FULL>(n, a.out, (uint64_t)off + (uint64_t)__popcll(below));
+                unsigned long long below = m & ((1ull << lane) - 1ull);
+                queue_store<FULL>(n, a.out, beg + wcount + (uint64_t)__popcll(below));
             }
+            wcount += (uint32_t)__popcll(m);
         }
         if (valid && !live) finalize<FULL>(n, a);
+    }
+    if (a.pilot) return;
+    if (!a.last && lane == 0) a.out_counts[g] = wcount;
+    uint32_t ev = wave_sum(hc.events), oob = wave_sum(hc.oob), ovf = wave_sum(hc.overflow);
+    if (lane == 0) {
+        if (ev) atomicAdd(&a.cnt[CNT_EVENTS], (unsigned long long)ev);
+        if (oob) atomicAdd(&a.cnt[CNT_OOB], (unsigned long long)oob);
+        if (ovf) atomicAdd(&a.cnt[CNT_OVERFLOW], (unsigned long long)ovf);
+    }
+}
+
+/* exclusive prefix of the wave-region counts of one segment (one block);
+ * also records the totals for the run statistics */
+__global__ __launch_bounds__(1024) void k_scan(const uint32_t *counts, uint64_t nreg,
+                                                const uint64_t *prod_meta, uint64_t *prefix,
+                                                uint64_t *cons_meta, unsigned long long *cnt,
+                                                int first)
+{
+    __shared__ uint64_t part[1024];
+    const uint32_t t = threadIdx.x;
+    const uint64_t per = (nreg + 1023) / 1024;
+    const uint64_t b = t * per, e = b + per < nreg ? b + per : nreg;
+    uint64_t sum = 0;
+    for (uint64_t i = b; i < e; i++) sum += counts[i];
+    part[t] = sum;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        uint64_t v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint64_t run = part[t] - sum;
+    for (uint64_t i = b; i < e; i++) {
+        prefix[i] = run;
+        run += counts[i];
+    }
+    if (t == 1023) {
+        prefix[nreg] = part[1023];
+        cons_meta[0] = prod_meta[0];
+        cons_meta[1] = nreg;
+        cnt[CNT_QOUT] += part[1023];
+        cnt[CNT_QIN] += part[1023];
+        if (first) cnt[CNT_QOUT0] += part[1023];
     }
 }
 
@@ -349,6 +426,11 @@ struct QueueBufs {
     DevBuf key, status;
 };
 
+/* feature-set instances of the segment kernel, cheapest first */
+static const uint32_t kVariants[] = {0u, FEAT_SAMPLE, FEAT_MONO, FEAT_MESH, FEAT_MESH | FEAT_SAMPLE,
+                                     FEAT_ALL};
+static const int kNumVariants = 6;
+
 struct ramp_plan {
     ramp_ctx *ctx = nullptr;
     std::vector<DComp> hcomps;
@@ -362,12 +444,13 @@ struct ramp_plan {
     QueueBufs q[2];
     uint64_t qcap = 0;
     bool qfull = false;
-    DevBuf slot_ctr, slot_iidx, st16, st8, stii;
+    DevBuf slot_ctr, slot_iidx, st16, st8, stii, rcounts, prefix;
     uint64_t st_n = 0;
     bool det_scale_set = false;
     bool has_atomic = false, has_det = false;
     std::vector<int> bounds;  /* segment boundaries (step indices) */
     bool bounds_set = false;
+    int variant = kNumVariants - 1;
     ramp_run_stats stats;
 };
 
@@ -443,6 +526,13 @@ static void derive_source(DPart &s)
         s.d[9] = EjJ;
         s.d[10] = s.p[SP_TAIL_JOIN] > 0.0f ? maxwell_ref_host(EjJ, T[0]) : 0.0f;
         set_df(s.d, 11, 1.0 / sqrt(RC_M_PI_D));
+        /* terms 2 and 3 with I == 0 add exactly +-0 when the Maxwellian is
+         * finite, which holds for 1 K <= T <= 1e5 K and E <= 1e4 meV */
+        for (int i = 1; i < 3; i++) {
+            float I = i == 1 ? s.p[SP_I2] : s.p[SP_I3];
+            bool finite = T[i] >= 1.0f && T[i] <= 1e5f && s.p[SP_EMAX] <= 1e4f && s.p[SP_EMIN] >= 0.0f;
+            s.d[15 + i] = (I == 0.0f && finite && s.d[6 + i] > 0.0f && std::isfinite(s.d[6 + i])) ? 1.0f : 0.0f;
+        }
     }
 }
 
@@ -754,7 +844,18 @@ extern "C" int ramp_plan_create(ramp_ctx *ctx, const ramp_plan_desc *d, ramp_pla
         dc.det = c.det;
         for (int i = 0; i < 3; i++) dc.pos[i] = c.pos[i];
         rotation_matrix(c.rot, dc.R);
+        dc.pad = (c.rot[0] == 0.0f && c.rot[1] == 0.0f && c.rot[2] == 0.0f) ? 1u : 0u;
     }
+    uint32_t feat = 0;
+    if (d->source.type == RAMP_SRC_ISIS) feat |= FEAT_ISIS;
+    for (uint32_t k = 0; k < d->n_comps; k++) {
+        uint32_t g = d->comps[k].geom.type, sc = d->comps[k].scat.type;
+        if (g == RAMP_G_MESH) feat |= FEAT_MESH;
+        if (sc == RAMP_S_ISOTROPIC || sc == RAMP_S_POWDER1 || sc == RAMP_S_POWDERN) feat |= FEAT_SAMPLE;
+        if (sc == RAMP_S_MONO) feat |= FEAT_MONO;
+    }
+    for (int v = 0; v < kNumVariants; v++)
+        if ((feat & ~kVariants[v]) == 0) { p->variant = v; break; }
     /* steps: blocks in order, starting from block 0 (mcramp.py:452-453) */
     for (uint32_t b = 0; b < d->n_blocks; b++) {
         const ramp_block_desc &bl = d->blocks[b];
@@ -805,7 +906,8 @@ extern "C" int ramp_plan_destroy(ramp_plan *p)
         (void)hipStreamSynchronize(p->ctx->stream);
     }
     DevBuf *single[] = {&p->comps, &p->steps, &p->src, &p->dets, &p->tables, &p->cnt, &p->count,
-                        &p->slot_ctr, &p->slot_iidx, &p->st16, &p->st8, &p->stii};
+                        &p->slot_ctr, &p->slot_iidx, &p->st16, &p->st8, &p->stii,
+                        &p->rcounts, &p->prefix};
     for (DevBuf *b : single) dev_free(*b);
     for (auto *v : {&p->hw, &p->hw2, &p->hq, &p->hq2, &p->mesh_nodes, &p->mesh_order})
         for (auto &b : *v) dev_free(b);
@@ -897,11 +999,42 @@ static DQueue queue_view(ramp_plan *p, int i)
     return q;
 }
 
-template <bool FULL>
-static void launch_segment(const SegArgs &a, bool first, unsigned grid, unsigned wg, hipStream_t s)
+
+typedef void (*seg_fn)(SegArgs);
+
+template <bool FULL, bool FIRST>
+static seg_fn seg_kernel(int v)
 {
-    if (first) hipLaunchKernelGGL((k_segment<FULL, true>), dim3(grid), dim3(wg), 0, s, a);
-    else hipLaunchKernelGGL((k_segment<FULL, false>), dim3(grid), dim3(wg), 0, s, a);
+    switch (v) {
+    case 0: return k_segment<FULL, FIRST, 0u>;
+    case 1: return k_segment<FULL, FIRST, FEAT_SAMPLE>;
+    case 2: return k_segment<FULL, FIRST, FEAT_MONO>;
+    case 3: return k_segment<FULL, FIRST, FEAT_MESH>;
+    case 4: return k_segment<FULL, FIRST, FEAT_MESH | FEAT_SAMPLE>;
+    default: return k_segment<FULL, FIRST, FEAT_ALL>;
+    }
+}
+
+static seg_fn pick_kernel(bool full, bool first, int v)
+{
+    if (full) return first ? seg_kernel<true, true>(v) : seg_kernel<true, false>(v);
+    return first ? seg_kernel<false, true>(v) : seg_kernel<false, false>(v);
+}
+
+static void launch_segment(const SegArgs &a, bool full, bool first, int v, unsigned grid,
+                           unsigned wg, hipStream_t s)
+{
+    hipLaunchKernelGGL(pick_kernel(full, first, v), dim3(grid), dim3(wg), 0, s, a);
+}
+
+/* resident blocks per CU of a kernel instance (for a persistent grid) */
+static unsigned blocks_per_cu(bool full, bool first, int v, unsigned wg)
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pick_kernel(full, first, v), (int)wg, 0) !=
+            hipSuccess || nb < 1)
+        nb = 1;
+    return (unsigned)nb;
 }
 
 /* choose segment boundaries from a pilot pass: a boundary follows every step
@@ -925,7 +1058,7 @@ static int choose_bounds(ramp_plan *p, const SegArgs &base, uint64_t chunk, unsi
         a.num_sim = (int)pn;
         launch_fill_u64(s, p->cnt.p, CNT_N);
         unsigned grid = (unsigned)std::max<uint64_t>(1, (pn + wg - 1) / wg);
-        launch_segment<false>(a, true, grid, wg, s);
+        launch_segment(a, false, true, p->variant, grid, wg, s);
         HIPCHK(hipGetLastError());
         std::vector<unsigned long long> cnt(CNT_N);
         HIPCHK(hipMemcpyAsync(cnt.data(), p->cnt.p, CNT_N * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
@@ -1017,43 +1150,50 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
         if (rc) return rc;
     }
     int nseg = (int)p->bounds.size() - 1;
+    /* persistent grids: as many blocks as are co-resident */
+    unsigned grid_first = (unsigned)ctx->n_cu * blocks_per_cu(full, true, p->variant, wg);
+    unsigned grid_rest = (unsigned)ctx->n_cu * blocks_per_cu(full, false, p->variant, wg);
+    uint64_t max_waves = (uint64_t)std::max(grid_first, grid_rest) * (wg / 64);
     if (nseg > 1) {
-        rc = ensure_queues(p, chunk, full);
+        rc = ensure_queues(p, chunk + max_waves, full);
+        if (!rc) rc = dev_alloc(p->rcounts, max_waves * sizeof(uint32_t));
+        if (!rc) rc = dev_alloc(p->prefix, (max_waves + 1) * sizeof(uint64_t));
         if (rc) return rc;
     }
+    uint64_t *meta = (uint64_t *)p->count.p; /* [0..1] producer, [2..3] consumer */
+    a.out_counts = (uint32_t *)p->rcounts.p;
+    a.out_meta = meta;
+    a.in_meta = meta + 2;
+    a.in_prefix = (const uint64_t *)p->prefix.p;
     launch_fill_u64(s, p->cnt.p, CNT_N);
     HIPCHK(hipEventRecord(ctx->ev[0], s));
-    unsigned grid_cap = (unsigned)ctx->n_cu * 16u * (256u / wg);
-    uint32_t *dcount = (uint32_t *)p->count.p;
     uint64_t segs = 0;
     for (uint64_t ck = 0; ck < n_chunks; ck++) {
         uint64_t torun = std::min(chunk, n_hist - ck * chunk);
         a.num_sim = (int)torun;
         a.seed = (int)seeds[ck];
         a.n = torun;
-        launch_fill_u32(s, dcount, 64);
         for (int sg = 0; sg < nseg; sg++) {
             a.s0 = p->bounds[sg];
             a.s1 = p->bounds[sg + 1];
             a.last = (sg == nseg - 1) ? 1 : 0;
             a.pilot = 0;
-            a.in_count = dcount + (sg % 2 == 1 ? 0 : 1) * 16;
-            a.out_count = dcount + (sg % 2 == 0 ? 0 : 1) * 16;
             if (sg >= 1) a.in = queue_view(p, (sg - 1) % 2);
             a.out = queue_view(p, sg % 2);
             unsigned grid;
-            if (sg == 0) grid = (unsigned)std::min<uint64_t>((torun + wg - 1) / wg, grid_cap);
-            else grid = grid_cap;
+            if (sg == 0) grid = (unsigned)std::min<uint64_t>((torun + wg - 1) / wg, grid_first);
+            else grid = grid_rest;
             if (sg == 0 && ck == 0) HIPCHK(hipEventRecord(ctx->ev[2], s));
-            if (full) launch_segment<true>(a, sg == 0, grid, wg, s);
-            else launch_segment<false>(a, sg == 0, grid, wg, s);
+            launch_segment(a, full, sg == 0, p->variant, grid, wg, s);
             if (sg == 0 && ck == 0) HIPCHK(hipEventRecord(ctx->ev[3], s));
             HIPCHK(hipGetLastError());
             segs++;
-            /* the next writer of this queue's counter must start from 0 */
             if (sg + 1 < nseg) {
-                uint32_t *next_out = dcount + (sg % 2 == 0 ? 1 : 0) * 16;
-                launch_fill_u32(s, next_out, 1);
+                uint64_t nreg = (uint64_t)grid * (wg / 64);
+                hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, (const uint32_t *)p->rcounts.p,
+                                   nreg, (const uint64_t *)meta, (uint64_t *)p->prefix.p, meta + 2,
+                                   (unsigned long long *)p->cnt.p, sg == 0 ? 1 : 0);
+                HIPCHK(hipGetLastError());
             }
         }
     }
