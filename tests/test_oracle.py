@@ -1,0 +1,216 @@
+"""The CPU oracle pinned by closed forms of the physics (CPU only).
+
+Each test builds a small instrument, runs it through the oracle and checks
+a quantity that the mathematics fixes independently of the code: chord
+lengths through spheres and cylinders (Beer-Lambert), Bragg angles, guide
+transmission at unit reflectivity, chopper duty cycles, sampled source
+moments, detector bin placement.  A dropped term, a wrong sign or a swapped
+axis in the restated kernels moves these numbers."""
+import json
+import math
+
+import numpy as np
+import pytest
+from scipy import integrate
+
+import ramp_amd
+
+
+def _inst(tmp_path, comps, name="t.json", **kw):
+    d = {"linear": True}
+    d.update(comps)
+    p = tmp_path / name
+    p.write_text(json.dumps({"b": d}))
+    return ramp_amd.Instrument(str(p), **kw)
+
+
+def _src_parallel(w=0.04, h=0.04, lmin=2.0, lmax=2.0000001):
+    return {"source": True, "moderator_kernel": {"name": "MParallel", "width": w, "height": h,
+                                                 "lambda_min": lmin, "lambda_max": lmax}}
+
+
+def _counter(z):
+    return {"position": [0, 0, z], "geom_kernel": {"name": "GPlane", "width": 10.0, "height": 10.0},
+            "scat_kernel": {"name": "SCounter"}}
+
+
+def test_sphere_chord_beer_lambert(tmp_path, oracle):
+    r, mu = 0.01, 1.5  # m, 1/cm
+    inst = _inst(tmp_path, {
+        "src": _src_parallel(w=0.02, h=0.02),
+        "ball": {"position": [0, 0, 1.0], "restore_neutron": True,
+                 "geom_kernel": {"name": "GSphere", "radius": r},
+                 "scat_kernel": {"name": "SAbsorber", "sigma_a": mu}},
+        "cnt": _counter(2.0)})
+    N = 400000
+    res = oracle.run(inst, N, [3])
+    w = res["hist"][0][0].sum()
+    # beam 2r x 2r square: the disk fraction pi/4 passes the ball with the
+    # mean transmission (2/r^2) * [1 - (1 + 2 m r) e^{-2 m r}] / (4 m^2)
+    m = mu * 100.0
+    t_disk = (2 / r**2) * (1 - (1 + 2 * m * r) * math.exp(-2 * m * r)) / (4 * m * m)
+    expect = (math.pi / 4) * t_disk + (1 - math.pi / 4)
+    assert abs(w - expect) < 4 * math.sqrt(expect * (1 - expect) / N) + 1e-4
+
+
+def test_cylinder_chord_beer_lambert(tmp_path, oracle):
+    r, h, mu = 0.005, 0.02, 2.0
+    inst = _inst(tmp_path, {
+        "src": _src_parallel(w=0.01, h=0.01),
+        "can": {"position": [0, 0, 1.0], "restore_neutron": True,
+                "geom_kernel": {"name": "GCylinderExt", "radius": r, "height": h},
+                "scat_kernel": {"name": "SAbsorber", "sigma_a": mu}},
+        "cnt": _counter(2.0)})
+    N = 400000
+    w = oracle.run(inst, N, [5])["hist"][0][0].sum()
+    m = mu * 100.0
+    # beam fully covers the cylinder width in x and lies within its height
+    t, _ = integrate.quad(lambda x: math.exp(-2 * m * math.sqrt(r * r - x * x)), -r, r)
+    expect = t / (2 * r)
+    assert abs(w - expect) < 4 * math.sqrt(expect * (1 - expect) / N) + 1e-4
+
+
+def test_box_beer_lambert_exact(tmp_path, oracle):
+    d, mu = 0.003, 0.7
+    inst = _inst(tmp_path, {
+        "src": _src_parallel(w=0.01, h=0.01),
+        "slab": {"position": [0, 0, 1.0],
+                 "geom_kernel": {"name": "GBox", "width": 0.05, "height": 0.05, "depth": d},
+                 "scat_kernel": {"name": "SAbsorber", "sigma_a": mu}},
+        "cnt": _counter(2.0)})
+    w = oracle.run(inst, 10000, [7])["hist"][0][0].sum()
+    assert w == pytest.approx(math.exp(-mu * 100 * d), rel=1e-5)
+
+
+def test_powder_bragg_angle(tmp_path, oracle):
+    d_sp, lam = 3.0, 1.8
+    E = 81.80420 / lam**2
+    inst = _inst(tmp_path, {
+        "src": {"source": True, "moderator_kernel": {
+            "name": "MFlat", "mod_dim": [0.001, 0.001], "target_dim": [0.001, 0.001],
+            "target_dist": 1.0, "E_min": E, "E_max": E}},
+        "sample": {"position": [0, 0, 1.0], "restore_neutron": True,
+                   "geom_kernel": {"name": "GCylinderExt", "radius": 0.0005, "height": 0.02},
+                   "scat_kernel": {"name": "SPowder1", "d_spacing": d_sp, "pack": 1.0, "vc": 75.0,
+                                   "sigma_abs": 0.0, "multiplicity": 1, "DW": 1.0, "F2": 500.0,
+                                   "d_phi": 3.0}},
+        "det": {"position": [0, 0, 1.0],
+                "geom_kernel": {"name": "GBanana", "radius": 1.0, "height": 0.5,
+                                "mintheta": -40.0, "maxtheta": 140.0},
+                "scat_kernel": {"name": "SDetector1D", "var": "theta",
+                                "binning": [-180.0, 0.05, 180.0]}}})
+    w = oracle.run(inst, 200000, [9])["hist"][0][0]
+    axis = -180.0 + 0.05 * (np.arange(w.size) + 0.5)
+    peak = np.sum(np.abs(axis) * w) / w.sum()  # scattering goes to both sides
+    expect = math.degrees(2 * math.asin(lam / (2 * d_sp)))
+    assert abs(peak - expect) < 0.05
+
+
+def test_guide_unit_reflectivity_transmits_everything(tmp_path, oracle):
+    # R0 = 1 with Qc far above every q: every reflection keeps the weight,
+    # |v| is conserved, and every neutron entering the guide leaves it
+    inst = _inst(tmp_path, {
+        "src": {"source": True, "moderator_kernel": {
+            "name": "MFlat", "mod_dim": [0.02, 0.02], "target_dim": [0.02, 0.02],
+            "target_dist": 0.5, "E_min": 5.0, "E_max": 5.0}},
+        "g": {"position": [0, 0, 0.5],
+              "geom_kernel": {"name": "GPlane", "width": 0.03, "height": 0.03},
+              "scat_kernel": {"name": "SGuide", "w1": 0.03, "h1": 0.03, "w2": 0.01, "h2": 0.01,
+                              "l": 5.0, "R0": 1.0, "Qc": 10.0, "alpha": 1.0, "m": 1.0, "W": 0.01}},
+        "mon": {"position": [0, 0, 5.6],
+                "geom_kernel": {"name": "GPlane", "width": 0.2, "height": 0.2},
+                "scat_kernel": {"name": "SDetector1D", "var": "energy", "binning": [4.905, 0.01, 5.105]}}})
+    N = 50000
+    res = oracle.run(inst, N, [11], keep_state=True)
+    n16 = res["state"][0]
+    w = res["hist"][0][0]
+    assert w.sum() == pytest.approx(1.0, rel=1e-5)          # all N reach the monitor, w = 1/N
+    # energy conserved by the reflections: one bin holds everything
+    assert (w > 0).sum() == 1
+    E = 5.22703725e-6 * (n16[:, 3:6].astype(np.float64) ** 2).sum(axis=1)
+    assert np.allclose(E, 5.0, rtol=1e-5)
+
+
+def test_chopper_duty_cycle(tmp_path, oracle):
+    # arrival times spread over ~20 chopper periods: the transmitted fraction
+    # is the open angle over the slit period, width/Tg with
+    # width = atan2(slit, R) for rays near x = 0
+    R, slit, ns, f = 0.3, 0.03, 2, 5000.0
+    inst = _inst(tmp_path, {
+        "src": {"source": True, "moderator_kernel": {
+            "name": "MFlat", "mod_dim": [0.0002, 0.0002], "target_dim": [0.0002, 0.0002],
+            "target_dist": 1.0, "E_min": 3.0, "E_max": 4.0}},
+        "ch": {"position": [0, 0, 10.0], "geom_kernel": {"name": "GPlane", "width": 0.5, "height": 0.5},
+               "scat_kernel": {"name": "SChopper", "radius": R, "freq": 2 * math.pi * f,
+                               "n_slits": ns, "jitter": 0.0, "slit_width": slit, "phase": 0.0}},
+        "cnt": _counter(10.5)})
+    N = 1000000
+    frac = oracle.run(inst, N, [13])["hist"][0][0].sum()
+    expect = math.atan2(slit, R) / (2 * math.pi / ns)
+    assert abs(frac - expect) < 4 * math.sqrt(expect / N) + 0.005 * expect
+
+
+def test_flat_source_energy_moments(tmp_path, oracle):
+    E0, E1 = 2.0, 20.0
+    inst = _inst(tmp_path, {
+        "src": {"source": True, "moderator_kernel": {
+            "name": "MFlat", "mod_dim": [0.01, 0.01], "target_dim": [0.01, 0.01],
+            "target_dist": 1.0, "E_min": E0, "E_max": E1}},
+        "mon": {"position": [0, 0, 0.5], "geom_kernel": {"name": "GPlane", "width": 1.0, "height": 1.0},
+                "scat_kernel": {"name": "SDetector1D", "var": "energy", "binning": [0.0, 0.05, 25.0]}}})
+    N = 300000
+    w = oracle.run(inst, N, [17])["hist"][0][0]
+    assert w.sum() == pytest.approx(1.0, rel=1e-5)
+    axis = 0.05 * (np.arange(w.size) + 0.5)
+    mean = np.sum(axis * w) / w.sum()
+    var = np.sum((axis - mean) ** 2 * w) / w.sum()
+    assert abs(mean - (E0 + E1) / 2) < 4 * (E1 - E0) / math.sqrt(12 * N) + 0.01
+    assert var == pytest.approx((E1 - E0) ** 2 / 12, rel=0.02)
+
+
+def test_lambda_source_unit_expected_weight(tmp_path, oracle):
+    inst = _inst(tmp_path, {
+        "src": {"source": True, "moderator_kernel": {
+            "name": "MLambda", "mod_dim": [0.01, 0.01], "target_dim": [0.01, 0.01],
+            "target_dist": 1.0, "lambda_min": 1.0, "lambda_max": 10.0}},
+        "mon": {"position": [0, 0, 0.5], "geom_kernel": {"name": "GPlane", "width": 1.0, "height": 1.0},
+                "scat_kernel": {"name": "SDetector1D", "var": "wavelength", "binning": [0.0, 0.1, 11.0]}}})
+    N = 300000
+    w = oracle.run(inst, N, [19])["hist"][0][0]
+    assert w.sum() == pytest.approx(1.0, abs=0.01)
+    # weight per unit wavelength proportional to 1/lambda
+    axis = 0.1 * (np.arange(w.size) + 0.5)
+    sel = (axis > 1.2) & (axis < 9.8)
+    ratio = (w[sel] * axis[sel]).std() / (w[sel] * axis[sel]).mean()
+    assert ratio < 0.05
+
+
+def test_detector1d_binning_floor(tmp_path, oracle):
+    # a monochromatic beam of energy E lands in bin floor((E - min)/step)
+    E = 7.3
+    inst = _inst(tmp_path, {
+        "src": {"source": True, "moderator_kernel": {
+            "name": "MFlat", "mod_dim": [0.01, 0.01], "target_dim": [0.01, 0.01],
+            "target_dist": 1.0, "E_min": E, "E_max": E}},
+        "mon": {"position": [0, 0, 0.5], "geom_kernel": {"name": "GPlane", "width": 1.0, "height": 1.0},
+                "scat_kernel": {"name": "SDetector1D", "var": "energy", "binning": [1.0, 0.5, 11.0]}}})
+    w = oracle.run(inst, 1000, [23])["hist"][0][0]
+    assert np.flatnonzero(w).tolist() == [int((E - 1.0) // 0.5)]
+
+
+def test_incoherent_sample_weight(tmp_path, oracle):
+    # forced single scattering: the scattered weight is
+    # p_int * Sigma_s/Sigma with p_int = 1 - exp(-Sigma L) per neutron;
+    # for a thin slab at normal incidence L = depth for every neutron
+    d, ss, sa = 0.002, 0.5, 0.3
+    inst = _inst(tmp_path, {
+        "src": _src_parallel(w=0.01, h=0.01, lmin=1.8, lmax=1.8000001),
+        "slab": {"position": [0, 0, 1.0],
+                 "geom_kernel": {"name": "GBox", "width": 0.05, "height": 0.05, "depth": d},
+                 "scat_kernel": {"name": "SIncoherent", "sigma_s": ss, "sigma_a": sa, "lambda_ref": 1.8}},
+        "sphere": {"position": [0, 0, 1.0], "geom_kernel": {"name": "GSphereInt", "radius": 5.0},
+                   "scat_kernel": {"name": "SCounter"}}})
+    w = oracle.run(inst, 20000, [29])["hist"][0][0].sum()
+    S = (ss + sa) * 100
+    expect = (1 - math.exp(-S * d)) * ss / (ss + sa)
+    assert w == pytest.approx(expect, rel=1e-4)
