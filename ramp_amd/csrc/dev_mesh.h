@@ -2,7 +2,7 @@
  *
  * The reference tests the ray against every triangle.  Here the triangles
  * are visited through a bounding-volume hierarchy built on the GPU
- * (bvh.hip): a node is skipped only when the ray provably cannot hit any
+ * (bvh_build.h): a node is skipped only when the ray provably cannot hit any
  * of its triangles with t > eps (slab test against the node box, widened
  * by a relative margin so that no rounding can reject a triangle the
  * brute-force loop would accept).  Every triangle that is visited runs the
@@ -99,7 +99,9 @@ DEV void g_mesh(NS &n, const DPart &g, uint32_t ci)
         int stack[64];
         int sp = 0;
         stack[sp++] = 0;
-        while (sp > 0) {
+        /* each of the 2n-1 nodes is visited at most once */
+        uint32_t budget = 2u * ntri + 1u;
+        while (sp > 0 && budget-- > 0u) {
             const BvhNode &nd = nodes[stack[--sp]];
             float lim = interior ? 3.0e38f : rm_fmax(out[3], out[7]) * 1.0001f + 1e-30f;
             if (!bvh_box_hit(nd, pos, inv, lim)) continue;

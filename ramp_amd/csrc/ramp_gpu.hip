@@ -27,6 +27,7 @@
 
 #include "../../include/ramp.h"
 #include "dev_components.h"
+#include "bvh_build.h"
 
 /* ======================================================================== */
 /* error handling                                                           */
@@ -627,98 +628,70 @@ static void derive_scat(DPart &s)
     }
 }
 
-/* ---- BVH over the triangles of a mesh (host build; median split) -------- */
-struct BvhBuild {
-    const float *pts;
-    std::vector<BvhNode> nodes;
-    std::vector<uint32_t> order;
-    std::vector<float> cen;
-};
-
-static void tri_bounds(const float *P, float lo[3], float hi[3])
+/* ---- BVH over the triangles of a mesh, built on the GPU (bvh_build.h) ---- */
+__global__ void k_fill_i32(int *p, uint64_t n, int v)
 {
-    for (int a = 0; a < 3; a++) {
-        lo[a] = std::min(P[a], std::min(P[3 + a], P[6 + a]));
-        hi[a] = std::max(P[a], std::max(P[3 + a], P[6 + a]));
-    }
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
 }
 
-static int bvh_rec(BvhBuild &b, uint32_t first, uint32_t count, int depth)
+static int build_mesh(ramp_plan *p, DPart &g, const float *dpts, uint32_t ntri)
 {
-    int idx = (int)b.nodes.size();
-    b.nodes.push_back(BvhNode());
-    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (uint32_t k = first; k < first + count; k++) {
-        uint32_t t = b.order[k];
-        float tl[3], th[3];
-        tri_bounds(b.pts + 9 * (size_t)t, tl, th);
-        for (int a = 0; a < 3; a++) {
-            lo[a] = std::min(lo[a], tl[a]);
-            hi[a] = std::max(hi[a], th[a]);
-            clo[a] = std::min(clo[a], b.cen[3 * t + a]);
-            chi[a] = std::max(chi[a], b.cen[3 * t + a]);
-        }
-    }
-    BvhNode nd;
-    std::memset(&nd, 0, sizeof nd);
+    g.aux = nullptr;
+    g.aux2 = nullptr;
+    if (ntri == 0) return 0;
+    hipStream_t s = p->ctx->stream;
+    int n = (int)ntri;
+    DevBuf keys, vals, skeys, order, nodes, parent, flags, temp;
+    int rc = dev_alloc(keys, n * 4ull);
+    if (!rc) rc = dev_alloc(vals, n * 4ull);
+    if (!rc) rc = dev_alloc(skeys, n * 4ull);
+    if (!rc) rc = dev_alloc(order, n * 4ull);
+    if (!rc) rc = dev_alloc(nodes, (2ull * n - 1) * sizeof(BvhNode));
+    if (!rc) rc = dev_alloc(parent, (2ull * n - 1) * 4ull);
+    if (!rc) rc = dev_alloc(flags, (uint64_t)std::max(1, n - 1) * 4ull);
+    auto cleanup = [&]() {
+        dev_free(keys); dev_free(vals); dev_free(skeys); dev_free(parent); dev_free(flags); dev_free(temp);
+    };
+    if (rc) { cleanup(); dev_free(order); dev_free(nodes); return rc; }
+    float lo[3] = {g.p[GP_XMIN], g.p[GP_YMIN], g.p[GP_ZMIN]};
+    float hi[3] = {g.p[GP_XMAX], g.p[GP_YMAX], g.p[GP_ZMAX]};
+    float ext = 0.0f, ie[3];
     for (int a = 0; a < 3; a++) {
-        float ext = hi[a] - lo[a];
-        float mag = std::max(std::fabs(lo[a]), std::fabs(hi[a]));
-        float margin = 1e-4f * ext + 1e-5f * mag + 1e-9f;
-        nd.lo[a] = lo[a] - margin;
-        nd.hi[a] = hi[a] + margin;
+        float e = hi[a] - lo[a];
+        ext = std::max(ext, e);
+        ie[a] = e > 0.0f ? 1.0f / e : 0.0f;
     }
-    if (count <= 4 || depth > 48) {
-        nd.left = -1;
-        nd.right = (int32_t)first;
-        nd.count = (int32_t)count;
-        b.nodes[idx] = nd;
-        return idx;
+    float margin = std::max(1e-3f * ext, 1e-6f);
+    unsigned b = 256, gblk = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_bvh_morton, dim3(gblk), dim3(b), 0, s, dpts, ntri, make_float3(lo[0], lo[1], lo[2]),
+                       make_float3(ie[0], ie[1], ie[2]), (uint32_t *)keys.p, (uint32_t *)vals.p);
+    size_t tbytes = 0;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tbytes, (uint32_t *)keys.p, (uint32_t *)skeys.p,
+                                              (uint32_t *)vals.p, (uint32_t *)order.p, n, 0, 30, s));
+    rc = dev_alloc(temp, tbytes + 16);
+    if (rc) { cleanup(); dev_free(order); dev_free(nodes); return rc; }
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(temp.p, tbytes, (uint32_t *)keys.p, (uint32_t *)skeys.p,
+                                              (uint32_t *)vals.p, (uint32_t *)order.p, n, 0, 30, s));
+    hipLaunchKernelGGL(k_fill_i32, dim3(64), dim3(256), 0, s, (int *)parent.p, 2ull * n - 1, -1);
+    hipLaunchKernelGGL(k_fill_i32, dim3(64), dim3(256), 0, s, (int *)flags.p, (uint64_t)std::max(1, n - 1), 0);
+    if (n > 1)
+        hipLaunchKernelGGL(k_bvh_tree, dim3((n - 1 + 255) / 256), dim3(256), 0, s, (const uint32_t *)skeys.p,
+                           n, (BvhNode *)nodes.p, (int *)parent.p);
+    hipLaunchKernelGGL(k_bvh_boxes, dim3(gblk), dim3(b), 0, s, dpts, (const uint32_t *)order.p, n, margin,
+                       (BvhNode *)nodes.p, (const int *)parent.p, (int *)flags.p);
+    hipError_t e = hipStreamSynchronize(s);
+    cleanup();
+    if (e != hipSuccess) {
+        dev_free(order); dev_free(nodes);
+        return fail(RAMP_E_DEVICE, "BVH build: %s", hipGetErrorString(e));
     }
-    int ax = 0;
-    for (int a = 1; a < 3; a++)
-        if (chi[a] - clo[a] > chi[ax] - clo[ax]) ax = a;
-    uint32_t mid = first + count / 2;
-    std::nth_element(b.order.begin() + first, b.order.begin() + mid, b.order.begin() + first + count,
-                     [&](uint32_t x, uint32_t y) { return b.cen[3 * x + ax] < b.cen[3 * y + ax]; });
-    int l = bvh_rec(b, first, mid - first, depth + 1);
-    int r = bvh_rec(b, mid, first + count - mid, depth + 1);
-    nd.left = l;
-    nd.right = r;
-    nd.count = 0;
-    b.nodes[idx] = nd;
-    return idx;
-}
-
-static int build_mesh(ramp_plan *p, DPart &g, const float *pts, uint32_t ntri)
-{
-    BvhBuild b;
-    b.pts = pts;
-    b.order.resize(ntri);
-    b.cen.resize(3 * (size_t)ntri);
-    for (uint32_t t = 0; t < ntri; t++) {
-        b.order[t] = t;
-        for (int a = 0; a < 3; a++)
-            b.cen[3 * t + a] = (pts[9 * t + a] + pts[9 * t + 3 + a] + pts[9 * t + 6 + a]) / 3.0f;
-    }
-    if (ntri == 0) {
-        g.aux = nullptr;
-        g.aux2 = nullptr;
-        return 0;
-    }
-    bvh_rec(b, 0, ntri, 0);
-    DevBuf nb, ob;
-    int rc = dev_alloc(nb, b.nodes.size() * sizeof(BvhNode));
-    if (rc) return rc;
-    rc = dev_alloc(ob, b.order.size() * sizeof(uint32_t));
-    if (rc) return rc;
-    HIPCHK(hipMemcpy(nb.p, b.nodes.data(), b.nodes.size() * sizeof(BvhNode), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(ob.p, b.order.data(), b.order.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-    g.aux = nb.p;
-    g.aux2 = ob.p;
-    p->mesh_nodes.push_back(nb);
-    p->mesh_order.push_back(ob);
+    /* node 0 is the root (for one triangle, the leaf itself) */
+    g.aux = nodes.p;
+    g.aux2 = order.p;
+    p->mesh_nodes.push_back(nodes);
+    p->mesh_order.push_back(order);
     return 0;
 }
 
@@ -838,7 +811,7 @@ extern "C" int ramp_plan_create(ramp_ctx *ctx, const ramp_plan_desc *d, ramp_pla
         derive_geom(dc.g);
         derive_scat(dc.s);
         if (c.geom.type == RAMP_G_MESH)
-            PCHK(build_mesh(p, dc.g, d->tables + c.geom.tab_off[GT_POINTS], c.geom.ni[GI_NTRI]));
+            PCHK(build_mesh(p, dc.g, dtab + c.geom.tab_off[GT_POINTS], c.geom.ni[GI_NTRI]));
         dc.comp_idx = c.comp_idx;
         dc.restore = c.restore;
         dc.det = c.det;

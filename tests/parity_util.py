@@ -7,9 +7,34 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
+_MESH_DIR = None
+
+
+def mesh_files():
+    """Small noisy icospheres written once per process to a temp dir."""
+    global _MESH_DIR
+    import tempfile
+    from ramp_amd import meshes
+    from ramp_amd.components import write_stl_binary
+    if _MESH_DIR is None:
+        _MESH_DIR = tempfile.mkdtemp(prefix="ramp_mesh_")
+        write_stl_binary(os.path.join(_MESH_DIR, "ico12.stl"),
+                         meshes.icosphere(12, radius=0.015, sigma=0.001, seed=3))
+        write_stl_binary(os.path.join(_MESH_DIR, "inner.stl"),
+                         meshes.icosphere(6, radius=0.008, sigma=0.0005, seed=4))
+        write_stl_binary(os.path.join(_MESH_DIR, "outer.stl"),
+                         meshes.icosphere(8, radius=0.012, sigma=0.0005, seed=5))
+    return {k: os.path.join(_MESH_DIR, k + ".stl") for k in ("ico12", "inner", "outer")}
+
+
 def instrument_cases():
     ang = float(np.arcsin((9.045 / np.sqrt(5.0)) / (2 * 3.355)))
+    m = mesh_files()
     return {
+        "imaging": (os.path.join(ROOT, "ramp_amd", "instruments", "imaging.json"),
+                    {"mesh": repr(m["ico12"])}),
+        "mesh_multi": (os.path.join(ROOT, "tests", "instruments", "mesh_multi.json"),
+                       {"inner": repr(m["inner"]), "outer": repr(m["outer"])}),
         "minimal": (os.path.join(ROOT, "ramp_amd", "instruments", "minimal.json"), {}),
         "guide_chopper": (os.path.join(ROOT, "tests", "instruments", "guide_chopper.json"),
                           {"v_foc": 841.3}),

@@ -12,7 +12,7 @@ from tests.parity_util import compare_states, instrument_cases, run_gpu, run_ora
 
 CASES = instrument_cases()
 # allowed rows differing beyond 1e-5 (out of N) per case
-BUDGET = {"minimal": 0, "guide_chopper": 0, "powder": 0, "tas": 8}
+BUDGET = {"minimal": 0, "guide_chopper": 0, "powder": 0, "tas": 8, "imaging": 0, "mesh_multi": 0}
 N = 65536
 
 
@@ -71,3 +71,22 @@ def test_atomic_mode_matches_oracle():
     _, gh, _ = run_gpu(path, kw, N, [5], hist_mode="atomic", keep_state=False)
     o = run_oracle(path, kw, N, [5])
     _hist_close(gh, o["hist"], 1e-5)
+
+
+@pytest.mark.gpu
+def test_full_size_mesh_bvh_equals_brute_force(tmp_path):
+    # the 50,000-triangle noisy icosphere of the imaging workload: the GPU
+    # BVH traversal must give the reference brute-force hits (oracle)
+    from ramp_amd import meshes
+    from ramp_amd.components import write_stl_binary
+    mesh = tmp_path / "ico50.stl"
+    write_stl_binary(str(mesh), meshes.icosphere(50, radius=0.015, sigma=0.001, seed=3))
+    path, _ = CASES["imaging"]
+    kw = {"mesh": repr(str(mesh))}
+    n = 16384
+    print(f"imaging 50k triangles N={n}", flush=True)
+    g, gh, _ = run_gpu(path, kw, n, [77])
+    o = run_oracle(path, kw, n, [77])
+    d = compare_states(g, o["state"])
+    assert d["dead"] == 0 and d["iidx"] == 0 and d["beyond_rows"] == 0 and d["bitdiff_rows"] == 0
+    _hist_close(gh, o["hist"], 1e-6)
