@@ -1,12 +1,16 @@
-"""Benchmark: neutron histories/s of the LET-like spectrometer on one MI355X.
+"""Benchmark: neutron histories/s of the RAMP ray-tracing path on one MI355X.
 
-A step is one run of 1e8 histories (one chunk) of the LET-like direct-
-geometry chopper spectrometer (ramp_amd/instruments/let_like.json): Maxwellian
-source with a 1/E tail, 20-segment tapered m=2 guide, 5 disk choppers,
-incoherent sample and a 768-tube x 128-pixel x 1000-TOF detector bank.  The
-histories are generated on the device by the source component (synthetic
-data); histograms accumulate in HBM and are not read back inside the timed
-region.
+The default workload (the headline) is the LET-like direct-geometry chopper
+spectrometer (ramp_amd/instruments/let_like.json), one step = one chunk of
+1e8 histories: Maxwellian source with a 1/E tail, 20-segment tapered m=2
+guide, 5 disk choppers, incoherent sample and a 768-tube x 128-pixel x
+1000-TOF detector bank.  `--workload imaging` runs the mesh-imaging
+instrument (parallel beam, 50,000-triangle noisy icosphere absorber traced
+through the GPU-built BVH, 2048x2048 panel; 1e8 histories per step) and
+`--workload minimal` the minimal instrument (flat source, slit, incoherent
+cylinder, 256x512 banana detector; 1e6 histories per step).  Histories are
+generated on the device by the source component (synthetic data);
+histograms accumulate in HBM and are not read back inside the timed region.
 
 Prints one JSON line (rank 0, N = 1).  The CPU baseline is the project's
 C/OpenMP restatement of the reference kernels (oracle/) timed on the host's
@@ -23,11 +27,28 @@ sys.path.insert(0, ROOT)
 
 METRIC = "neutron histories/s, 1 MI355X (LET-like 1e8); batched-scan & fwd+adjoint n/s"
 HBM_PEAK_GBS = 8000.0
-# gfx950 vector issue: 256 CUs x 64 lanes x 2.4 GHz = 39.3e12 32-bit lane-ops/s
+# gfx950 vector issue: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz = 39.3e12 32-bit
+# lane-ops/s, i.e. one wave64 VALU instruction per CU per clock
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
-# VALU instructions (wave64) per history of the source segment, measured with
-# rocprofv3 --pmc SQ_INSTS_VALU (profiles/let_like_source_segment_pmc.txt)
-VALU_INSTS_PER_HISTORY = None
+
+# Per-history counts of the source-segment kernel (the dominant one), from
+# rocprofv3 --pmc runs of this script (files under profiles/):
+#   valu  = SQ_INSTS_VALU / histories (wave64 instructions, so x64 lane-ops)
+#   fetch = FETCH_SIZE / histories, write = WRITE_SIZE / histories (bytes,
+#           after the plain-copy calibration of the counters)
+PMC_PER_HISTORY = {
+    "let_like": None,
+    "imaging": None,
+    "minimal": None,
+}
+
+WORKLOADS = {
+    "let_like": dict(file="let_like.json", histories=1e8, name="let_like_1e8",
+                     bins=768 * 128 * 1000),
+    "imaging": dict(file="imaging.json", histories=1e8, name="imaging_mesh50k_1e8",
+                    bins=2048 * 2048),
+    "minimal": dict(file="minimal.json", histories=1e6, name="minimal_1e6", bins=256 * 512),
+}
 
 
 def parse():
@@ -35,7 +56,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--histories", type=float, default=1e8)
+    ap.add_argument("--workload", default="let_like", choices=sorted(WORKLOADS))
+    ap.add_argument("--histories", type=float, default=None)
     ap.add_argument("--hist-mode", default="atomic", choices=["atomic", "deterministic"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -44,23 +66,40 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(path, target_s):
+def instrument_args(workload, tmpdir):
+    """Keyword arguments of the workload's instrument file; the imaging
+    object (50,000-triangle icosphere, r = 15 mm, sigma = 1 mm, seed 3) is
+    written as binary STL at run time."""
+    if workload != "imaging":
+        return {}
+    from ramp_amd import meshes
+    from ramp_amd.components import write_stl_binary
+    path = os.path.join(tmpdir, "icosphere50k.stl")
+    write_stl_binary(path, meshes.icosphere(50, radius=0.015, sigma=0.001, seed=3))
+    return {"mesh": repr(path)}
+
+
+def cpu_baseline(path, kw, target_s, label):
     """Oracle (C/OpenMP restatement of the reference kernels) on host cores."""
-    import numpy as np
     import ramp_amd
     from oracle import pyoracle
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    inst = ramp_amd.Instrument(path)
-    n0 = 1_000_000
-    t = time.perf_counter()
-    pyoracle.run(inst, n0, [1], threads=threads)
-    r0 = n0 / (time.perf_counter() - t)
-    n = int(min(max(r0 * target_s, 1e6), 2e9))
+    inst = ramp_amd.Instrument(path, **kw)
+    n0 = 20_000
+    while True:
+        t = time.perf_counter()
+        pyoracle.run(inst, n0, [1], threads=threads)
+        dt0 = time.perf_counter() - t
+        if dt0 > 0.5 or n0 >= 4_000_000:
+            break
+        n0 *= 4
+    r0 = n0 / dt0
+    n = int(min(max(r0 * target_s, 1e4), 2e9))
     t = time.perf_counter()
     pyoracle.run(inst, n, [2], threads=threads)
     dt = time.perf_counter() - t
     return {"value": n / dt, "unit": "histories/s", "cores": threads, "kind": "port",
-            "sample": f"{n} histories of the same LET-like instrument, one chunk, {dt:.1f} s"}
+            "sample": f"{n} histories of the same {label} instrument, one chunk, {dt:.1f} s"}
 
 
 def main():
@@ -69,6 +108,7 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the path shards by "
               "history range across GPUs, which this build does not launch", file=sys.stderr)
         sys.exit(2)
+    import tempfile
     import ramp_amd
     try:
         import torch
@@ -82,9 +122,12 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    path = os.path.join(ROOT, "ramp_amd", "instruments", "let_like.json")
-    N = int(a.histories)
-    inst = ramp_amd.Instrument(path, hist_mode=a.hist_mode, chunk=N)
+    wl = WORKLOADS[a.workload]
+    tmpdir = tempfile.mkdtemp(prefix="ramp_bench_")
+    kw = instrument_args(a.workload, tmpdir)
+    path = os.path.join(ROOT, "ramp_amd", "instruments", wl["file"])
+    N = int(a.histories if a.histories else wl["histories"])
+    inst = ramp_amd.Instrument(path, hist_mode=a.hist_mode, chunk=N, **kw)
     seed = 20261203
     for w in range(a.warmup):
         inst.execute(N, seeds=[(seed + w) % 4097])
@@ -112,13 +155,14 @@ def main():
     roof = {"bound": "valu", "kernel": "k_segment<false,true> (source segment)",
             "hbm": {"achieved": hbm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": hbm_gbs / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": bytes_seg0},
-            "traffic": None, "launch_ms": seg_avg_ms}
-    if VALU_INSTS_PER_HISTORY:
-        ach = VALU_INSTS_PER_HISTORY * 64 * N / (seg_avg_ms * 1e-3) / 1e12
-        roof.update({"achieved": ach, "peak": VALU_PEAK_TOPS, "unit": "TOP/s",
-                     "frac": ach / VALU_PEAK_TOPS})
-    else:
-        roof.update({"achieved": None, "peak": VALU_PEAK_TOPS, "unit": "TOP/s", "frac": None})
+            "traffic": None, "launch_ms": seg_avg_ms,
+            "achieved": None, "peak": VALU_PEAK_TOPS, "unit": "TOP/s", "frac": None}
+    pmc = PMC_PER_HISTORY.get(a.workload)
+    if pmc and seg_avg_ms > 0:
+        ach = pmc["valu"] * 64 * N / (seg_avg_ms * 1e-3) / 1e12
+        roof.update({"achieved": ach, "frac": ach / VALU_PEAK_TOPS,
+                     "valu_insts_per_history": pmc["valu"],
+                     "traffic": (pmc["fetch"] + pmc["write"]) * N})
     out = {
         "metric": METRIC,
         "value": N * K / dt,
@@ -132,14 +176,14 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic: histories generated on the device by the source component",
-        "config": {"workload": "let_like_1e8", "histories_per_step": N, "chunk": N,
-                   "hist_mode": a.hist_mode, "instrument": "ramp_amd/instruments/let_like.json",
-                   "detector_bins": 768 * 128 * 1000, "events_per_step": ev / K,
+        "config": {"workload": wl["name"], "histories_per_step": N, "chunk": N,
+                   "hist_mode": a.hist_mode, "instrument": "ramp_amd/instruments/" + wl["file"],
+                   "detector_bins": wl["bins"], "events_per_step": ev / K,
                    "queue_in_per_step": qin / K, "queue_out_per_step": qout / K},
         "roofline": roof,
     }
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(path, a.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(path, kw, a.cpu_seconds, wl["name"])
         out["cpu_baseline"]["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
     inst.close()
     print(json.dumps(out), flush=True)

@@ -124,6 +124,10 @@ typedef struct {
 /* ---- run options -------------------------------------------------------- */
 #define RAMP_HIST_ATOMIC 0        /* float atomics, order-dependent sums */
 #define RAMP_HIST_DETERMINISTIC 1 /* int64 fixed-point sums, order-independent */
+/* run flags */
+#define RAMP_RUN_MESH_BRUTE_FORCE 1  /* test every triangle of every GMesh (the
+                                        reference's loop, mesh.cl:52-112) instead
+                                        of traversing the BVH; for checking the BVH */
 
 typedef struct {
     uint32_t hist_mode;      /* RAMP_HIST_* */
@@ -131,7 +135,8 @@ typedef struct {
     uint32_t wg;             /* workgroup size (0: default) */
     uint32_t npt;            /* neutrons per thread in the source segment (0: default) */
     uint32_t max_segments;   /* compaction boundaries allowed (0: automatic) */
-    uint32_t pad[3];
+    uint32_t flags;          /* RAMP_RUN_* bits */
+    uint32_t pad[2];
     double weight_bound;     /* deterministic mode: bound on one event's weight (0: derive) */
 } ramp_run_opts;
 

@@ -92,7 +92,8 @@ class RunOpts(C.Structure):
         ("wg", C.c_uint32),
         ("npt", C.c_uint32),
         ("max_segments", C.c_uint32),
-        ("pad", C.c_uint32 * 3),
+        ("flags", C.c_uint32),
+        ("pad", C.c_uint32 * 2),
         ("weight_bound", C.c_double),
     ]
 

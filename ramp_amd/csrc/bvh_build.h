@@ -5,9 +5,17 @@
  * centroids in the mesh's bounding box, radix sort of (code, index) pairs
  * (hipcub), one thread per internal node determines its key range and split
  * from longest common prefixes (equal codes ordered by sorted position),
- * then leaf boxes propagate to the root bottom-up with one atomic flag per
- * internal node.  Internal nodes are 0..n-2, leaf k is node n-1+k and refers
- * to sorted position k; order[k] is the triangle index.
+ * then leaf boxes and subtree sizes propagate to the root bottom-up with one
+ * atomic flag per internal node.  In that build layout internal nodes are
+ * 0..n-2, leaf k is node n-1+k and refers to sorted position k; order[k] is
+ * the triangle index.
+ *
+ * A last pass writes the tree in depth-first preorder as 32-byte BvhFlat
+ * nodes: the left child of an internal node is the next node, and `esc` is
+ * the node after the subtree (preorder index + subtree size), so traversal
+ * needs neither child pointers nor a stack.  A node's preorder index is its
+ * depth plus, over its ancestors whose right subtree holds it, the sizes of
+ * their left subtrees.
  *
  * Every leaf box is the triangle's box widened by a margin of 1e-3 of the
  * mesh extent (and at least 1e-6 m), far above the rounding of both the
@@ -98,7 +106,7 @@ __global__ void k_bvh_boxes(const float *pts, const uint32_t *order, int n, floa
     }
     lf.left = -1;
     lf.right = k;
-    lf.count = 1;
+    lf.count = 1;               /* subtree size */
     if (n == 1) return;
     __threadfence();
     int p = parent[node];
@@ -111,6 +119,9 @@ __global__ void k_bvh_boxes(const float *pts, const uint32_t *order, int n, floa
          * children through agent-scope atomic loads, not the CU cache */
         int cl = __hip_atomic_load(&nodes[p].left, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         int cr = __hip_atomic_load(&nodes[p].right, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int sl = __hip_atomic_load(&nodes[cl].count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int sr = __hip_atomic_load(&nodes[cr].count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        nodes[p].count = sl + sr + 1;
         for (int ax = 0; ax < 3; ax++) {
             float alo = __hip_atomic_load(&nodes[cl].lo[ax], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             float blo = __hip_atomic_load(&nodes[cr].lo[ax], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -122,6 +133,31 @@ __global__ void k_bvh_boxes(const float *pts, const uint32_t *order, int n, floa
         __threadfence();
         p = parent[p];
     }
+}
+
+/* Preorder flattening, one thread per build-layout node (2n-1 of them).
+ * Launched after k_bvh_boxes has completed, so plain loads see its writes. */
+__global__ void k_bvh_flatten(const BvhNode *nodes, const int *parent, const uint32_t *order, int n,
+                              BvhFlat *flat)
+{
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= 2 * n - 1) return;
+    int pre = 0;
+    int y = x;
+    /* depth is at most 30 Morton bits + 32 index bits of splits */
+    for (int guard = 0; guard < 96; guard++) {
+        int p = parent[y];
+        if (p < 0) break;
+        int l = nodes[p].left;
+        pre += (y == l) ? 1 : 1 + nodes[l].count;
+        y = p;
+    }
+    const BvhNode &nd = nodes[x];
+    BvhFlat f;
+    for (int a = 0; a < 3; a++) { f.lo[a] = nd.lo[a]; f.hi[a] = nd.hi[a]; }
+    f.esc = pre + nd.count;
+    f.tri = (nd.left < 0) ? (int32_t)order[nd.right] : -1;
+    flat[pre] = f;
 }
 
 #endif

@@ -2,14 +2,17 @@
  *
  * The reference tests the ray against every triangle.  Here the triangles
  * are visited through a bounding-volume hierarchy built on the GPU
- * (bvh_build.h): a node is skipped only when the ray provably cannot hit any
- * of its triangles with t > eps (slab test against the node box, widened
- * by a relative margin so that no rounding can reject a triangle the
- * brute-force loop would accept).  Every triangle that is visited runs the
- * reference's exact Moller-Trumbore arithmetic; ties in t are broken by
- * the lowest triangle index, as the reference's in-order strict '<' does,
- * and the crossing count for 'interior' meshes counts every triangle with
- * t > eps, so results equal the brute-force loop.
+ * (bvh_build.h) and stored in depth-first preorder, traversed without a
+ * stack: on a box hit an internal node continues with the next node (its
+ * left child), a leaf tests its triangle; a miss or a finished leaf jumps to
+ * the node after the subtree.  A node is skipped only when the ray provably
+ * cannot hit any of its triangles with t > eps (slab test against the node
+ * box, widened by a relative margin so that no rounding can reject a
+ * triangle the brute-force loop would accept).  Every triangle that is
+ * visited runs the reference's exact Moller-Trumbore arithmetic; ties in t
+ * are broken by the lowest triangle index, as the reference's in-order
+ * strict '<' does, and the crossing count for 'interior' meshes counts every
+ * triangle with t > eps, so results equal the brute-force loop.
  */
 #ifndef RAMP_DEV_MESH_H
 #define RAMP_DEV_MESH_H
@@ -21,6 +24,15 @@ struct BvhNode {
     int32_t right;   /* inner: right child; leaf: first primitive */
     int32_t count;   /* leaf: number of primitives */
     int32_t pad[3];
+};
+
+/* Traversal layout: 32 bytes, preorder; esc = preorder index of the node
+ * after this subtree; tri = triangle index of a leaf, -1 for an inner node. */
+struct BvhFlat {
+    float lo[3];
+    int32_t esc;
+    float hi[3];
+    int32_t tri;
 };
 
 /* One Moller-Trumbore test of the reference (mesh.cl:54-111). */
@@ -60,7 +72,7 @@ DEV void mesh_tri(const float *P, rf3 pos, rf3 vel, float *out, uint32_t *out_id
 
 /* conservative slab test: may the ray meet the box at some t > 0 that
  * could improve a hit bounded by tmax_need (or any t when counting)? */
-DEV bool bvh_box_hit(const BvhNode &nd, rf3 pos, rf3 inv, float tlimit)
+DEV bool bvh_box_hit(const BvhFlat &nd, rf3 pos, rf3 inv, float tlimit)
 {
     float tx1 = (nd.lo[0] - pos.x) * inv.x, tx2 = (nd.hi[0] - pos.x) * inv.x;
     float ty1 = (nd.lo[1] - pos.y) * inv.y, ty2 = (nd.hi[1] - pos.y) * inv.y;
@@ -88,32 +100,25 @@ DEV void g_mesh(NS &n, const DPart &g, uint32_t ci)
     float out[8] = {0.0f, 0.0f, 0.0f, 1e8f, 0.0f, 0.0f, 0.0f, 1e8f};
     uint32_t oidx[2] = {0xffffffffu, 0xffffffffu};
     uint32_t nint = 0;
-    const BvhNode *nodes = (const BvhNode *)g.aux;
-    const uint32_t *order = (const uint32_t *)g.aux2;
+    const BvhFlat *nodes = (const BvhFlat *)g.aux;
     if (nodes == nullptr) {
         for (uint32_t i = 0; i < ntri; i++) mesh_tri(pts + 9 * (size_t)i, pos, vel, out, oidx, i, nint);
     } else {
-        /* box tests in a slightly enlarged frame; the margin is applied to
-         * the node boxes when they are built */
         rf3 inv = rf3_make(1.0f / vel.x, 1.0f / vel.y, 1.0f / vel.z);
-        int stack[64];
-        int sp = 0;
-        stack[sp++] = 0;
-        /* each of the 2n-1 nodes is visited at most once */
-        uint32_t budget = 2u * ntri + 1u;
-        while (sp > 0 && budget-- > 0u) {
-            const BvhNode &nd = nodes[stack[--sp]];
+        const int end = 2 * (int)ntri - 1;
+        int cur = 0;
+        /* cur strictly increases, so the loop ends after at most 2n-1 nodes */
+        while (cur < end) {
+            const BvhFlat nd = nodes[cur];
             float lim = interior ? 3.0e38f : rm_fmax(out[3], out[7]) * 1.0001f + 1e-30f;
-            if (!bvh_box_hit(nd, pos, inv, lim)) continue;
-            if (nd.left < 0) {
-                for (int k = 0; k < nd.count; k++) {
-                    uint32_t tri = order[nd.right + k];
-                    mesh_tri(pts + 9 * (size_t)tri, pos, vel, out, oidx, tri, nint);
-                }
-            } else if (sp < 62) {
-                stack[sp++] = nd.right;
-                stack[sp++] = nd.left;
+            int next = nd.esc;
+            if (bvh_box_hit(nd, pos, inv, lim)) {
+                if ((uint32_t)nd.tri < ntri)
+                    mesh_tri(pts + 9 * (size_t)nd.tri, pos, vel, out, oidx, (uint32_t)nd.tri, nint);
+                else if (nd.tri < 0)
+                    next = cur + 1;
             }
+            cur = next > cur ? next : cur + 1;
         }
     }
     if (interior == 1 && (nint % 2) == 1) {

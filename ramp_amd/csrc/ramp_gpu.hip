@@ -441,7 +441,8 @@ struct ramp_plan {
     std::vector<ramp_det_desc> det_desc;
     DevBuf comps, steps, src, dets, tables, cnt, count;
     std::vector<DevBuf> hw, hw2, hq, hq2;
-    std::vector<DevBuf> mesh_nodes, mesh_order;
+    std::vector<DevBuf> mesh_nodes;
+    DevBuf comps_brute;  /* comps with every mesh's BVH removed (RAMP_RUN_MESH_BRUTE_FORCE) */
     QueueBufs q[2];
     uint64_t qcap = 0;
     bool qfull = false;
@@ -681,17 +682,24 @@ static int build_mesh(ramp_plan *p, DPart &g, const float *dpts, uint32_t ntri)
                            n, (BvhNode *)nodes.p, (int *)parent.p);
     hipLaunchKernelGGL(k_bvh_boxes, dim3(gblk), dim3(b), 0, s, dpts, (const uint32_t *)order.p, n, margin,
                        (BvhNode *)nodes.p, (const int *)parent.p, (int *)flags.p);
+    DevBuf flat;
+    rc = dev_alloc(flat, (2ull * n - 1) * sizeof(BvhFlat));
+    if (rc) { cleanup(); dev_free(order); dev_free(nodes); return rc; }
+    hipLaunchKernelGGL(k_bvh_flatten, dim3((unsigned)((2 * n - 1 + 255) / 256)), dim3(256), 0, s,
+                       (const BvhNode *)nodes.p, (const int *)parent.p, (const uint32_t *)order.p, n,
+                       (BvhFlat *)flat.p);
     hipError_t e = hipStreamSynchronize(s);
     cleanup();
+    dev_free(order);
+    dev_free(nodes);
     if (e != hipSuccess) {
-        dev_free(order); dev_free(nodes);
+        dev_free(flat);
         return fail(RAMP_E_DEVICE, "BVH build: %s", hipGetErrorString(e));
     }
-    /* node 0 is the root (for one triangle, the leaf itself) */
-    g.aux = nodes.p;
-    g.aux2 = order.p;
-    p->mesh_nodes.push_back(nodes);
-    p->mesh_order.push_back(order);
+    /* preorder node 0 is the root (for one triangle, the leaf itself) */
+    g.aux = flat.p;
+    g.aux2 = nullptr;
+    p->mesh_nodes.push_back(flat);
     return 0;
 }
 
@@ -880,9 +888,9 @@ extern "C" int ramp_plan_destroy(ramp_plan *p)
     }
     DevBuf *single[] = {&p->comps, &p->steps, &p->src, &p->dets, &p->tables, &p->cnt, &p->count,
                         &p->slot_ctr, &p->slot_iidx, &p->st16, &p->st8, &p->stii,
-                        &p->rcounts, &p->prefix};
+                        &p->rcounts, &p->prefix, &p->comps_brute};
     for (DevBuf *b : single) dev_free(*b);
-    for (auto *v : {&p->hw, &p->hw2, &p->hq, &p->hq2, &p->mesh_nodes, &p->mesh_order})
+    for (auto *v : {&p->hw, &p->hw2, &p->hq, &p->hq2, &p->mesh_nodes})
         for (auto &b : *v) dev_free(b);
     for (int i = 0; i < 2; i++) {
         for (auto &b : p->q[i].f) dev_free(b);
@@ -1060,6 +1068,8 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
     if (opts) o = *opts;
     if (o.hist_mode != RAMP_HIST_ATOMIC && o.hist_mode != RAMP_HIST_DETERMINISTIC)
         return fail(RAMP_E_INVALID, "unknown histogram mode %u", o.hist_mode);
+    if (o.flags & ~(uint32_t)RAMP_RUN_MESH_BRUTE_FORCE)
+        return fail(RAMP_E_INVALID, "unknown run flags 0x%x", o.flags);
     if (n_hist == 0) return 0;
     if (chunk == 0 || chunk > n_hist) chunk = n_hist;
     if (chunk > 0x7fffffffull) return fail(RAMP_E_RANGE, "chunk %llu exceeds 2^31-1", (unsigned long long)chunk);
@@ -1106,6 +1116,17 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
     SegArgs a;
     std::memset(&a, 0, sizeof a);
     a.comps = (const DComp *)p->comps.p;
+    if (o.flags & RAMP_RUN_MESH_BRUTE_FORCE) {
+        if (!p->comps_brute.p) {
+            std::vector<DComp> bc = p->hcomps;
+            for (auto &c : bc)
+                if (c.g.type == RAMP_G_MESH) { c.g.aux = nullptr; c.g.aux2 = nullptr; }
+            rc = dev_alloc(p->comps_brute, bc.size() * sizeof(DComp));
+            if (rc) return rc;
+            HIPCHK(hipMemcpy(p->comps_brute.p, bc.data(), bc.size() * sizeof(DComp), hipMemcpyHostToDevice));
+        }
+        a.comps = (const DComp *)p->comps_brute.p;
+    }
     a.steps = (const DStep *)p->steps.p;
     a.src = (const DPart *)p->src.p;
     a.dets = (const DDet *)p->dets.p;

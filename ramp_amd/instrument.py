@@ -263,8 +263,14 @@ class Instrument:
 
     # ---- execution ----------------------------------------------------------
     def execute(self, N, buf_max_mod=None, debug=False, trace=False, keep_state=False,
-                seeds=None):
-        """Run N histories (mcramp.py:403-455)."""
+                seeds=None, mesh_brute_force=False, wg=0, npt=0):
+        """Run N histories (mcramp.py:403-455).
+
+        Extensions: `seeds` (one per chunk), `keep_state` (keep the final
+        per-neutron state of the last chunk), `mesh_brute_force` (test every
+        triangle of every GMesh as the reference does instead of traversing
+        the BVH; for checking the BVH), `wg`/`npt` (launch shape of the
+        source segment; 0 = library default)."""
         lib = self._ensure_plan()
         N = int(N)
         chunk = self.chunking(N, buf_max_mod)
@@ -277,6 +283,9 @@ class Instrument:
                           else capi.RAMP_HIST_ATOMIC)
         opts.keep_state = 1 if (keep_state or debug) else 0
         opts.weight_bound = self.source.weight_bound(chunk) if N else 0.0
+        opts.flags = capi.RAMP_RUN_MESH_BRUTE_FORCE if mesh_brute_force else 0
+        opts.wg = int(wg)
+        opts.npt = int(npt)
         capi.check(lib.ramp_run(self._plan, N, chunk,
                                 seeds.ctypes.data_as(C.POINTER(C.c_uint32)), len(seeds),
                                 C.byref(opts)))
