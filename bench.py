@@ -18,6 +18,7 @@ cores on a bounded sample of the same workload.
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -48,6 +49,7 @@ WORKLOADS = {
     "imaging": dict(file="imaging.json", histories=1e8, name="imaging_mesh50k_1e8",
                     bins=2048 * 2048),
     "minimal": dict(file="minimal.json", histories=1e6, name="minimal_1e6", bins=256 * 512),
+    "tas_scan": dict(file="tas_scan.json", histories=None, name="tas_scan_256", bins=256 * 128),
 }
 
 
@@ -123,6 +125,8 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     wl = WORKLOADS[a.workload]
+    if a.workload == "tas_scan":
+        return bench_tas_scan(a, wl, sync)
     tmpdir = tempfile.mkdtemp(prefix="ramp_bench_")
     kw = instrument_args(a.workload, tmpdir)
     path = os.path.join(ROOT, "ramp_amd", "instruments", wl["file"])
@@ -186,6 +190,90 @@ def main():
         out["cpu_baseline"] = cpu_baseline(path, kw, a.cpu_seconds, wl["name"])
         out["cpu_baseline"]["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
     inst.close()
+    print(json.dumps(out), flush=True)
+
+
+def bench_tas_scan(a, wl, sync):
+    """256-point triple-axis scan: one batched launch per step (graph
+    replay) after the autotuner has picked the launch shape, against the
+    same scan run the reference's way (one plain run per setting)."""
+    import numpy as np
+    import ramp_amd
+    sys.path.insert(0, os.path.join(ROOT, "ramp_amd", "instruments"))
+    import tas_scan
+    settings, counts = tas_scan.settings()
+    counts = np.array(counts, dtype=np.uint64)
+    if a.histories:
+        counts = np.maximum(counts * (a.histories / counts.sum()), 1).astype(np.uint64)
+    total = int(counts.sum())
+    path = os.path.join(ROOT, "ramp_amd", "instruments", wl["file"])
+    b = ramp_amd.BatchInstrument(path, settings, hist_mode=a.hist_mode)
+    t = time.perf_counter()
+    best, tuning = b.tune(counts, scale=1.0 / 16)
+    tune_s = time.perf_counter() - t
+    seeds = np.random.default_rng(20261203).integers(0, 4097, size=len(settings))
+
+    def rel_err():
+        r = []
+        for inst in b.instruments:
+            w, w2 = inst.hist64("tube")
+            r.append(math.sqrt(w2.sum()) / w.sum() if w.sum() > 0 else float("nan"))
+        return np.array(r)
+
+    # one scan each way with the same seeds (also warms both paths): the
+    # histories are the same, so the statistical errors must match
+    b.reset()
+    b.execute(counts, seeds=seeds)
+    err_batch = rel_err()
+    b.reset()
+    b.execute_sequential(counts, seeds=seeds)
+    err_seq = rel_err()
+    b.reset()
+    ok = np.isfinite(err_seq)
+    err_dev = float(np.max(np.abs(err_batch[ok] / err_seq[ok] - 1.0))) if ok.any() else None
+    for _ in range(a.warmup):
+        b.execute(counts, seeds=seeds)
+    sync()
+    dev_ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        dev_ms += b.execute(counts, seeds=seeds)["ms_total"]
+    sync()
+    dt = time.perf_counter() - t0
+    K = max(a.steps, 1)
+    sync()
+    t0 = time.perf_counter()
+    b.execute_sequential(counts, seeds=seeds)
+    sync()
+    dt_seq = time.perf_counter() - t0
+    seq_ms_dev = sum(i.last_stats["ms_total"] for i, c in zip(b.instruments, counts) if int(c))
+    out = {
+        "metric": METRIC,
+        "value": total * K / dt,
+        "unit": "histories/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": a.warmup,
+        "ms_per_step": dt / K * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: 256 scan points from numpy default_rng(4), histories generated on the device",
+        "config": {"workload": wl["name"], "settings": len(settings), "histories_per_step": total,
+                   "counts": "log-uniform 1e5..4e6 per setting", "hist_mode": a.hist_mode,
+                   "instrument": "ramp_amd/instruments/tas_scan.json",
+                   "autotuned": {"npt": best[0], "wg": best[1], "tune_s": tune_s,
+                                 "ms_by_shape": {f"{k[0]}x{k[1]}": v for k, v in tuning.items()}},
+                   "device_ms_per_step": dev_ms / K},
+        "sequential": {"histories_per_s": total / dt_seq, "ms": dt_seq * 1e3,
+                       "device_ms_sum": seq_ms_dev,
+                       "batch_speedup": (total * K / dt) / (total / dt_seq),
+                       "settings_with_counts": int(ok.sum()),
+                       "max_rel_error_mismatch": err_dev},
+        "roofline": None,
+    }
+    b.close()
     print(json.dumps(out), flush=True)
 
 

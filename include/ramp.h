@@ -191,6 +191,26 @@ int ramp_state_read(ramp_plan *plan, float *neutrons16, float *isect8, uint32_t 
 
 int ramp_stats_read(const ramp_plan *plan, ramp_run_stats *out);
 
+/* ---- batches of instrument settings --------------------------------------
+ * A batch runs many plans - one per instrument setting (scan point, chopper
+ * phase, Ei), all created on one context - in one persistent launch with a
+ * device work queue, replayed as a HIP graph.  Replaces the reference's
+ * per-setting loop of Instrument(...).execute(N) calls (one Instrument per
+ * setting, mcramp.py:271, 403-455).  Setting s runs counts[s] histories as
+ * one chunk with seeds[s]; its histograms accumulate in plan s (read them
+ * with ramp_hist_read) and equal those of ramp_run(plan s, counts[s],
+ * counts[s], &seeds[s], 1) - bit for bit in deterministic mode.
+ * weight_bounds (may be NULL: 1/counts[s]) bounds one event's weight per
+ * setting for the deterministic fixed-point scale.  opts: hist_mode, wg
+ * (64/128/256/512), npt (histories per lane per work tile: 1/2/4/8/16);
+ * keep_state and flags must be 0. */
+typedef struct ramp_batch ramp_batch;
+int ramp_batch_create(ramp_ctx *ctx, ramp_plan *const *plans, uint32_t n_settings, ramp_batch **out);
+int ramp_batch_destroy(ramp_batch *batch);
+int ramp_batch_run(ramp_batch *batch, const uint64_t *counts, const uint32_t *seeds,
+                   const double *weight_bounds, const ramp_run_opts *opts);
+int ramp_batch_stats_read(const ramp_batch *batch, ramp_run_stats *out);
+
 /* Evaluate the shared FP32 math library (fn = RAMP_FN_*) on n inputs, on the
  * host (on_device = 0) or on the GPU of ctx (on_device = 1).  Test hook for
  * the bit-identity of host and device arithmetic; no reference counterpart. */

@@ -1176,10 +1176,24 @@ static void s_incoherent(on_t *n, const ramp_part_desc *s, uint32_t ci)
     float t_start = v3_eq(is_pos1(n), is_pos2(n)) ? 0.0f : n->is[3];
     set_pos(n, v3_add(start, v3_scale(frac, path)));
     n->N_TOF += t_start + depth / v;
-    float ct = 1.0f - 2.0f * ornd(n);
-    float ph = 6.28318530717958647692f * ornd(n);
-    float st = rm_sqrt(rm_fmax(0.0f, 1.0f - ct * ct));
-    set_vel(n, rf3_make(v * (st * rm_cos(ph)), v * (st * rm_sin(ph)), v * ct));
+    if (p[IN_FOCUS_R] > 0.0f) {
+        /* focusing cone (the SRescal cone of scat/rescal.cl:31-36, weighted
+         * by its solid-angle fraction) */
+        float ct = 1.0f - ornd(n) * p[IN_OMC];
+        float ph = 6.28318530717958647692f * ornd(n);
+        float st = rm_sqrt(rm_fmax(0.0f, 1.0f - ct * ct));
+        float cp = rm_cos(ph), sp = rm_sin(ph);
+        float dx = ct * p[IN_U + 0] + st * (cp * p[IN_E1 + 0] + sp * p[IN_E2 + 0]);
+        float dy = ct * p[IN_U + 1] + st * (cp * p[IN_E1 + 1] + sp * p[IN_E2 + 1]);
+        float dz = ct * p[IN_U + 2] + st * (cp * p[IN_E1 + 2] + sp * p[IN_E2 + 2]);
+        set_vel(n, rf3_make(v * dx, v * dy, v * dz));
+        n->N_P *= p[IN_WF];
+    } else {
+        float ct = 1.0f - 2.0f * ornd(n);
+        float ph = 6.28318530717958647692f * ornd(n);
+        float st = rm_sqrt(rm_fmax(0.0f, 1.0f - ct * ct));
+        set_vel(n, rf3_make(v * (st * rm_cos(ph)), v * (st * rm_sin(ph)), v * ct));
+    }
     s_done(n, ci);
 }
 

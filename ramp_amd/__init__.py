@@ -7,6 +7,7 @@ kernels behind a C-ABI); there is no CPU execution path.
 """
 from .components import *  # noqa: F401,F403
 from .components import SOURCES, GEOMS, SCATS  # noqa: F401
-from .instrument import Instrument, ExecutionBlock, frame_rotate  # noqa: F401
+from .instrument import Context, Instrument, ExecutionBlock, frame_rotate  # noqa: F401
+from .batch import BatchInstrument  # noqa: F401
 
 __version__ = "0.1.0"

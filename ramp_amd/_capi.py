@@ -143,6 +143,12 @@ _SIGS = {
                                  C.POINTER(C.c_float), C.c_uint64, C.c_int]),
     "ramp_rng_eval": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_float),
                                 C.POINTER(C.c_float), C.c_uint64, C.c_int]),
+    "ramp_batch_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_uint32,
+                                    C.POINTER(C.c_void_p)]),
+    "ramp_batch_destroy": (C.c_int, [C.c_void_p]),
+    "ramp_batch_run": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32),
+                                 C.POINTER(C.c_double), C.POINTER(RunOpts)]),
+    "ramp_batch_stats_read": (C.c_int, [C.c_void_p, C.POINTER(RunStats)]),
     "ramp_last_error": (C.c_char_p, []),
     "ramp_version": (C.c_char_p, []),
 }

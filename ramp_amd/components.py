@@ -11,6 +11,7 @@ one plan executed by libramp.so.
 Classes marked "no reference counterpart" implement configuration elements
 of the benchmark workloads that the reference cannot express.
 """
+import math
 import os
 import re
 
@@ -857,15 +858,42 @@ class SIncoherent(_Scat):
 
     sigma_s: macroscopic scattering cross-section (1/cm);
     sigma_a: macroscopic absorption cross-section at lambda_ref (1/cm),
-    scaled linearly with wavelength."""
+    scaled linearly with wavelength;
+    target, focus_r: optional focusing, as SRescal's (scat/rescal.cl:31-51):
+    the outgoing direction is uniform in the cone around `target` (a vector
+    in the sample's frame, m) whose half-angle theta_max has
+    cos(theta_max) = |target| / sqrt(focus_r^2 + |target|^2), and the weight
+    is multiplied by the cone's solid-angle fraction (1 - cos(theta_max))/2,
+    so the expected weight into any direction inside the cone is that of
+    isotropic scattering.  The cone frame (u, e1, e2) is built here in
+    float64: u = target/|target|, e1 along SRescal's perpendicular
+    (-t_z, 0, t_x) (or x when t_x = t_z = 0), e2 = u x e1."""
     type_code = capi.RAMP_S_INCOHERENT
 
-    def __init__(self, sigma_s=0.0, sigma_a=0.0, lambda_ref=1.8, idx=0, ctx=None, **kwargs):
+    def __init__(self, sigma_s=0.0, sigma_a=0.0, lambda_ref=1.8, target=None, focus_r=0.0,
+                 idx=0, ctx=None, **kwargs):
         super().__init__()
         self.idx = np.uint32(idx)
         self.p[S.IN_SIGMA_S] = np.float32(sigma_s)
         self.p[S.IN_SIGMA_A] = np.float32(sigma_a)
         self.p[S.IN_LREF] = np.float32(lambda_ref)
+        if focus_r and target is not None and np.any(np.asarray(target, dtype=np.float64) != 0):
+            t = np.asarray(target, dtype=np.float64)[:3]
+            l2 = float(np.dot(t, t))
+            r2 = float(focus_r) ** 2
+            u = t / math.sqrt(l2)
+            perp = np.array([1.0, 0.0, 0.0]) if (t[0] == 0 and t[2] == 0) else \
+                np.array([-t[2], 0.0, t[0]])
+            e1 = perp / np.linalg.norm(perp)
+            e2 = np.cross(u, e1)
+            cosmax = math.sqrt(l2 / (r2 + l2))
+            omc = (r2 / (r2 + l2)) / (1.0 + cosmax)
+            self.p[S.IN_FOCUS_R] = np.float32(focus_r)
+            self.p[S.IN_U:S.IN_U + 3] = u.astype(np.float32)
+            self.p[S.IN_E1:S.IN_E1 + 3] = e1.astype(np.float32)
+            self.p[S.IN_E2:S.IN_E2 + 3] = e2.astype(np.float32)
+            self.p[S.IN_OMC] = np.float32(omc)
+            self.p[S.IN_WF] = np.float32(0.5 * omc)
 
 
 class SAbsorber(_Scat):

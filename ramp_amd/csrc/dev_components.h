@@ -40,7 +40,9 @@ struct NS {
     float x, y, z, vx, vy, vz, px, py, pz, w, t, ctr, sc, sd, se;
     float is[8];
     uint32_t iidx, key;
+    uint32_t aux;  /* state of a component split at a segment boundary */
     bool dead;
+    bool pend;     /* the first half of a split component is done, the second is due */
 };
 
 DEV float drw(NS &n) { return rr_draw(&n.ctr, n.key); }
@@ -137,6 +139,8 @@ DEV void src_clear(NS &n, int seed, float sb_old)
     n.w = n.t = n.sc = n.sd = n.se = 0.0f;
     n.ctr = (float)seed + sb_old;
     n.dead = false;
+    n.pend = false;
+    n.aux = 0u;
 }
 
 /* position on the face and direction to a point of the target window
@@ -694,12 +698,28 @@ DEV void s_incoherent(NS &n, const DComp &c)
     float t_start = inside ? 0.0f : n.is[3];
     setpos(n, v3_add(start, v3_scale(frac, path)));
     n.t += t_start + depth / v;
-    float ct = 1.0f - 2.0f * drw(n);
-    float ph = 6.28318530717958647692f * drw(n);
-    float st = rm_sqrt(rm_fmax(0.0f, 1.0f - ct * ct));
-    n.vx = v * (st * rm_cos(ph));
-    n.vy = v * (st * rm_sin(ph));
-    n.vz = v * ct;
+    if (s.p[IN_FOCUS_R] > 0.0f) {
+        /* uniform in the cone around u (sample frame), weight by its solid angle */
+        const float *P = s.p;
+        float ct = 1.0f - drw(n) * P[IN_OMC];
+        float ph = 6.28318530717958647692f * drw(n);
+        float st = rm_sqrt(rm_fmax(0.0f, 1.0f - ct * ct));
+        float cp = rm_cos(ph), sp = rm_sin(ph);
+        float dx = ct * P[IN_U + 0] + st * (cp * P[IN_E1 + 0] + sp * P[IN_E2 + 0]);
+        float dy = ct * P[IN_U + 1] + st * (cp * P[IN_E1 + 1] + sp * P[IN_E2 + 1]);
+        float dz = ct * P[IN_U + 2] + st * (cp * P[IN_E1 + 2] + sp * P[IN_E2 + 2]);
+        n.vx = v * dx;
+        n.vy = v * dy;
+        n.vz = v * dz;
+        n.w *= P[IN_WF];
+    } else {
+        float ct = 1.0f - 2.0f * drw(n);
+        float ph = 6.28318530717958647692f * drw(n);
+        float st = rm_sqrt(rm_fmax(0.0f, 1.0f - ct * ct));
+        n.vx = v * (st * rm_cos(ph));
+        n.vy = v * (st * rm_sin(ph));
+        n.vz = v * ct;
+    }
     s_done(n, c.comp_idx);
 }
 

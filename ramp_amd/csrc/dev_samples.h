@@ -286,93 +286,157 @@ DEV float mono_gauss(float x, float rms, float sq2pi)
  * d[2] mos_rms_max, d[3] Q, d[4] zmax, d[5] ymax, d[8] sqrt(2 pi),
  * d[9]/d[10] 2*pi split.  Box-Muller's cos and the GAUSS weights are FP32
  * approximations of the reference's double evaluation. */
-DEV void s_mono(NS &n, const DComp &c)
+/* slab of the monochromator grid (monochromator.cl:62-82): tilts, centre
+ * and the rotation into the slab frame, from the slab's column and row */
+struct MonoSlab {
+    rmat3 SR;
+    float cy, cz;
+};
+
+DEV MonoSlab mono_slab(const DPart &s, float col, float row)
+{
+    uint32_t nh = s.ni[MOI_NH], nv = s.ni[MOI_NV];
+    float sw = s.p[MO_SLAB_W], sh = s.p[MO_SLAB_H], gap = s.p[MO_GAP];
+    float zmin = -s.d[4], ymin = -s.d[5];
+    float tilt_h = s.p[MO_RAD_H] > 0.0f
+                       ? rm_asin(((float)col - (float)(nh + 1) / 2.0f) * (sw + gap) / s.p[MO_RAD_H])
+                       : 0.0f;
+    float tilt_v = s.p[MO_RAD_V] > 0.0f
+                       ? -rm_asin(((float)row - (float)(nv + 1) / 2.0f) * (sh + gap) / s.p[MO_RAD_V])
+                       : 0.0f;
+    MonoSlab m;
+    m.cz = zmin + (col - 0.5f) * (sw + gap) - gap / 2.0f;
+    m.cy = ymin + (row - 0.5f) * (sh + gap) - gap / 2.0f;
+    m.SR = rmat3_euler(0.0f, tilt_h, tilt_v);
+    return m;
+}
+
+/* Bragg geometry of the neutron in the slab frame (monochromator.cl:90-110) */
+struct MonoBragg {
+    rf3 ki, ku;
+    float kl, q0x, theta, delta, l12;
+};
+
+DEV MonoBragg mono_bragg(const NS &n, float Q)
+{
+    MonoBragg g;
+    g.ki = v3_scale(RC_V2K, nvel(n));
+    float ratio = -2.0f * g.ki.x / Q;
+    float Q_order = rm_floor(ratio + 0.5f);
+    if (Q_order == 0.0f) Q_order = ratio < 0.0f ? -1.0f : 1.0f;
+    if (Q_order < 0.0f) Q_order = -Q_order;
+    g.ku = v3_norm(g.ki);
+    g.kl = v3_len(g.ki);
+    if (Q_order > 2.0f * g.kl / Q) Q_order -= 1.0f;
+    float q0 = Q_order * Q;
+    g.q0x = ratio < 0.0f ? -q0 : q0;
+    g.theta = rm_asin(q0 / (2.0f * g.kl));
+    g.delta = rm_asin(rm_fabs(g.ku.x)) - g.theta;
+    g.l12 = v2_len(rf2_make(g.ki.y, g.ki.z));
+    return g;
+}
+
+DEV void mono_leave(NS &n, const MonoSlab &m)
+{
+    setpos(n, rmat3_rotate(&m.SR, npos(n)));
+    setpos(n, rf3_make(n.x + 0.0f, n.y + m.cy, n.z + m.cz));
+    setvel(n, rmat3_rotate(&m.SR, nvel(n)));
+}
+
+/* monochromator.cl:10-197, first half: slab selection and the reflection
+ * test (one draw).  Returns true when the neutron reflects; it is then
+ * left in the slab frame, aux = column | row << 16, and s_mono_b finishes
+ * it.  Otherwise the component is complete. */
+DEV bool s_mono_a(NS &n, const DComp &c, uint32_t &aux)
 {
     const DPart &s = c.s;
-    const float *gx = s.tab[MOT_GX], *gw = s.tab[MOT_GW];
-    uint32_t nh = s.ni[MOI_NH], nv = s.ni[MOI_NV], glen = s.ni[MOI_GLEN];
     setpos(n, ipos2(n));
     n.t += n.is[7];
-    float mry = s.d[0], mrz = s.d[1], mrmax = s.d[2], Q = s.d[3];
+    float mry = s.d[0], mrz = s.d[1], Q = s.d[3];
     float sw = s.p[MO_SLAB_W], sh = s.p[MO_SLAB_H], gap = s.p[MO_GAP];
     float zmax = s.d[4], zmin = -zmax, ymax = s.d[5], ymin = -ymax;
     if (n.z > zmin && n.z < zmax && n.y > ymin && n.y < ymax) {
         float col = rm_ceil((n.z - zmin) / (sw + gap));
         float row = rm_ceil((n.y - ymin) / (sh + gap));
-        float tilt_h = s.p[MO_RAD_H] > 0.0f
-                           ? rm_asin(((float)col - (float)(nh + 1) / 2.0f) * (sw + gap) / s.p[MO_RAD_H])
-                           : 0.0f;
-        float tilt_v = s.p[MO_RAD_V] > 0.0f
-                           ? -rm_asin(((float)row - (float)(nv + 1) / 2.0f) * (sh + gap) / s.p[MO_RAD_V])
-                           : 0.0f;
-        float cz = zmin + (col - 0.5f) * (sw + gap) - gap / 2.0f;
-        float cy = ymin + (row - 0.5f) * (sh + gap) - gap / 2.0f;
-        rmat3 SR = rmat3_euler(0.0f, tilt_h, tilt_v);
-        setvel(n, rmat3_derotate(&SR, nvel(n)));
-        setpos(n, rf3_make(n.x - 0.0f, n.y - cy, n.z - cz));
-        setpos(n, rmat3_derotate(&SR, npos(n)));
+        MonoSlab m = mono_slab(s, col, row);
+        setvel(n, rmat3_derotate(&m.SR, nvel(n)));
+        setpos(n, rf3_make(n.x - 0.0f, n.y - m.cy, n.z - m.cz));
+        setpos(n, rmat3_derotate(&m.SR, npos(n)));
         if (rm_fabs(n.z) <= sw / 2.0f && rm_fabs(n.y) <= sh / 2.0f) {
-            rf3 ki = v3_scale(RC_V2K, nvel(n));
-            float ratio = -2.0f * ki.x / Q;
-            float Q_order = rm_floor(ratio + 0.5f);
-            if (Q_order == 0.0f) Q_order = ratio < 0.0f ? -1.0f : 1.0f;
-            if (Q_order < 0.0f) Q_order = -Q_order;
-            rf3 ku = v3_norm(ki);
-            float kl = v3_len(ki);
-            if (Q_order > 2.0f * kl / Q) Q_order -= 1.0f;
-            float q0 = Q_order * Q;
-            float q0x = ratio < 0.0f ? -q0 : q0;
-            float theta = rm_asin(q0 / (2.0f * kl));
-            float delta = rm_asin(rm_fabs(ku.x)) - theta;
-            float l12 = v2_len(rf2_make(ki.y, ki.z));
+            MonoBragg g = mono_bragg(n, Q);
             float p_reflect = rm_fabs(s.p[MO_R0]) *
-                              rm_exp(-ki.z * ki.z / (l12 * l12) * delta * delta / (2.0f * mry * mry)) *
-                              rm_exp(-ki.y * ki.y / (l12 * l12) * delta * delta / (2.0f * mrz * mrz));
+                              rm_exp(-g.ki.z * g.ki.z / (g.l12 * g.l12) * g.delta * g.delta / (2.0f * mry * mry)) *
+                              rm_exp(-g.ki.y * g.ki.y / (g.l12 * g.l12) * g.delta * g.delta / (2.0f * mrz * mrz));
             if (drw(n) <= p_reflect) {
-                float cos_2theta = rm_cos(2.0f * theta);
-                float k_sin_2theta = kl * rm_sin(2.0f * theta);
-                rf3 b = v3_mulv(v3_norm(v3_cross(ki, rf3_make(q0x, 0.0f, 0.0f))), k_sin_2theta);
-                rf3 a = v3_cross(b, ku);
-                float total = 0.0f, w = 0.0f;
-                float mos_sample = mrmax / rm_cos(theta);
-                rf3 c1 = v3_mulv(ki, (cos_2theta - 1.0f));
-                rf3 qv;
-                for (uint32_t i = 0; i < 100; i++) {
-                    w = 5.0f * mos_sample;
-                    float cphi = rm_cos(w), sphi = rm_sin(w);
-                    qv = v3_add(v3_add(c1, v3_scale(cphi, a)), v3_scale(sphi, b));
-                    qv.y = qv.y / mrz;
-                    qv.z = qv.z / mry;
-                    if (qv.z * qv.z + qv.y * qv.y < (25.0f / (2.0f / 3.0f)) * (qv.x * qv.x)) break;
-                    mos_sample *= (2.0f / 3.0f);
-                }
-                float sq2pi = s.d[8];
-                for (uint32_t i = 0; i < glen; i++) {
-                    float phi = w * gx[i];
-                    float cphi = rm_cos(phi), sphi = rm_sin(phi);
-                    qv = v3_add(v3_add(c1, v3_scale(cphi, a)), v3_scale(sphi, b));
-                    float pr = mono_gauss(qv.z / qv.x, mry, sq2pi) * mono_gauss(qv.y / qv.x, mrz, sq2pi);
-                    total += gw[i] * pr;
-                }
-                total *= w;
-                float u1 = drw(n), u2 = drw(n);
-                rdf tp; tp.hi = s.d[9]; tp.lo = s.d[10];
-                float rn = rm_sqrt(-2.0f * rm_log(u1)) * rm_cos(df_mulc(u2, tp));
-                float phi = mos_sample * rn;
-                float cphi = rm_cos(phi), sphi = rm_sin(phi);
-                qv = v3_add(v3_add(c1, v3_scale(cphi, a)), v3_scale(sphi, b));
-                p_reflect = mono_gauss(qv.z / qv.x, mry, sq2pi) * mono_gauss(qv.y / qv.x, mrz, sq2pi);
-                setvel(n, v3_scale(RC_K2V, v3_add(ki, qv)));
-                p_reflect = p_reflect / (total * mono_gauss(phi, mos_sample, sq2pi));
-                if (p_reflect > 1.0f) p_reflect = 1.0f;
-                n.w *= p_reflect;
+                aux = (uint32_t)col | ((uint32_t)row << 16);
+                return true;
             }
         }
-        setpos(n, rmat3_rotate(&SR, npos(n)));
-        setpos(n, rf3_make(n.x + 0.0f, n.y + cy, n.z + cz));
-        setvel(n, rmat3_rotate(&SR, nvel(n)));
+        mono_leave(n, m);
     }
     s_done(n, c.comp_idx);
+    return false;
+}
+
+/* second half: the mosaic reflection of a neutron s_mono_a selected (two
+ * draws), back to the component frame */
+DEV void s_mono_b(NS &n, const DComp &c, uint32_t aux)
+{
+    const DPart &s = c.s;
+    const float *gx = s.tab[MOT_GX], *gw = s.tab[MOT_GW];
+    uint32_t glen = s.ni[MOI_GLEN];
+    float mry = s.d[0], mrz = s.d[1], mrmax = s.d[2], Q = s.d[3];
+    MonoBragg g = mono_bragg(n, Q);
+    rf3 ki = g.ki, ku = g.ku;
+    float kl = g.kl, theta = g.theta;
+    float cos_2theta = rm_cos(2.0f * theta);
+    float k_sin_2theta = kl * rm_sin(2.0f * theta);
+    rf3 b = v3_mulv(v3_norm(v3_cross(ki, rf3_make(g.q0x, 0.0f, 0.0f))), k_sin_2theta);
+    rf3 a = v3_cross(b, ku);
+    float total = 0.0f, w = 0.0f;
+    float mos_sample = mrmax / rm_cos(theta);
+    rf3 c1 = v3_mulv(ki, (cos_2theta - 1.0f));
+    rf3 qv;
+    for (uint32_t i = 0; i < 100; i++) {
+        w = 5.0f * mos_sample;
+        float cphi = rm_cos(w), sphi = rm_sin(w);
+        qv = v3_add(v3_add(c1, v3_scale(cphi, a)), v3_scale(sphi, b));
+        qv.y = qv.y / mrz;
+        qv.z = qv.z / mry;
+        if (qv.z * qv.z + qv.y * qv.y < (25.0f / (2.0f / 3.0f)) * (qv.x * qv.x)) break;
+        mos_sample *= (2.0f / 3.0f);
+    }
+    float sq2pi = s.d[8];
+    for (uint32_t i = 0; i < glen; i++) {
+        float phi = w * gx[i];
+        float cphi = rm_cos(phi), sphi = rm_sin(phi);
+        qv = v3_add(v3_add(c1, v3_scale(cphi, a)), v3_scale(sphi, b));
+        float pr = mono_gauss(qv.z / qv.x, mry, sq2pi) * mono_gauss(qv.y / qv.x, mrz, sq2pi);
+        total += gw[i] * pr;
+    }
+    total *= w;
+    float u1 = drw(n), u2 = drw(n);
+    rdf tp; tp.hi = s.d[9]; tp.lo = s.d[10];
+    float rn = rm_sqrt(-2.0f * rm_log(u1)) * rm_cos(df_mulc(u2, tp));
+    float phi = mos_sample * rn;
+    float cphi = rm_cos(phi), sphi = rm_sin(phi);
+    qv = v3_add(v3_add(c1, v3_scale(cphi, a)), v3_scale(sphi, b));
+    float p_reflect = mono_gauss(qv.z / qv.x, mry, sq2pi) * mono_gauss(qv.y / qv.x, mrz, sq2pi);
+    setvel(n, v3_scale(RC_K2V, v3_add(ki, qv)));
+    p_reflect = p_reflect / (total * mono_gauss(phi, mos_sample, sq2pi));
+    if (p_reflect > 1.0f) p_reflect = 1.0f;
+    n.w *= p_reflect;
+    /* the slab frame is rebuilt here rather than kept live through the
+     * quadrature (register pressure) */
+    MonoSlab m = mono_slab(s, (float)(aux & 0xffffu), (float)(aux >> 16));
+    mono_leave(n, m);
+    s_done(n, c.comp_idx);
+}
+
+DEV void s_mono(NS &n, const DComp &c)
+{
+    uint32_t aux = 0;
+    if (s_mono_a(n, c, aux)) s_mono_b(n, c, aux);
 }
 
 #endif

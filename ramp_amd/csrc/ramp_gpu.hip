@@ -71,6 +71,7 @@ struct SegArgs {
     int hist_mode;
     int carry;
     int keep;
+    int defer_k;                /* step whose monochromator reflections are deferred (-1: none) */
     int num_sim;
     int seed;
     uint64_t n;                 /* items of the first segment */
@@ -87,60 +88,6 @@ struct SegArgs {
     uint32_t *stii;
 };
 
-template <bool FULL, uint32_t F>
-DEV void run_steps(NS &n, const SegArgs &a, HistCtx &hc)
-{
-    for (int k = a.s0; k < a.s1; k++) {
-        if (!a.pilot && !__any(!n.dead)) break;
-        const DStep st = a.steps[k];
-        if (st.kind == 0) {
-            const DComp &c = a.comps[st.first];
-            tr_in<FULL>(n, c);
-            geom_dispatch<F>(n, c);
-            terminate(n, c.restore);
-            scat_dispatch<FULL, F>(n, c, hc);
-            tr_out<FULL>(n, c);
-        } else {
-            for (uint32_t ev = 0; ev < st.multi; ev++) {
-                for (uint32_t j = 0; j < st.count; j++) {
-                    const DComp &c = a.comps[st.first + j];
-                    tr_in<FULL>(n, c);
-                    geom_dispatch<F>(n, c);
-                    tr_out<FULL>(n, c);
-                }
-                for (uint32_t j = 0; j < st.count; j++) {
-                    const DComp &c = a.comps[st.first + j];
-                    tr_in<FULL>(n, c);
-                    scat_dispatch<FULL, F>(n, c, hc);
-                    tr_out<FULL>(n, c);
-                }
-            }
-        }
-        if (a.pilot) {
-            unsigned long long m = __ballot(!n.dead);
-            if (__lane_id() == 0 && m) atomicAdd(&a.cnt[CNT_NSTEP0 + k], (unsigned long long)__popcll(m));
-        }
-    }
-}
-
-template <bool FULL>
-DEV void finalize(const NS &n, const SegArgs &a)
-{
-    if (a.carry) {
-        a.slot_ctr[n.key] = n.ctr;
-        a.slot_iidx[n.key] = n.iidx;
-    }
-    if (FULL && a.keep) {
-        float *o = a.st16 + 16 * (size_t)n.key;
-        o[0] = n.x; o[1] = n.y; o[2] = n.z; o[3] = n.vx; o[4] = n.vy; o[5] = n.vz;
-        o[6] = n.px; o[7] = n.py; o[8] = n.pz; o[9] = n.w; o[10] = n.t; o[11] = n.ctr;
-        o[12] = n.sc; o[13] = n.sd; o[14] = n.se; o[15] = n.dead ? 1.0f : 0.0f;
-        float *q = a.st8 + 8 * (size_t)n.key;
-        for (int i = 0; i < 8; i++) q[i] = n.is[i];
-        a.stii[n.key] = n.iidx;
-    }
-}
-
 /* status word of the queue: iidx in bits 0..23, bit 24: intersection is all
  * zeros (GBox miss with restore) instead of the sentinel */
 #define ST_ZERO_ISECT (1u << 24)
@@ -155,6 +102,8 @@ DEV void queue_load(NS &n, const DQueue &q, uint64_t i)
     uint32_t st = q.status[i];
     n.iidx = st & 0x00ffffffu;
     n.dead = false;
+    n.pend = false;
+    n.aux = 0u;
     if (st & ST_ZERO_ISECT) {
         for (int k = 0; k < 8; k++) n.is[k] = 0.0f;
     } else {
@@ -183,6 +132,187 @@ DEV void queue_store(const NS &n, const DQueue &q, uint64_t i)
     }
 }
 
+/* Deferral of the monochromator's reflection branch.  s_mono is cheap
+ * for the many neutrons it does not reflect and long (mosaic search,
+ * 15-point quadrature, ~100 transcendentals) for the few it reflects, so
+ * in a wave of 64 that branch runs at one or two live lanes in most groups.
+ * At the deferred step (SegArgs.defer_k) the reflecting neutrons instead
+ * park in a per-wave LDS queue (state in the slab frame, slab in aux);
+ * when 64 are waiting the wave runs s_mono_b and the remaining steps for
+ * all of them at once.  Each neutron's arithmetic and draws are unchanged,
+ * only the order in which a wave processes them differs. */
+#define DQ_CAP 128
+#define DQ_WORDS 12
+#define DQ_BYTES_PER_WAVE (DQ_CAP * DQ_WORDS * 4)
+
+struct DeferQ {
+    float *b;     /* this wave's DQ_WORDS x DQ_CAP words of LDS (wave-uniform) */
+    uint32_t n;   /* parked neutrons, wave-uniform, < DQ_CAP */
+};
+
+/* order this wave's LDS accesses: the compiler may not move loads of
+ * other lanes' entries across it (a wave executes LDS operations in order) */
+DEV void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+DEV void defer_init(DeferQ &d, float *lds)
+{
+    const uint32_t wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    d.b = lds + wid * (DQ_CAP * DQ_WORDS);
+    d.n = 0;
+}
+
+/* the record of a parked neutron: the segment-queue record (x y z vx vy vz
+ * w t ctr key status) and the slab in aux */
+DEV void defer_store(const NS &n, float *b, uint32_t i)
+{
+    b[0 * DQ_CAP + i] = n.x; b[1 * DQ_CAP + i] = n.y; b[2 * DQ_CAP + i] = n.z;
+    b[3 * DQ_CAP + i] = n.vx; b[4 * DQ_CAP + i] = n.vy; b[5 * DQ_CAP + i] = n.vz;
+    b[6 * DQ_CAP + i] = n.w; b[7 * DQ_CAP + i] = n.t; b[8 * DQ_CAP + i] = n.ctr;
+    uint32_t *u = (uint32_t *)b;
+    u[9 * DQ_CAP + i] = n.key;
+    u[10 * DQ_CAP + i] = n.iidx & 0x00ffffffu;
+    u[11 * DQ_CAP + i] = n.aux;
+}
+
+DEV void defer_load(NS &n, const float *b, uint32_t i)
+{
+    n.x = b[0 * DQ_CAP + i]; n.y = b[1 * DQ_CAP + i]; n.z = b[2 * DQ_CAP + i];
+    n.vx = b[3 * DQ_CAP + i]; n.vy = b[4 * DQ_CAP + i]; n.vz = b[5 * DQ_CAP + i];
+    n.w = b[6 * DQ_CAP + i]; n.t = b[7 * DQ_CAP + i]; n.ctr = b[8 * DQ_CAP + i];
+    const uint32_t *u = (const uint32_t *)b;
+    n.key = u[9 * DQ_CAP + i];
+    n.iidx = u[10 * DQ_CAP + i];
+    n.aux = u[11 * DQ_CAP + i];
+    n.dead = false;
+    n.pend = true;
+    isect_reset(n);
+    n.px = n.py = n.pz = n.sc = n.sd = n.se = 0.0f;
+}
+
+/* Steps [k0, k1); with defer, the step defer_k parks reflecting neutrons
+ * in dq (they leave the range dead with pend set). */
+template <bool FULL, uint32_t F>
+DEV void run_range(NS &n, const SegArgs &a, HistCtx &hc, int k0, int k1, DeferQ &dq, bool defer)
+{
+    for (int k = k0; k < k1; k++) {
+        if (!a.pilot && !__any(!n.dead)) break;
+        const DStep st = a.steps[k];
+        if (st.kind == 0) {
+            const DComp &c = a.comps[st.first];
+            tr_in<FULL>(n, c);
+            geom_dispatch<F>(n, c);
+            terminate(n, c.restore);
+            if constexpr (!FULL && (F & FEAT_MONO) != 0) {
+                if (defer && k == a.defer_k) {
+                    bool park = false;
+                    if (n.iidx == c.comp_idx && !n.dead) park = s_mono_a(n, c, n.aux);
+                    if (!park) tr_out<FULL>(n, c);
+                    const unsigned long long m = __ballot(park);
+                    if (m) {
+                        if (park) {
+                            const unsigned long long below = (1ull << __lane_id()) - 1ull;
+                            defer_store(n, dq.b, dq.n + (uint32_t)__popcll(m & below));
+                            n.pend = true;
+                            n.dead = true;
+                        }
+                        dq.n += (uint32_t)__popcll(m);
+                    }
+                    continue;
+                }
+            }
+            scat_dispatch<FULL, F>(n, c, hc);
+            tr_out<FULL>(n, c);
+        } else {
+            for (uint32_t ev = 0; ev < st.multi; ev++) {
+                for (uint32_t j = 0; j < st.count; j++) {
+                    const DComp &c = a.comps[st.first + j];
+                    tr_in<FULL>(n, c);
+                    geom_dispatch<F>(n, c);
+                    tr_out<FULL>(n, c);
+                }
+                for (uint32_t j = 0; j < st.count; j++) {
+                    const DComp &c = a.comps[st.first + j];
+                    tr_in<FULL>(n, c);
+                    scat_dispatch<FULL, F>(n, c, hc);
+                    tr_out<FULL>(n, c);
+                }
+            }
+        }
+        if (a.pilot) {
+            unsigned long long m = __ballot(!n.dead);
+            if (__lane_id() == 0 && m) atomicAdd(&a.cnt[CNT_NSTEP0 + k], (unsigned long long)__popcll(m));
+        }
+    }
+}
+
+template <bool FULL, uint32_t F>
+DEV void run_steps(NS &n, const SegArgs &a, HistCtx &hc, DeferQ &dq, bool defer)
+{
+    run_range<FULL, F>(n, a, hc, a.s0, a.s1, dq, defer);
+}
+
+/* Take up to 64 parked neutrons (lane i gets entry i); returns how many. */
+DEV uint32_t defer_pop(DeferQ &dq, NS &r)
+{
+    const uint32_t lane = __lane_id();
+    const uint32_t take = dq.n < 64u ? dq.n : 64u;
+    wave_lds_sync();
+    if (lane < take) {
+        defer_load(r, dq.b, lane);
+    } else {
+        r.dead = true;
+        r.pend = false;
+        r.key = 0;
+        r.iidx = 0;
+    }
+    wave_lds_sync();
+    /* the dq.n - take < 64 entries beyond them move down */
+    if (lane < dq.n - take) {
+        uint32_t *u = (uint32_t *)dq.b;
+        for (int f = 0; f < DQ_WORDS; f++) u[f * DQ_CAP + lane] = u[f * DQ_CAP + take + lane];
+    }
+    wave_lds_sync();
+    dq.n -= take;
+    return take;
+}
+
+/* reflection half of the deferred step, then the rest of the range */
+template <uint32_t F>
+DEV void defer_finish(NS &r, const SegArgs &a, HistCtx &hc, int k1, DeferQ &dq)
+{
+    if constexpr ((F & FEAT_MONO) != 0) {
+        const DComp &c = a.comps[a.steps[a.defer_k].first];
+        if (r.pend && !r.dead) {
+            s_mono_b(r, c, r.aux);
+            tr_out<false>(r, c);
+        }
+        r.pend = false;
+        run_range<false, F>(r, a, hc, a.defer_k + 1, k1, dq, false);
+    }
+}
+
+template <bool FULL>
+DEV void finalize(const NS &n, const SegArgs &a)
+{
+    if (a.carry) {
+        a.slot_ctr[n.key] = n.ctr;
+        a.slot_iidx[n.key] = n.iidx;
+    }
+    if (FULL && a.keep) {
+        float *o = a.st16 + 16 * (size_t)n.key;
+        o[0] = n.x; o[1] = n.y; o[2] = n.z; o[3] = n.vx; o[4] = n.vy; o[5] = n.vz;
+        o[6] = n.px; o[7] = n.py; o[8] = n.pz; o[9] = n.w; o[10] = n.t; o[11] = n.ctr;
+        o[12] = n.sc; o[13] = n.sd; o[14] = n.se; o[15] = n.dead ? 1.0f : 0.0f;
+        float *q = a.st8 + 8 * (size_t)n.key;
+        for (int i = 0; i < 8; i++) q[i] = n.is[i];
+        a.stii[n.key] = n.iidx;
+    }
+}
+
 DEV uint32_t wave_sum(uint32_t v)
 {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -202,13 +332,38 @@ DEV uint64_t region_slot(const uint64_t *prefix, uint64_t nreg, uint64_t stride,
     return lo * stride + (v - prefix[lo]);
 }
 
+/* end of a group of 64 in a segment: survivors to the wave's output
+ * region, finished neutrons to finalize (parked ones are neither) */
+template <bool FULL>
+DEV void seg_tail(const NS &n, bool valid, const SegArgs &a, uint64_t beg, uint32_t &wcount)
+{
+    const bool live = valid && !n.dead && !a.last;
+    if (!a.last) {
+        const unsigned long long m = __ballot(live);
+        if (live) {
+            const unsigned long long below = m & ((1ull << __lane_id()) - 1ull);
+            queue_store<FULL>(n, a.out, beg + wcount + (uint64_t)__popcll(below));
+        }
+        wcount += (uint32_t)__popcll(m);
+    }
+    if (valid && !live && !n.pend) finalize<FULL>(n, a);
+}
+
 /* One segment.  Work is split statically: wave g of G handles the dense
  * item range [g*per, (g+1)*per) and appends its survivors to its own region
  * [g*per, ...) of the output queue, so compaction needs no atomics; the
- * region counts are prefix-summed by k_scan before the next segment. */
+ * region counts are prefix-summed by k_scan before the next segment.  A
+ * segment holding the deferred monochromator step drains the wave's
+ * deferral queue whenever it holds 64, and at the end. */
+/* the monochromator instances sit a few registers above the limit of
+ * three waves per SIMD; ask the compiler to fit them there */
+#define SEG_MIN_WAVES(FULL, F) ((!(FULL) && ((F) & FEAT_MONO) != 0) ? 3 : 1)
+
 template <bool FULL, bool FIRST, uint32_t F>
-__global__ __launch_bounds__(256) void k_segment(SegArgs a)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SEG_MIN_WAVES(FULL, F))))
+void k_segment(SegArgs a)
 {
+    extern __shared__ float seg_lds[];
     HistCtx hc;
     hc.dets = a.dets;
     hc.mode = a.hist_mode;
@@ -230,6 +385,14 @@ __global__ __launch_bounds__(256) void k_segment(SegArgs a)
     const uint64_t beg = g * per;
     const uint64_t end = beg + per < total ? beg + per : total;
     uint32_t wcount = 0;
+    DeferQ dq;
+    dq.b = seg_lds;
+    dq.n = 0;
+    bool defer = false;
+    if constexpr (!FULL && (F & FEAT_MONO) != 0) {
+        defer = !a.pilot && a.defer_k >= a.s0 && a.defer_k < a.s1;
+        if (defer) defer_init(dq, seg_lds);
+    }
     for (uint64_t base = beg; base < end; base += 64) {
         uint64_t i = base + lane;
         bool valid = i < end;
@@ -245,23 +408,31 @@ __global__ __launch_bounds__(256) void k_segment(SegArgs a)
             }
         } else {
             n.dead = true;
+            n.pend = false;
             n.key = 0;
             n.iidx = 0;
         }
-        run_steps<FULL, F>(n, a, hc);
+        run_steps<FULL, F>(n, a, hc, dq, defer);
         if (a.pilot) continue;
-        bool live = valid && !n.dead && !a.last;
-        if (!a.last) {
-            unsigned long long m = __ballot(live);
-            if (live) {
-                unsigned long long below = m & ((1ull << lane) - 1ull);
-                queue_store<FULL>(n, a.out, beg + wcount + (uint64_t)__popcll(below));
+        seg_tail<FULL>(n, valid, a, beg, wcount);
+        if constexpr (!FULL && (F & FEAT_MONO) != 0) {
+            if (defer && dq.n >= 64u) {
+                NS r;
+                const uint32_t take = defer_pop(dq, r);
+                defer_finish<F>(r, a, hc, a.s1, dq);
+                seg_tail<FULL>(r, lane < take, a, beg, wcount);
             }
-            wcount += (uint32_t)__popcll(m);
         }
-        if (valid && !live) finalize<FULL>(n, a);
     }
     if (a.pilot) return;
+    if constexpr (!FULL && (F & FEAT_MONO) != 0) {
+        while (defer && dq.n > 0u) {
+            NS r;
+            const uint32_t take = defer_pop(dq, r);
+            defer_finish<F>(r, a, hc, a.s1, dq);
+            seg_tail<FULL>(r, lane < take, a, beg, wcount);
+        }
+    }
     if (!a.last && lane == 0) a.out_counts[g] = wcount;
     uint32_t ev = wave_sum(hc.events), oob = wave_sum(hc.oob), ovf = wave_sum(hc.overflow);
     if (lane == 0) {
@@ -451,8 +622,10 @@ struct ramp_plan {
     bool det_scale_set = false;
     bool has_atomic = false, has_det = false;
     std::vector<int> bounds;  /* segment boundaries (step indices) */
+    int defer_k = -1;         /* first linear monochromator step (reflections deferred) */
     bool bounds_set = false;
     int variant = kNumVariants - 1;
+    uint32_t feat = FEAT_ALL;  /* FEAT_* bits the plan's components need */
     ramp_run_stats stats;
 };
 
@@ -765,6 +938,8 @@ static int validate_plan(const ramp_plan_desc *d)
             return fail(RAMP_E_INVALID, "component %u: chopper with 0 slits", k);
         if (c.scat.type == RAMP_S_MONO && c.scat.p[MO_D] == 0.0f)
             return fail(RAMP_E_INVALID, "component %u: monochromator d-spacing 0", k);
+        if (c.scat.type == RAMP_S_MONO && (c.scat.ni[MOI_NH] > 65535u || c.scat.ni[MOI_NV] > 65535u))
+            return fail(RAMP_E_INVALID, "component %u: more than 65535 monochromator slabs in a row", k);
         if (c.geom.type == RAMP_G_MESH && c.geom.ni[GI_NTRI] * 9ull > c.geom.tab_len[GT_POINTS])
             return fail(RAMP_E_INVALID, "component %u: mesh table shorter than 9*n_tri", k);
         const ramp_part_desc &s = c.scat;
@@ -835,6 +1010,7 @@ extern "C" int ramp_plan_create(ramp_ctx *ctx, const ramp_plan_desc *d, ramp_pla
         if (sc == RAMP_S_ISOTROPIC || sc == RAMP_S_POWDER1 || sc == RAMP_S_POWDERN) feat |= FEAT_SAMPLE;
         if (sc == RAMP_S_MONO) feat |= FEAT_MONO;
     }
+    p->feat = feat;
     for (int v = 0; v < kNumVariants; v++)
         if ((feat & ~kVariants[v]) == 0) { p->variant = v; break; }
     /* steps: blocks in order, starting from block 0 (mcramp.py:452-453) */
@@ -853,6 +1029,9 @@ extern "C" int ramp_plan_create(ramp_ctx *ctx, const ramp_plan_desc *d, ramp_pla
     }
     if (p->hsteps.size() > CNT_N - CNT_NSTEP0)
         PCHK(fail(RAMP_E_RANGE, "too many steps (%zu)", p->hsteps.size()));
+    for (size_t k = 0; k < p->hsteps.size() && p->defer_k < 0; k++)
+        if (p->hsteps[k].kind == 0 && p->hcomps[p->hsteps[k].first].s.type == RAMP_S_MONO)
+            p->defer_k = (int)k;
     p->det_desc.assign(d->dets, d->dets + d->n_dets);
     p->hdets.resize(d->n_dets);
     p->hw.resize(d->n_dets);
@@ -950,6 +1129,30 @@ static int ensure_hist(ramp_plan *p, int mode)
     return 0;
 }
 
+/* histogram buffers of the mode, the fixed-point scale of deterministic
+ * mode (chosen once per plan from the first run's size and weight bound),
+ * and the detector table upload */
+static int prepare_dets(ramp_plan *p, int mode, double weight_bound, uint64_t n_hist, uint64_t chunk)
+{
+    int rc = ensure_hist(p, mode);
+    if (rc) return rc;
+    if (mode == RAMP_HIST_DETERMINISTIC && !p->det_scale_set) {
+        double wb = weight_bound > 0 ? weight_bound : 1.0 / (double)chunk;
+        double tot = wb * (double)n_hist * 16.0;
+        double tot2 = wb * wb * (double)n_hist * 16.0;
+        int sh = (int)std::floor(std::log2(std::ldexp(1.0, 61) / tot));
+        int sh2 = (int)std::floor(std::log2(std::ldexp(1.0, 61) / tot2));
+        sh = std::max(-120, std::min(120, sh));
+        sh2 = std::max(-120, std::min(120, sh2));
+        for (auto &d : p->hdets) { d.sh_w = sh; d.sh_w2 = sh2; }
+        p->det_scale_set = true;
+    }
+    if (!p->hdets.empty())
+        HIPCHK(hipMemcpyAsync(p->dets.p, p->hdets.data(), p->hdets.size() * sizeof(DDet), hipMemcpyHostToDevice,
+                              p->ctx->stream));
+    return 0;
+}
+
 static int ensure_queues(ramp_plan *p, uint64_t cap, bool full)
 {
     if (cap <= p->qcap && (!full || p->qfull)) return 0;
@@ -1002,18 +1205,24 @@ static seg_fn pick_kernel(bool full, bool first, int v)
     return first ? seg_kernel<false, true>(v) : seg_kernel<false, false>(v);
 }
 
+/* LDS of a segment kernel instance: the per-wave deferral queues */
+static size_t seg_lds_bytes(bool full, int v, unsigned wg)
+{
+    return (!full && (kVariants[v] & FEAT_MONO)) ? (size_t)(wg / 64) * DQ_BYTES_PER_WAVE : 0;
+}
+
 static void launch_segment(const SegArgs &a, bool full, bool first, int v, unsigned grid,
                            unsigned wg, hipStream_t s)
 {
-    hipLaunchKernelGGL(pick_kernel(full, first, v), dim3(grid), dim3(wg), 0, s, a);
+    hipLaunchKernelGGL(pick_kernel(full, first, v), dim3(grid), dim3(wg), seg_lds_bytes(full, v, wg), s, a);
 }
 
 /* resident blocks per CU of a kernel instance (for a persistent grid) */
 static unsigned blocks_per_cu(bool full, bool first, int v, unsigned wg)
 {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pick_kernel(full, first, v), (int)wg, 0) !=
-            hipSuccess || nb < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pick_kernel(full, first, v), (int)wg,
+                                                     seg_lds_bytes(full, v, wg)) != hipSuccess || nb < 1)
         nb = 1;
     return (unsigned)nb;
 }
@@ -1084,21 +1293,8 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
     unsigned wg = o.wg ? o.wg : 256;
     if (wg != 64 && wg != 128 && wg != 256) return fail(RAMP_E_INVALID, "workgroup size %u", wg);
     bool full = o.keep_state != 0;
-    int rc = ensure_hist(p, (int)o.hist_mode);
+    int rc = prepare_dets(p, (int)o.hist_mode, o.weight_bound, n_hist, chunk);
     if (rc) return rc;
-    if (o.hist_mode == RAMP_HIST_DETERMINISTIC && !p->det_scale_set) {
-        double wb = o.weight_bound > 0 ? o.weight_bound : 1.0 / (double)chunk;
-        double tot = wb * (double)n_hist * 16.0;
-        double tot2 = wb * wb * (double)n_hist * 16.0;
-        int sh = (int)std::floor(std::log2(std::ldexp(1.0, 61) / tot));
-        int sh2 = (int)std::floor(std::log2(std::ldexp(1.0, 61) / tot2));
-        sh = std::max(-120, std::min(120, sh));
-        sh2 = std::max(-120, std::min(120, sh2));
-        for (auto &d : p->hdets) { d.sh_w = sh; d.sh_w2 = sh2; }
-        p->det_scale_set = true;
-    }
-    if (!p->hdets.empty())
-        HIPCHK(hipMemcpyAsync(p->dets.p, p->hdets.data(), p->hdets.size() * sizeof(DDet), hipMemcpyHostToDevice, s));
     bool carry = n_chunks > 1;
     if (carry) {
         rc = dev_alloc(p->slot_ctr, chunk * sizeof(float));
@@ -1134,6 +1330,7 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
     a.hist_mode = (int)o.hist_mode;
     a.carry = carry ? 1 : 0;
     a.keep = full ? 1 : 0;
+    a.defer_k = full ? -1 : p->defer_k;
     a.slot_ctr = (float *)p->slot_ctr.p;
     a.slot_iidx = (uint32_t *)p->slot_iidx.p;
     a.st16 = (float *)p->st16.p;
@@ -1333,3 +1530,5 @@ extern "C" int ramp_rng_eval(ramp_ctx *ctx, const uint32_t *tid, const float *ct
     if (e != hipSuccess) return fail(RAMP_E_DEVICE, "rng eval: %s", hipGetErrorString(e));
     return 0;
 }
+
+#include "batch_run.h"

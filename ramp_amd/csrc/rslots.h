@@ -133,6 +133,12 @@
 #define IN_SIGMA_S 0          /* 1/cm */
 #define IN_SIGMA_A 1          /* 1/cm at IN_LREF */
 #define IN_LREF 2             /* Angstrom */
+#define IN_FOCUS_R 3          /* > 0: scatter into the cone towards the target */
+#define IN_U 4                /* 4..6 unit vector towards the target */
+#define IN_E1 7               /* 7..9 unit vector perpendicular to it */
+#define IN_E2 10              /* 10..12 u x e1 */
+#define IN_OMC 13             /* 1 - cos(cone half-angle) */
+#define IN_WF 14              /* weight factor: solid-angle fraction (1 - cos)/2 */
 /* SAbsorber (no reference counterpart) */
 #define AB_SIGMA_A 0          /* 1/cm */
 /* SDetectorTubes (no reference counterpart) */

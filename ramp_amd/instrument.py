@@ -117,6 +117,31 @@ class ExecutionBlock:
         return ExecutionBlock(source, comps, linear, events)
 
 
+class Context:
+    """A libramp device context (one HIP device, one stream) that several
+    Instruments share when it is passed as their ctx argument (extension:
+    the reference's ctx is a PyOpenCL context, which is accepted and
+    ignored).  Batches need their instruments on one Context."""
+
+    def __init__(self, device=0):
+        lib = capi.load_library()
+        h = C.c_void_p()
+        capi.check(lib.ramp_ctx_create(int(device), C.byref(h)))
+        self.device = int(device)
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None) is not None:
+            capi.load_library().ramp_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Instrument:
     """Drop-in for mcramp.Instrument(fn, ctx, queue, **kwargs).
 
@@ -248,16 +273,21 @@ class Instrument:
             return np.asarray(self._seed_rng.integers(0, 4097, size=n_chunks), dtype=np.uint32)
         return np.array([random.randint(0, 4096) for _ in range(n_chunks)], dtype=np.uint32)
 
+    def _ctx_handle(self):
+        if isinstance(self.ctx, Context):
+            return self.ctx.handle
+        return self._gctx
+
     def _ensure_plan(self):
         lib = capi.load_library()
-        if self._gctx is None:
+        if self._ctx_handle() is None:
             h = C.c_void_p()
             capi.check(lib.ramp_ctx_create(int(self.device), C.byref(h)))
             self._gctx = h
         if self._plan is None:
             desc = self.plan_desc()
             h = C.c_void_p()
-            capi.check(lib.ramp_plan_create(self._gctx, C.byref(desc), C.byref(h)))
+            capi.check(lib.ramp_plan_create(self._ctx_handle(), C.byref(desc), C.byref(h)))
             self._plan = h
         return lib
 

@@ -206,3 +206,32 @@ def test_run_argument_errors_without_a_gpu(lib):
 def test_missing_library_fails_loudly(tmp_path):
     with pytest.raises(_capi.RampLibraryError):
         _capi.load_library(str(tmp_path / "nope.so"))
+
+
+def test_tas_scan_settings_follow_bragg_law():
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "ramp_amd", "instruments"))
+    import tas_scan
+    s, c = tas_scan.settings()
+    assert len(s) == 256 and len(c) == 256
+    assert min(c) >= 1e5 and max(c) <= 4e6
+    for x in s:
+        assert 3.0 <= x["Ei"] <= 15.0 and tas_scan.EF_MIN <= x["Ef"] <= x["Ei"]
+        assert x["Ei"] - x["Ef"] <= 5.0
+        # lambda = 2 d sin(theta), E = 81.8042 / lambda^2 (meV, A)
+        lam = 2 * 3.355 * math.sin(x["theta_m"])
+        assert 81.8042 / lam**2 == pytest.approx(x["Ei"], rel=2e-4)
+    inst = ramp_amd.Instrument(os.path.join(ROOT, "ramp_amd", "instruments", "tas_scan.json"), **s[0])
+    names = [k.name for k in inst.kernel_refs]
+    assert names[-1] == "tube" and inst.blocks[0].components["tube"].scat.n_bins == 128
+
+
+def test_batch_argument_errors_without_a_gpu(lib):
+    h = C.c_void_p()
+    rc, msg = _err(lib, lib.ramp_batch_create(None, None, 0, C.byref(h)))
+    assert rc == _capi.RAMP_E_INVALID and "context" in msg
+    rc, msg = _err(lib, lib.ramp_batch_run(None, None, None, None, None))
+    assert rc == _capi.RAMP_E_INVALID and "batch" in msg
+    st = _capi.RunStats()
+    rc, _ = _err(lib, lib.ramp_batch_stats_read(None, C.byref(st)))
+    assert rc == _capi.RAMP_E_INVALID

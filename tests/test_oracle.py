@@ -214,3 +214,57 @@ def test_incoherent_sample_weight(tmp_path, oracle):
     S = (ss + sa) * 100
     expect = (1 - math.exp(-S * d)) * ss / (ss + sa)
     assert w == pytest.approx(expect, rel=1e-4)
+
+
+def _focus_inst(tmp_path, name, target=None, focus_r=0.0, with_det=True):
+    samp = {"name": "SIncoherent", "sigma_s": 0.36, "sigma_a": 0.26, "lambda_ref": 1.8}
+    if target is not None:
+        samp.update({"target": list(target), "focus_r": focus_r})
+    comps = {"src": _src_parallel(w=0.008, h=0.008),
+             "samp": {"position": [0, 0, 1.0],
+                      "geom_kernel": {"name": "GCylinderExt", "radius": 0.005, "height": 0.02},
+                      "scat_kernel": samp}}
+    if with_det:
+        # 0.1 x 0.1 m plane 1 m from the sample along (sin 0.7, 0, cos 0.7)
+        comps["det"] = {"position": [math.sin(0.7), 0.0, 1.0 + math.cos(0.7)],
+                        "rotation": [0.0, 0.7, 0.0],
+                        "geom_kernel": {"name": "GPlane", "width": 0.1, "height": 0.1},
+                        "scat_kernel": {"name": "SCounter"}}
+    return _inst(tmp_path, comps, name=name)
+
+
+def test_incoherent_focusing_cone_and_weight(tmp_path, oracle):
+    # directions are uniform in the cone: cos(angle to u) uniform on
+    # [cos theta_max, 1]; the weight is the isotropic one times (1-cos)/2
+    t = (math.sin(0.7), 0.0, math.cos(0.7))
+    fr = 0.075
+    foc = _focus_inst(tmp_path, "f.json", t, fr, with_det=False)
+    iso = _focus_inst(tmp_path, "i.json", None, with_det=False)
+    N = 200000
+    a = oracle.run(foc, N, [11], keep_state=True)["state"][0]
+    b = oracle.run(iso, N, [11], keep_state=True)["state"][0]
+    hit = b[:, 12] == 2.0  # scattered by the sample (prevcomp)
+    assert hit.sum() > 10000
+    cosmax = 1.0 / math.sqrt(1.0 + fr * fr)
+    v = a[hit, 3:6].astype(np.float64)
+    c = v @ np.array(t) / np.linalg.norm(v, axis=1)
+    assert c.min() > cosmax - 1e-6 and c.max() <= 1.0 + 1e-6
+    u = (c - cosmax) / (1.0 - cosmax)
+    assert abs(u.mean() - 0.5) < 4 * math.sqrt(1 / 12 / len(u))
+    assert abs((u < 0.25).mean() - 0.25) < 4 * math.sqrt(0.25 * 0.75 / len(u))
+    wf = np.float32(0.5 * (fr * fr / (1 + fr * fr)) / (1 + cosmax))
+    assert np.array_equal(a[hit, 9], (b[hit, 9] * wf).astype(np.float32))
+
+
+def test_incoherent_focusing_preserves_weight_into_the_cone(tmp_path, oracle):
+    # a detector inside the cone receives the same expected weight with and
+    # without focusing (importance sampling of the isotropic distribution)
+    t = (math.sin(0.7), 0.0, math.cos(0.7))
+    foc = _focus_inst(tmp_path, "f.json", t, 0.075)
+    iso = _focus_inst(tmp_path, "i.json", None)
+    wf = oracle.run(foc, 400000, [21])["hist"][0][0].sum()
+    wi_runs = [oracle.run(iso, 4000000, [31 + k])["hist"][0][0].sum() for k in range(2)]
+    wi = float(np.mean(wi_runs))
+    # isotropic: ~8e-4 of the histories reach the 0.01 sr plane (4 sigma)
+    rel = 4.0 / math.sqrt(8e6 * 8e-4)
+    assert wi > 0 and abs(wf / wi - 1.0) < rel
