@@ -69,6 +69,7 @@ extern "C" {
 #define RAMP_S_INCOHERENT 12 /* SIncoherent    constant-Sigma incoherent sample (no reference counterpart) */
 #define RAMP_S_ABSORBER 13   /* SAbsorber      Beer-Lambert absorber (no reference counterpart) */
 #define RAMP_S_DETPIX 14     /* SDetectorTubes tube x pixel x TOF bank (no reference counterpart) */
+#define RAMP_S_POWDERDIFF 15 /* SPowderDiff    differentiable powder + incoherent sample (no reference counterpart) */
 
 /* ---- plan descriptors --------------------------------------------------- */
 #define RAMP_NP 48  /* float parameters per component part */
@@ -190,6 +191,20 @@ int ramp_hist_reset(ramp_plan *plan);
 int ramp_state_read(ramp_plan *plan, float *neutrons16, float *isect8, uint32_t *iidx, uint64_t n);
 
 int ramp_stats_read(const ramp_plan *plan, ramp_run_stats *out);
+
+/* ---- gradients ----------------------------------------------------------
+ * Gradient of a loss of one detector's histogram with respect to the model
+ * parameters of the plan's SPowderDiff sample and its MCont source
+ * (no reference counterpart; the forward it differentiates is ramp_run with
+ * the same n_hist, chunk and seeds).  dL_dh holds dL/dh per bin of detector
+ * `det` (n_bins floats, the layout of ramp_hist_read); grad receives, in
+ * FP64, [dL/dT1, dL/dsigma_inc, dL/dsigma_abs, dL/dU, dL/dd_0 ..
+ * dL/dd_{n-1}] (n_grad = 4 + number of lines; T1 only for a
+ * single-temperature MCont, else 0).  opts: wg only.  The run statistics
+ * afterwards describe the adjoint pass. */
+int ramp_adjoint(ramp_plan *plan, uint64_t n_hist, uint64_t chunk, const uint32_t *seeds,
+                 uint32_t n_seeds, uint32_t det, const float *dL_dh, uint64_t n_bins, double *grad,
+                 uint32_t n_grad, const ramp_run_opts *opts);
 
 /* ---- batches of instrument settings --------------------------------------
  * A batch runs many plans - one per instrument setting (scan point, chopper

@@ -34,6 +34,14 @@ int oracle_run(const ramp_plan_desc *desc, uint64_t n_hist, uint64_t chunk,
                float *neutrons16, float *isect8, uint32_t *iidx, int n_threads,
                uint64_t *n_oob_out);
 
+/* FP64 reference gradient of a loss of detector `det`'s histogram (dL_dh:
+ * dL/dh per bin) over the same histories as oracle_run: grad =
+ * [T1, sigma_inc, sigma_abs, U, d_0 .. d_{nl-1}] of the plan's SPowderDiff
+ * and MCont source, n_grad = 4 + nl. */
+int oracle_adjoint(const ramp_plan_desc *desc, uint64_t n_hist, uint64_t chunk,
+                   const uint32_t *seeds, uint32_t n_seeds, uint32_t det, const float *dL_dh,
+                   double *grad, uint32_t n_grad, int n_threads);
+
 const char *oracle_last_error(void);
 
 #ifdef __cplusplus

@@ -33,6 +33,11 @@ def load():
                                    C.POINTER(C.POINTER(C.c_double)),
                                    C.POINTER(C.c_float), C.POINTER(C.c_float),
                                    C.POINTER(C.c_uint32), C.c_int, C.POINTER(C.c_uint64)]
+        lib.oracle_adjoint.restype = C.c_int
+        lib.oracle_adjoint.argtypes = [C.POINTER(capi.PlanDesc), C.c_uint64, C.c_uint64,
+                                       C.POINTER(C.c_uint32), C.c_uint32, C.c_uint32,
+                                       C.POINTER(C.c_float), C.POINTER(C.c_double), C.c_uint32,
+                                       C.c_int]
         lib.oracle_last_error.restype = C.c_char_p
         lib.oracle_threefry4x32.restype = None
         lib.oracle_threefry4x32.argtypes = [C.POINTER(C.c_uint32)] * 3
@@ -78,3 +83,20 @@ def run(inst, n_hist, seeds, chunk=0, keep_state=False, threads=0):
     if keep_state:
         out["state"] = (n16, i8, ii)
     return out
+
+
+def adjoint(inst, n_hist, seeds, det, dL_dh, n_grad, chunk=0, threads=0):
+    """FP64 reference gradient (oracle_adjoint) of a loss of detector slot
+    `det` of `inst`, given dL/dh (float32, one per bin)."""
+    lib = load()
+    desc = inst.plan_desc()
+    seeds = np.ascontiguousarray(seeds, dtype=np.uint32)
+    dl = np.ascontiguousarray(dL_dh, dtype=np.float32).reshape(-1)
+    grad = np.zeros(int(n_grad), np.float64)
+    rc = lib.oracle_adjoint(C.byref(desc), int(n_hist), int(chunk),
+                            seeds.ctypes.data_as(C.POINTER(C.c_uint32)), len(seeds), int(det),
+                            dl.ctypes.data_as(C.POINTER(C.c_float)),
+                            grad.ctypes.data_as(C.POINTER(C.c_double)), int(n_grad), int(threads))
+    if rc != 0:
+        raise RuntimeError(f"oracle error {rc}: {lib.oracle_last_error().decode()}")
+    return grad

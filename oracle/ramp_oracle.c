@@ -46,6 +46,10 @@ typedef struct {
     float is[8];     /* pos1, t1, pos2, t2 */
     uint32_t iidx;
     uint32_t tid;
+    /* adjoint pass only: d log w / d theta terms of this history (FP64) */
+    double a_gT, a_cs, a_ca, a_cU, a_xivv, a_cd;
+    float a_twok;
+    int a_ch;        /* chosen line, -1 incoherent, -2 no SPowderDiff scattering */
 } on_t;
 
 #define N_X s[0]
@@ -90,11 +94,48 @@ static inline float ocl_max(float x, float y) { return (x < y) ? y : x; }
 /* 2*M_PI*u evaluated as the reference does: double product, float result */
 static inline float twopi_u(float u) { return (float)(2 * RC_M_PI_D * (double)u); }
 
+/* adjoint pass: the loss detector, dL/dh, the SPowderDiff line table and
+ * the FP64 gradient sums [T, sigma_inc, sigma_abs, U, d_0..] */
+typedef struct {
+    int det;
+    const float *dLdh;
+    const float *lines;
+    uint32_t nl;
+    double *grad;
+} oadj;
+
 typedef struct {
     const ramp_plan_desc *d;
     double **hw, **hw2;
     uint64_t oob;
+    oadj *adj;
 } octx;
+
+/* the history a thread is running (the adjoint terms live in it) */
+static __thread on_t *t_cur;
+
+static void adj_event(octx *c, uint64_t idx, float w)
+{
+    const oadj *a = c->adj;
+    const on_t *n = t_cur;
+    double sw = (double)a->dLdh[idx] * (double)w;
+    double g[4] = {sw * n->a_gT, sw * n->a_cs, sw * n->a_ca, sw * n->a_cU};
+    for (int i = 0; i < 4; i++) {
+#pragma omp atomic
+        a->grad[i] += g[i];
+    }
+    if (n->a_ch == -2) return;
+    for (uint32_t j = 0; j < a->nl; j++) {
+        const float *L = a->lines + (size_t)j * PD_LW;
+        double t = 0.0;
+        if (L[PDL_Q0] < n->a_twok) t = -n->a_xivv * (double)L[PDL_SSV2] * (double)L[PDL_DLS];
+        if ((int)j == n->a_ch) t += n->a_cd;
+        if (t != 0.0) {
+#pragma omp atomic
+            a->grad[4 + j] += sw * t;
+        }
+    }
+}
 
 static inline void hist_add(octx *c, int det, uint64_t idx, float w)
 {
@@ -103,6 +144,10 @@ static inline void hist_add(octx *c, int det, uint64_t idx, float w)
     if (idx >= nb) {
 #pragma omp atomic
         c->oob++;
+        return;
+    }
+    if (c->adj) {
+        if (det == c->adj->det) adj_event(c, idx, w);
         return;
     }
     double wd = (double)w, w2d = (double)(w * w);
@@ -200,6 +245,13 @@ static void src_cont(on_t *n, const float *p, int num_sim, int seed)
     M += p[SP_I2] * maxwell_ref(E_J, T2) / dm2;
     M += p[SP_I3] * maxwell_ref(E_J, T3) / dm3;
     n->N_P = M / (float)num_sim;
+    /* adjoint: d log w / dT1 of a single-temperature source, in FP64:
+     * w ~ sqrt(E/kT) e^{1 - E/kT}, and ~ T on the 1/E tail */
+    n->a_gT = 0.0;
+    if (p[SP_I2] == 0.0f && p[SP_I3] == 0.0f) {
+        double kT = (double)RC_K_B * (double)T1;
+        n->a_gT = (join > 0.0f && E_J > EjJ) ? 1.0 / (double)T1 : ((double)E_J / kT - 0.5) / (double)T1;
+    }
     float vel = RC_SE2V * rm_sqrt(E_val);
     src_position_direction(n, p, vel);
     src_polarisation(n);
@@ -1197,6 +1249,115 @@ static void s_incoherent(on_t *n, const ramp_part_desc *s, uint32_t ci)
     s_done(n, ci);
 }
 
+/* SPowderDiff (no reference counterpart): powder lines + incoherent +
+ * absorption drawn from a fixed proposal, weight = model / proposal
+ * probability (depth, channel, grain d-spacing); see the product's
+ * statement of the same model.  Float arithmetic in the product's order;
+ * the adjoint terms in FP64. */
+static void s_powderdiff(on_t *n, const float *Lt, const ramp_part_desc *s, uint32_t ci)
+{
+    const float *p = s->p;
+    const uint32_t nl = s->ni[PDI_NLINES];
+    rf3 path = sample_path(n);
+    float Lp = v3_len(path);
+    float v = v3_len(n_vel(n));
+    float k = RC_V2K * v;
+    float vv = v * v * RC_V2K * RC_V2K;
+    float twok = 2.0f * k;
+    float lam = 6.28318530717958647692f / k;
+    float inc = 100.0f * p[PD_SINC], inc0 = 100.0f * p[PD_SINC0];
+    float ab = 100.0f * p[PD_SABS] * (lam / p[PD_LREF]);
+    float ab0 = 100.0f * p[PD_SABS0] * (lam / p[PD_LREF]);
+    float coh = 0.0f, coh0 = 0.0f;
+    for (uint32_t j = 0; j < nl; j++) {
+        const float *L = Lt + j * PD_LW;
+        if (L[PDL_Q0] < twok) {
+            coh += L[PDL_SSV2] / vv;
+            coh0 += L[PDL_SSV2_0] / vv;
+        }
+    }
+    float mus = inc + coh, mu = mus + ab;
+    float mus0 = inc0 + coh0, mu0 = mus0 + ab0;
+    (void)mus;
+    if (!(mus0 > 0.0f) || !(Lp > 0.0f)) {
+        n->N_DIE = 1.0f;
+        s_done(n, ci);
+        return;
+    }
+    float pint0 = 1.0f - rm_exp(-mu0 * Lp);
+    float x = -rm_log(1.0f - ornd(n) * pint0) / mu0;
+    float uc = ornd(n) * mus0;
+    int ch = -1;
+    float sig = inc, sig0 = inc0;
+    if (uc >= inc0) {
+        float acc = inc0;
+        for (uint32_t j = 0; j < nl; j++) {
+            const float *L = Lt + j * PD_LW;
+            if (L[PDL_Q0] < twok) {
+                ch = (int)j;
+                acc += L[PDL_SSV2_0] / vv;
+                if (uc < acc) break;
+            }
+        }
+        if (ch >= 0) {
+            sig = Lt[ch * PD_LW + PDL_SSV2] / vv;
+            sig0 = Lt[ch * PD_LW + PDL_SSV2_0] / vv;
+        }
+    }
+    n->N_P *= (sig * rm_exp(-mu * x)) * pint0 * mus0 / ((mu0 * rm_exp(-mu0 * x)) * sig0);
+    float frac = x / Lp;
+    int inside = v3_eq(is_pos1(n), is_pos2(n));
+    rf3 start = inside ? n_pos(n) : is_pos1(n);
+    float t_start = inside ? 0.0f : n->is[3];
+    set_pos(n, v3_add(start, v3_scale(frac, path)));
+    n->N_TOF += t_start + x / v;
+    double cd = 0.0;
+    if (ch < 0) {
+        float ct = 1.0f - 2.0f * ornd(n);
+        float ph = 6.28318530717958647692f * ornd(n);
+        float st = rm_sqrt(rm_fmax(0.0f, 1.0f - ct * ct));
+        set_vel(n, rf3_make(v * (st * rm_cos(ph)), v * (st * rm_sin(ph)), v * ct));
+    } else {
+        const float *L = Lt + ch * PD_LW;
+        float d = L[PDL_D], d0 = L[PDL_D0], sr = p[PD_STRAIN];
+        float u1 = rm_fmax(ornd(n), 1e-30f), u2 = ornd(n);
+        float g = rm_sqrt(-2.0f * rm_log(u1)) * rm_cos(6.28318530717958647692f * u2);
+        float D = d0 * (1.0f + sr * g);
+        float z = (D - d) / (sr * d), z0 = (D - d0) / (sr * d0);
+        n->N_P *= (d0 / d) * rm_exp(0.5f * (z0 * z0 - z * z));
+        /* d/dd of log sigma_ch + log(d0/d) - z^2/2 */
+        cd = (double)L[PDL_DLS] - 1.0 / (double)d +
+             (double)z * ((double)D / ((double)sr * (double)d * (double)d));
+        float sth = 3.14159265358979323846f / (k * D);
+        if (!(sth < 1.0f) || !(D > 0.0f)) {
+            n->N_DIE = 1.0f;
+            s_done(n, ci);
+            return;
+        }
+        float tt = 2.0f * rm_asin(sth);
+        rf3 para, perp;
+        powder_dirs(n_vel(n), &para, &perp);
+        float phi = 6.28318530717958647692f * ornd(n);
+        perp = rot_about_axis(phi, para, perp);
+        para = rot_about_axis(tt, perp, para);
+        set_vel(n, v3_scale(v, v3_norm(para)));
+    }
+    /* adjoint terms: log w depends on the model through sig_ch and -mu x */
+    double sumq2 = 0.0;
+    for (uint32_t j = 0; j < nl; j++) {
+        const float *L = Lt + j * PD_LW;
+        if (L[PDL_Q0] < twok) sumq2 += ((double)L[PDL_SSV2] / (double)vv) * (double)L[PDL_Q2];
+    }
+    n->a_cs = (ch < 0 && p[PD_SINC] > 0.0f ? 1.0 / (double)p[PD_SINC] : 0.0) - 100.0 * (double)x;
+    n->a_ca = -100.0 * (double)x * ((double)lam / (double)p[PD_LREF]);
+    n->a_cU = (ch >= 0 ? -(double)Lt[ch * PD_LW + PDL_Q2] : 0.0) + (double)x * sumq2;
+    n->a_xivv = (double)x / (double)vv;
+    n->a_twok = twok;
+    n->a_cd = cd;
+    n->a_ch = ch;
+    s_done(n, ci);
+}
+
 /* SAbsorber (no reference counterpart): Beer-Lambert transmission along the
  * segment pos1 -> pos2; weight *= exp(-Sigma_a L); the neutron moves to pos2. */
 static void s_absorber(on_t *n, const ramp_part_desc *s, uint32_t ci)
@@ -1256,6 +1417,7 @@ static void o_scatter(octx *cx, on_t *n, const ramp_component_desc *c)
     case RAMP_S_INCOHERENT: s_incoherent(n, s, c->comp_idx); break;
     case RAMP_S_ABSORBER: s_absorber(n, s, c->comp_idx); break;
     case RAMP_S_DETPIX: s_detpix(cx, n, c); break;
+    case RAMP_S_POWDERDIFF: s_powderdiff(n, tab(cx, s, PDT_LINES), s, c->comp_idx); break;
     default: break;
     }
 }
@@ -1304,6 +1466,9 @@ static void o_source(octx *cx, on_t *n, int num_sim, int seed)
     }
 }
 
+/* set by oracle_adjoint around its oracle_run call */
+static oadj *g_adj;
+
 /* Instrument.execute: mcramp.py:403-455 (chunk loop; buffers zeroed per
  * execute; the rng counter and iidx carry over between chunks per slot) */
 int oracle_run(const ramp_plan_desc *desc, uint64_t n_hist, uint64_t chunk, const uint32_t *seeds,
@@ -1336,6 +1501,7 @@ int oracle_run(const ramp_plan_desc *desc, uint64_t n_hist, uint64_t chunk, cons
     cx.hw = hist_w;
     cx.hw2 = hist_w2;
     cx.oob = 0;
+    cx.adj = g_adj;
 #ifdef _OPENMP
     if (n_threads > 0) omp_set_num_threads(n_threads);
 #else
@@ -1351,6 +1517,10 @@ int oracle_run(const ramp_plan_desc *desc, uint64_t n_hist, uint64_t chunk, cons
         for (int64_t i = 0; i < nt; i++) {
             on_t *n = &slots[i];
             n->tid = (uint32_t)i;
+            t_cur = n;
+            n->a_gT = n->a_cs = n->a_ca = n->a_cU = n->a_xivv = n->a_cd = 0.0;
+            n->a_twok = 0.0f;
+            n->a_ch = -2;
             o_source(&cx, n, (int)torun, seed);
             for (uint32_t b = 0; b < desc->n_blocks; b++) o_block(&cx, n, &desc->blocks[b]);
         }
@@ -1365,6 +1535,41 @@ int oracle_run(const ramp_plan_desc *desc, uint64_t n_hist, uint64_t chunk, cons
     if (n_oob_out) *n_oob_out = cx.oob;
     free(slots);
     return 0;
+}
+
+/* FP64 reference gradient of a histogram loss (the adjoint pass of the
+ * product, restated): the same histories as oracle_run, and at every event
+ * of detector `det` grad += dL/dh[bin] * w * d log w / d theta, all in FP64.
+ * grad: [T1, sigma_inc, sigma_abs, U, d_0 .. d_{nl-1}], n_grad = 4 + nl. */
+int oracle_adjoint(const ramp_plan_desc *desc, uint64_t n_hist, uint64_t chunk, const uint32_t *seeds,
+                   uint32_t n_seeds, uint32_t det, const float *dL_dh, double *grad, uint32_t n_grad,
+                   int n_threads)
+{
+    if (!desc || !dL_dh || !grad || det >= desc->n_dets) {
+        snprintf(g_err, sizeof g_err, "bad adjoint arguments");
+        return RAMP_E_INVALID;
+    }
+    oadj a;
+    memset(&a, 0, sizeof a);
+    a.det = (int)det;
+    a.dLdh = dL_dh;
+    a.grad = grad;
+    for (uint32_t k = 0; k < desc->n_comps; k++) {
+        const ramp_part_desc *s = &desc->comps[k].scat;
+        if (s->type == RAMP_S_POWDERDIFF) {
+            a.lines = desc->tables + s->tab_off[PDT_LINES];
+            a.nl = s->ni[PDI_NLINES];
+        }
+    }
+    if (n_grad != 4 + a.nl) {
+        snprintf(g_err, sizeof g_err, "gradient needs %u entries", 4 + a.nl);
+        return RAMP_E_INVALID;
+    }
+    memset(grad, 0, n_grad * sizeof(double));
+    g_adj = &a;
+    int rc = oracle_run(desc, n_hist, chunk, seeds, n_seeds, NULL, NULL, NULL, NULL, NULL, n_threads, NULL);
+    g_adj = NULL;
+    return rc;
 }
 
 /* Test hook: the full 4-word Threefry-4x32-20 block of the shared RNG

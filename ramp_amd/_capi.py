@@ -149,6 +149,9 @@ _SIGS = {
     "ramp_batch_run": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32),
                                  C.POINTER(C.c_double), C.POINTER(RunOpts)]),
     "ramp_batch_stats_read": (C.c_int, [C.c_void_p, C.POINTER(RunStats)]),
+    "ramp_adjoint": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint32), C.c_uint32,
+                               C.c_uint32, C.POINTER(C.c_float), C.c_uint64, C.POINTER(C.c_double),
+                               C.c_uint32, C.POINTER(RunOpts)]),
     "ramp_last_error": (C.c_char_p, []),
     "ramp_version": (C.c_char_p, []),
 }

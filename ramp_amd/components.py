@@ -896,6 +896,82 @@ class SIncoherent(_Scat):
             self.p[S.IN_WF] = np.float32(0.5 * omc)
 
 
+class SPowderDiff(_Scat):
+    """Powder sample for gradients: Bragg lines (cross-sections as SPowderN,
+    scat/powderN.py:59-64, with the Debye-Waller factor exp(-U q^2)),
+    incoherent scattering sigma_inc and absorption sigma_abs (1/cm, the
+    latter at lambda_ref, scaling with wavelength), and a Gaussian spread
+    of each line's d-spacing of relative width `strain`.  No reference
+    counterpart.
+
+    Every random choice is drawn from a fixed proposal - the same sample
+    at `proposal` (a dict of any of d_spacing, F2, U, sigma_inc,
+    sigma_abs; default: the model itself) - and the weight carries the
+    exact ratio of model to proposal probabilities: depth of the single
+    scattering (truncated exponential), channel (incoherent or line j),
+    and the grain's d-spacing.  Trajectories therefore do not depend on the
+    model parameters, the weight is a smooth function of them, and the
+    gradient of any histogram loss is the sum over detector events of
+    dL/dh_bin * dw/dtheta (ramp_adjoint).  Scattering is forced (single
+    scattering with weight correction, no exit attenuation); a line is
+    active while the proposal's q0 < 2k."""
+    type_code = capi.RAMP_S_POWDERDIFF
+    N_LW = 8
+
+    def __init__(self, d_spacing=(2.0,), F2=(1.0,), multiplicity=None, pack=1.0, vc=1.0, DW=1.0,
+                 U=0.0, sigma_inc=0.0, sigma_abs=0.0, lambda_ref=1.8, strain=1e-3, proposal=None,
+                 idx=0, ctx=None, **kwargs):
+        super().__init__()
+        self.idx = np.uint32(idx)
+        d = np.asarray(d_spacing, dtype=np.float64)
+        F2 = np.asarray(F2, dtype=np.float64)
+        mult = np.ones_like(d) if multiplicity is None else np.asarray(multiplicity, np.float64)
+        prop = dict(proposal or {})
+        d0 = np.asarray(prop.get("d_spacing", d), dtype=np.float64)
+        F20 = np.asarray(prop.get("F2", F2), dtype=np.float64)
+        U0 = float(prop.get("U", U))
+        if not (len(d) == len(F2) == len(mult) == len(d0) == len(F20)) or len(d) == 0:
+            raise ValueError("SPowderDiff: d_spacing, F2, multiplicity and the proposal's lines must "
+                             "have the same non-zero length")
+        if strain <= 0:
+            raise ValueError("SPowderDiff: strain must be > 0")
+        self.model = {"d_spacing": d, "F2": F2, "U": float(U), "sigma_inc": float(sigma_inc),
+                      "sigma_abs": float(sigma_abs)}
+        self.kw = {"multiplicity": mult, "pack": pack, "vc": vc, "DW": DW, "lambda_ref": lambda_ref,
+                   "strain": strain}
+        self.prop = {"d_spacing": d0, "F2": F20, "U": U0,
+                     "sigma_inc": float(prop.get("sigma_inc", sigma_inc)),
+                     "sigma_abs": float(prop.get("sigma_abs", sigma_abs))}
+
+        def ssv2(dd, ff, uu):
+            q = 2 * np.pi / dd
+            return _powder_sigma(dd, pack, vc, 0.0, mult, DW, ff * np.exp(-uu * q * q))[1]
+
+        q = 2 * np.pi / d
+        lines = np.zeros((len(d), self.N_LW), dtype=np.float32)
+        lines[:, S.PDL_SSV2] = ssv2(d, F2, float(U))
+        lines[:, S.PDL_SSV2_0] = ssv2(d0, F20, U0)
+        lines[:, S.PDL_Q0] = (2 * np.pi / d0).astype(np.float32)
+        lines[:, S.PDL_D] = d.astype(np.float32)
+        lines[:, S.PDL_D0] = d0.astype(np.float32)
+        lines[:, S.PDL_DLS] = (1.0 / d + 2.0 * float(U) * q * q / d).astype(np.float32)
+        lines[:, S.PDL_Q2] = (q * q).astype(np.float32)
+        self.lines = lines
+        self.num_lines = np.uint32(len(d))
+        self.p[S.PD_SINC] = np.float32(sigma_inc)
+        self.p[S.PD_SABS] = np.float32(sigma_abs)
+        self.p[S.PD_LREF] = np.float32(lambda_ref)
+        self.p[S.PD_U] = np.float32(U)
+        self.p[S.PD_STRAIN] = np.float32(strain)
+        self.p[S.PD_SINC0] = np.float32(self.prop["sigma_inc"])
+        self.p[S.PD_SABS0] = np.float32(self.prop["sigma_abs"])
+        self.ni[S.PDI_NLINES] = self.num_lines
+        self.tables[S.PDT_LINES] = lines.reshape(-1)
+
+    def param_names(self):
+        return ["sigma_inc", "sigma_abs", "U"] + [f"d{j}" for j in range(int(self.num_lines))]
+
+
 class SAbsorber(_Scat):
     """Pure Beer-Lambert absorber (weight *= exp(-sigma_a L)).  No reference
     counterpart."""
@@ -974,4 +1050,4 @@ SOURCES = ("MFlat", "MCont", "MISIS", "MLambda", "MParallel")
 GEOMS = ("GArm", "GPlane", "GBox", "GSphere", "GSphereInt", "GCylinderExt", "GBanana", "GMesh")
 SCATS = ("SArm", "SBeamstop", "SChopper", "SGuide", "SIsotropic", "SPowder1", "SPowderN",
          "SMonochromator", "SDetector1D", "SDetector2D", "SCounter", "SIncoherent", "SAbsorber",
-         "SDetectorTubes")
+         "SDetectorTubes", "SPowderDiff")

@@ -438,6 +438,7 @@ DEV void geom_dispatch(NS &n, const DComp &c)
 /* per-lane event tallies are kept in registers and reduced once per wave
  * at the end of a segment kernel (no same-address atomics on the hot path) */
 struct HistCtx {
+    static constexpr bool adjoint = false;
     const DDet *dets;
     int mode;       /* RAMP_HIST_* */
     int enabled;    /* 0 in pilot runs */
@@ -469,6 +470,8 @@ DEV void hist_add(HistCtx &hc, int det, uint64_t idx, float w)
     hc.events++;
 }
 
+#include "dev_adjoint.h"
+
 /* wavelength variable: 2*M_PI/(V2K*|v|) with a double numerator */
 DEV float det_wavelength(const NS &n, const DPart &s)
 {
@@ -496,7 +499,8 @@ DEV void det_finish(NS &n, uint32_t restore)
 }
 
 /* scat/detector1D.cl:20-77 */
-DEV void s_det1d(NS &n, const DComp &c, HistCtx &hc)
+template <class HC>
+DEV void s_det1d(NS &n, const DComp &c, HC &hc)
 {
     const DPart &s = c.s;
     float mn = s.p[D1_MIN], st = s.p[D1_STEP], mx = s.p[D1_MAX];
@@ -540,7 +544,8 @@ DEV int det2d_idx(float v, float mn, float st, float mx)
 }
 
 /* scat/detector2D.cl:31-186 */
-DEV void s_det2d(NS &n, const DComp &c, HistCtx &hc)
+template <class HC>
+DEV void s_det2d(NS &n, const DComp &c, HC &hc)
 {
     const DPart &s = c.s;
     float v1 = det2d_var(n, s, s.ni[D2I_VAR1], 1);
@@ -556,7 +561,8 @@ DEV void s_det2d(NS &n, const DComp &c, HistCtx &hc)
 }
 
 /* tube x pixel x TOF bank (no reference counterpart) */
-DEV void s_detpix(NS &n, const DComp &c, HistCtx &hc)
+template <class HC>
+DEV void s_detpix(NS &n, const DComp &c, HC &hc)
 {
     const DPart &s = c.s;
     uint32_t nt = s.ni[DTI_NTUBE], np = s.ni[DTI_NPIX], nf = s.ni[DTI_NTOF];
@@ -575,7 +581,8 @@ DEV void s_detpix(NS &n, const DComp &c, HistCtx &hc)
 }
 
 /* scat/counter.cl:18-49 */
-DEV void s_counter(NS &n, const DComp &c, HistCtx &hc)
+template <class HC>
+DEV void s_counter(NS &n, const DComp &c, HC &hc)
 {
     hist_add(hc, c.det, 0, n.w);
     n.se = 1.0f;
@@ -736,8 +743,8 @@ DEV void s_absorber(NS &n, const DComp &c)
 
 #include "dev_samples.h"
 
-template <bool FULL, uint32_t F>
-DEV void scat_dispatch(NS &n, const DComp &c, HistCtx &hc)
+template <bool FULL, uint32_t F, class HC>
+DEV void scat_dispatch(NS &n, const DComp &c, HC &hc)
 {
     if (n.iidx != c.comp_idx) return;
     if (n.dead) return;
@@ -770,6 +777,9 @@ DEV void scat_dispatch(NS &n, const DComp &c, HistCtx &hc)
     case RAMP_S_INCOHERENT: s_incoherent(n, c); break;
     case RAMP_S_ABSORBER: s_absorber(n, c); break;
     case RAMP_S_DETPIX: s_detpix(n, c, hc); break;
+    case RAMP_S_POWDERDIFF:
+        if constexpr ((F & FEAT_SAMPLE) != 0) s_powderdiff(n, c, hc);
+        break;
     default: break;
     }
 }

@@ -286,6 +286,131 @@ DEV float mono_gauss(float x, float rms, float sq2pi)
  * d[2] mos_rms_max, d[3] Q, d[4] zmax, d[5] ymax, d[8] sqrt(2 pi),
  * d[9]/d[10] 2*pi split.  Box-Muller's cos and the GAUSS weights are FP32
  * approximations of the reference's double evaluation. */
+/* Differentiable powder + incoherent sample (no reference counterpart).
+ * Depth, channel and grain d-spacing are drawn from the proposal; the
+ * weight is multiplied by model / proposal probability:
+ *   depth x and channel ch:  sig_ch e^{-mu x} (1 - e^{-mu0 L}) mus0 / (mu0 e^{-mu0 x} sig0_ch)
+ *   grain d-spacing D of line j:  N(D; d_j, s d_j) / N(D; d0_j, s d0_j)
+ * with mu = mus + mu_abs, mus = incoherent + active lines; at the proposal
+ * the first factor is the forced-scattering weight (1 - e^{-mu L}) mus/mu
+ * and the second is 1.  The incoherent direction is uniform on the sphere;
+ * a line scatters by 2 asin(pi / (k D)) on a uniform Debye-Scherrer cone.
+ * In the adjoint pass the log-derivative terms go to the context. */
+template <class HC>
+DEV void s_powderdiff(NS &n, const DComp &c, HC &hc)
+{
+    const DPart &s = c.s;
+    const float *Lt = s.tab[PDT_LINES];
+    const uint32_t nl = s.ni[PDI_NLINES];
+    rf3 path = sample_path(n);
+    float Lp = v3_len(path);
+    float v = v3_len(nvel(n));
+    float k = RC_V2K * v;
+    float vv = v * v * RC_V2K * RC_V2K;
+    float twok = 2.0f * k;
+    float lam = 6.28318530717958647692f / k;
+    float inc = 100.0f * s.p[PD_SINC], inc0 = 100.0f * s.p[PD_SINC0];
+    float ab = 100.0f * s.p[PD_SABS] * (lam / s.p[PD_LREF]);
+    float ab0 = 100.0f * s.p[PD_SABS0] * (lam / s.p[PD_LREF]);
+    float coh = 0.0f, coh0 = 0.0f;
+    for (uint32_t j = 0; j < nl; j++) {
+        const float *L = Lt + j * PD_LW;
+        if (L[PDL_Q0] < twok) {
+            coh += L[PDL_SSV2] / vv;
+            coh0 += L[PDL_SSV2_0] / vv;
+        }
+    }
+    float mus = inc + coh, mu = mus + ab;
+    float mus0 = inc0 + coh0, mu0 = mus0 + ab0;
+    if (!(mus0 > 0.0f) || !(Lp > 0.0f)) {
+        n.dead = true;
+        s_done(n, c.comp_idx);
+        return;
+    }
+    float pint0 = 1.0f - rm_exp(-mu0 * Lp);
+    float x = -rm_log(1.0f - drw(n) * pint0) / mu0;
+    /* channel from the proposal: incoherent, then the active lines in order */
+    float uc = drw(n) * mus0;
+    int ch = -1;
+    float sig = inc, sig0 = inc0;
+    if (uc >= inc0) {
+        float acc = inc0;
+        for (uint32_t j = 0; j < nl; j++) {
+            const float *L = Lt + j * PD_LW;
+            if (L[PDL_Q0] < twok) {
+                ch = (int)j;
+                acc += L[PDL_SSV2_0] / vv;
+                if (uc < acc) break;
+            }
+        }
+        if (ch >= 0) {
+            sig = Lt[ch * PD_LW + PDL_SSV2] / vv;
+            sig0 = Lt[ch * PD_LW + PDL_SSV2_0] / vv;
+        } else {
+            sig = inc;
+            sig0 = inc0;
+        }
+    }
+    n.w *= (sig * rm_exp(-mu * x)) * pint0 * mus0 / ((mu0 * rm_exp(-mu0 * x)) * sig0);
+    float frac = x / Lp;
+    bool inside = v3_eq(ipos1(n), ipos2(n));
+    rf3 start = inside ? npos(n) : ipos1(n);
+    float t_start = inside ? 0.0f : n.is[3];
+    setpos(n, v3_add(start, v3_scale(frac, path)));
+    n.t += t_start + x / v;
+    float cd = 0.0f;
+    if (ch < 0) {
+        float ct = 1.0f - 2.0f * drw(n);
+        float ph = 6.28318530717958647692f * drw(n);
+        float st = rm_sqrt(rm_fmax(0.0f, 1.0f - ct * ct));
+        n.vx = v * (st * rm_cos(ph));
+        n.vy = v * (st * rm_sin(ph));
+        n.vz = v * ct;
+    } else {
+        const float *L = Lt + ch * PD_LW;
+        float d = L[PDL_D], d0 = L[PDL_D0], sr = s.p[PD_STRAIN];
+        float u1 = rm_fmax(drw(n), 1e-30f), u2 = drw(n);
+        float g = rm_sqrt(-2.0f * rm_log(u1)) * rm_cos(6.28318530717958647692f * u2);
+        float D = d0 * (1.0f + sr * g);
+        float z = (D - d) / (sr * d), z0 = (D - d0) / (sr * d0);
+        n.w *= (d0 / d) * rm_exp(0.5f * (z0 * z0 - z * z));
+        cd = L[PDL_DLS] - 1.0f / d + z * (D / (sr * d * d));
+        float sth = 3.14159265358979323846f / (k * D);
+        if (!(sth < 1.0f) || !(D > 0.0f)) {
+            n.dead = true;
+            s_done(n, c.comp_idx);
+            return;
+        }
+        float tt = 2.0f * rm_asin(sth);
+        rf3 para, perp;
+        powder_dirs(nvel(n), para, perp);
+        float phi = 6.28318530717958647692f * drw(n);
+        perp = rot_about_axis(phi, para, perp);
+        para = rot_about_axis(tt, perp, para);
+        setvel(n, v3_scale(v, v3_norm(para)));
+    }
+    if constexpr (HC::adjoint) {
+        /* d log w: sig_ch and -mu x depend on the model */
+        float sumq2 = 0.0f;
+        int m = 0;
+        for (uint32_t j = 0; j < nl; j++) {
+            const float *L = Lt + j * PD_LW;
+            if (L[PDL_Q0] < twok) {
+                sumq2 += (L[PDL_SSV2] / vv) * L[PDL_Q2];
+                m++;
+            }
+        }
+        hc.cs = (ch < 0 && s.p[PD_SINC] > 0.0f ? 1.0f / s.p[PD_SINC] : 0.0f) - 100.0f * x;
+        hc.ca = -100.0f * x * (lam / s.p[PD_LREF]);
+        hc.cU = (ch >= 0 ? -Lt[ch * PD_LW + PDL_Q2] : 0.0f) + x * sumq2;
+        hc.xivv = x / vv;
+        hc.cd = cd;
+        hc.ch = ch;
+        hc.m = m;
+    }
+    s_done(n, c.comp_idx);
+}
+
 /* slab of the monochromator grid (monochromator.cl:62-82): tilts, centre
  * and the rotation into the slab frame, from the slab's column and row */
 struct MonoSlab {

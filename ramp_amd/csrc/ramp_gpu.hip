@@ -623,6 +623,7 @@ struct ramp_plan {
     bool has_atomic = false, has_det = false;
     std::vector<int> bounds;  /* segment boundaries (step indices) */
     int defer_k = -1;         /* first linear monochromator step (reflections deferred) */
+    std::vector<float> pd_lines; /* host copy of the SPowderDiff line table (adjoint) */
     bool bounds_set = false;
     int variant = kNumVariants - 1;
     uint32_t feat = FEAT_ALL;  /* FEAT_* bits the plan's components need */
@@ -923,7 +924,7 @@ static int validate_plan(const ramp_plan_desc *d)
         const ramp_component_desc &c = d->comps[k];
         if (c.geom.type < RAMP_G_ARM || c.geom.type > RAMP_G_MESH)
             return fail(RAMP_E_INVALID, "component %u: unknown geometry type %u", k, c.geom.type);
-        if (c.scat.type < RAMP_S_ARM || c.scat.type > RAMP_S_DETPIX)
+        if (c.scat.type < RAMP_S_ARM || c.scat.type > RAMP_S_POWDERDIFF)
             return fail(RAMP_E_INVALID, "component %u: unknown scatter type %u", k, c.scat.type);
         if (c.comp_idx == 0 || c.comp_idx > 0x00ffffffu)
             return fail(RAMP_E_INVALID, "component %u: index %u out of range", k, c.comp_idx);
@@ -949,6 +950,11 @@ static int validate_plan(const ramp_plan_desc *d)
         if (s.type == RAMP_S_MONO &&
             (s.tab_len[MOT_GX] < s.ni[MOI_GLEN] || s.tab_len[MOT_GW] < s.ni[MOI_GLEN]))
             return fail(RAMP_E_INVALID, "component %u: quadrature tables too short", k);
+        if (s.type == RAMP_S_POWDERDIFF &&
+            (s.ni[PDI_NLINES] == 0 || s.tab_len[PDT_LINES] < (uint64_t)s.ni[PDI_NLINES] * PD_LW ||
+             !(s.p[PD_STRAIN] > 0.0f) || !(s.p[PD_LREF] > 0.0f)))
+            return fail(RAMP_E_INVALID, "component %u: SPowderDiff needs lines, a line table, strain > 0 "
+                        "and lambda_ref > 0", k);
         if (s.type == RAMP_S_ISOTROPIC)
             for (int ch = 0; ch < 3; ch++) {
                 uint64_t nq = s.ni[ISI_NQ(ch)], nw = s.ni[ISI_NW(ch)];
@@ -993,6 +999,9 @@ extern "C" int ramp_plan_create(ramp_ctx *ctx, const ramp_plan_desc *d, ramp_pla
         PCHK(copy_part(dc.s, c.scat, dtab, d->n_tables));
         derive_geom(dc.g);
         derive_scat(dc.s);
+        if (c.scat.type == RAMP_S_POWDERDIFF && d->tables)
+            p->pd_lines.assign(d->tables + c.scat.tab_off[PDT_LINES],
+                               d->tables + c.scat.tab_off[PDT_LINES] + (size_t)c.scat.ni[PDI_NLINES] * PD_LW);
         if (c.geom.type == RAMP_G_MESH)
             PCHK(build_mesh(p, dc.g, dtab + c.geom.tab_off[GT_POINTS], c.geom.ni[GI_NTRI]));
         dc.comp_idx = c.comp_idx;
@@ -1007,7 +1016,8 @@ extern "C" int ramp_plan_create(ramp_ctx *ctx, const ramp_plan_desc *d, ramp_pla
     for (uint32_t k = 0; k < d->n_comps; k++) {
         uint32_t g = d->comps[k].geom.type, sc = d->comps[k].scat.type;
         if (g == RAMP_G_MESH) feat |= FEAT_MESH;
-        if (sc == RAMP_S_ISOTROPIC || sc == RAMP_S_POWDER1 || sc == RAMP_S_POWDERN) feat |= FEAT_SAMPLE;
+        if (sc == RAMP_S_ISOTROPIC || sc == RAMP_S_POWDER1 || sc == RAMP_S_POWDERN || sc == RAMP_S_POWDERDIFF)
+            feat |= FEAT_SAMPLE;
         if (sc == RAMP_S_MONO) feat |= FEAT_MONO;
     }
     p->feat = feat;
@@ -1532,3 +1542,4 @@ extern "C" int ramp_rng_eval(ramp_ctx *ctx, const uint32_t *tid, const float *ct
 }
 
 #include "batch_run.h"
+#include "adjoint_run.h"

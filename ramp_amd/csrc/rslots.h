@@ -139,6 +139,25 @@
 #define IN_E2 10              /* 10..12 u x e1 */
 #define IN_OMC 13             /* 1 - cos(cone half-angle) */
 #define IN_WF 14              /* weight factor: solid-angle fraction (1 - cos)/2 */
+/* SPowderDiff (no reference counterpart): model parameters (differentiated)
+ * and the proposal they are sampled from (fixed) */
+#define PD_SINC 0             /* incoherent Sigma_s, 1/cm */
+#define PD_SABS 1             /* absorption Sigma_a at PD_LREF, 1/cm */
+#define PD_LREF 2             /* Angstrom */
+#define PD_U 3                /* Debye-Waller U, A^2 */
+#define PD_STRAIN 4           /* relative rms width of a line's d-spacing */
+#define PD_SINC0 5            /* proposal Sigma_s */
+#define PD_SABS0 6            /* proposal Sigma_a */
+#define PDI_NLINES 0
+#define PDT_LINES 0           /* PD_LW floats per line: */
+#define PD_LW 8
+#define PDL_SSV2 0            /*   model sigma * v^2 (powderN convention, with exp(-U q^2)) */
+#define PDL_SSV2_0 1          /*   proposal sigma * v^2 */
+#define PDL_Q0 2              /*   proposal q = 2 pi / d0 (line active while q0 < 2k) */
+#define PDL_D 3               /*   model d */
+#define PDL_D0 4              /*   proposal d */
+#define PDL_DLS 5             /*   d log(sigma) / d d (model) */
+#define PDL_Q2 6              /*   model q^2 */
 /* SAbsorber (no reference counterpart) */
 #define AB_SIGMA_A 0          /* 1/cm */
 /* SDetectorTubes (no reference counterpart) */

@@ -1,6 +1,7 @@
 """Helpers shared by the GPU parity tests: run an instrument on the GPU
 (libramp.so) and on the CPU oracle with the same seeds, and compare."""
 import os
+import sys
 
 import numpy as np
 
@@ -40,7 +41,16 @@ def instrument_cases():
                           {"v_foc": 841.3}),
         "powder": (os.path.join(ROOT, "tests", "instruments", "powder.json"), {"Ei": 25.0}),
         "tas": (os.path.join(ROOT, "tests", "instruments", "tas.json"), {"Mono_angle": ang}),
+        "powder_diff": (os.path.join(ROOT, "ramp_amd", "instruments", "powder_diff.json"),
+                        _powder_diff_vars()),
     }
+
+
+def _powder_diff_vars():
+    sys.path.insert(0, os.path.join(ROOT, "ramp_amd", "instruments"))
+    import powder_diff
+    d, F2 = powder_diff.lines()
+    return powder_diff.variables(powder_diff.theta0(), F2)
 
 
 def run_gpu(path, kw, n, seeds, chunk=None, hist_mode="deterministic", keep_state=True):

@@ -235,3 +235,31 @@ def test_batch_argument_errors_without_a_gpu(lib):
     st = _capi.RunStats()
     rc, _ = _err(lib, lib.ramp_batch_stats_read(None, C.byref(st)))
     assert rc == _capi.RAMP_E_INVALID
+
+
+def test_adjoint_argument_errors_without_a_gpu(lib):
+    g = (C.c_double * 4)()
+    rc, msg = _err(lib, lib.ramp_adjoint(None, 10, 10, None, 0, 0, None, 0, g, 4, None))
+    assert rc == _capi.RAMP_E_INVALID and "plan" in msg
+
+
+def test_powderdiff_line_table_and_log_derivatives():
+    d = [2.0, 3.0]
+    p = ramp_amd.SPowderDiff(d_spacing=d, F2=[10.0, 20.0], pack=1.0, vc=40.0, U=0.005,
+                             sigma_inc=0.2, sigma_abs=0.1, proposal={"d_spacing": [2.1, 2.9], "U": 0.004})
+    L = p.lines
+    q = 2 * np.pi / np.array(d)
+    # sigma v^2 of SPowderN (powderN.py:59-64) with F2 -> F2 exp(-U q^2)
+    s = ramp_amd.SPowderN(d_spacing=d, pack=1.0, vc=40.0, multiplicity=[1, 1], DW=1.0,
+                          F2=[10.0 * np.exp(-0.005 * q[0] ** 2), 20.0 * np.exp(-0.005 * q[1] ** 2)])
+    assert np.allclose(L[:, _slots.PDL_SSV2], s.sigma_scat_v2, rtol=1e-6)
+    # d log(sigma)/dd by central differences of the same formula
+    def logsig(dd, U=0.005):
+        qq = 2 * np.pi / dd
+        return np.log(dd * np.exp(-U * qq * qq))
+    num = (logsig(np.array(d) + 1e-6) - logsig(np.array(d) - 1e-6)) / 2e-6
+    assert np.allclose(L[:, _slots.PDL_DLS], num, rtol=1e-5)
+    assert np.allclose(L[:, _slots.PDL_Q0], 2 * np.pi / np.array([2.1, 2.9]), rtol=1e-6)
+    assert np.allclose(L[:, _slots.PDL_D0], [2.1, 2.9]) and np.allclose(L[:, _slots.PDL_D], d)
+    with pytest.raises(ValueError):
+        ramp_amd.SPowderDiff(d_spacing=[1.0, 2.0], F2=[1.0])

@@ -268,3 +268,63 @@ def test_incoherent_focusing_preserves_weight_into_the_cone(tmp_path, oracle):
     # isotropic: ~8e-4 of the histories reach the 0.01 sr plane (4 sigma)
     rel = 4.0 / math.sqrt(8e6 * 8e-4)
     assert wi > 0 and abs(wf / wi - 1.0) < rel
+
+
+def _pd_build(N):
+    import sys
+    import os as _os
+    sys.path.insert(0, _os.path.join(_os.path.dirname(_os.path.dirname(_os.path.abspath(__file__))),
+                                     "ramp_amd", "instruments"))
+    import powder_diff
+    from ramp_amd.diff import DiffInstrument
+    path = _os.path.join(_os.path.dirname(_os.path.dirname(_os.path.abspath(__file__))), "ramp_amd",
+                         "instruments", "powder_diff.json")
+    d, F2 = powder_diff.lines()
+    v = powder_diff.variables(powder_diff.theta0(), F2)
+    di = DiffInstrument.__new__(DiffInstrument)
+    di.fn, di.variables, di.sample_name, di.chunk = path, v, "sample", N
+    base = ramp_amd.Instrument(path, **v)
+    _, samp = di._sample(base)
+    di.kw, di.n_lines, di.F2 = samp.kw, int(samp.num_lines), samp.model["F2"]
+    di.proposal = {k: samp.model[k] for k in ("d_spacing", "F2", "U", "sigma_inc", "sigma_abs")}
+    di._instrument = lambda: ramp_amd.Instrument(path, hist_mode="deterministic", chunk=N, **v)
+    return di, powder_diff.theta0()
+
+
+def test_oracle_adjoint_is_the_derivative_of_the_forward(oracle):
+    # the FP64 reference gradient equals same-seed central differences of
+    # the oracle's own forward histogram (weights are smooth in theta)
+    N = 200000
+    di, th0 = _pd_build(N)
+    rng = np.random.default_rng(1)
+    dl = rng.normal(size=2048 * 256).astype(np.float32)
+
+    def L(th):
+        return float(np.dot(oracle.run(di.build(th), N, [3])["hist"][0][0], dl.astype(np.float64)))
+
+    g = oracle.adjoint(di.build(th0), N, [3], 0, dl, 44)
+    for i in (0, 1, 2, 3, 4, 9):
+        h = th0[i] * 1e-3
+        e = np.zeros(44)
+        e[i] = h
+        fd = (L(th0 + e) - L(th0 - e)) / (2 * h)
+        assert abs(g[i] - fd) <= 0.02 * abs(fd) + 1e-9, (i, g[i], fd)
+
+
+def test_powderdiff_at_the_proposal_is_forced_scattering(oracle, tmp_path):
+    # with model == proposal the weight factor is the forced-scattering one:
+    # an incoherent-only sample (no lines active: d below lambda/2) must
+    # reproduce SIncoherent's weight (1 - e^{-mu L}) mu_s/mu for every event
+    comps = {"src": _src_parallel(w=0.006, h=0.006, lmin=4.0, lmax=4.0000001),
+             "samp": {"position": [0, 0, 1.0],
+                      "geom_kernel": {"name": "GCylinderExt", "radius": 0.004, "height": 0.02},
+                      "scat_kernel": {"name": "SPowderDiff", "d_spacing": [1.0], "F2": [10.0],
+                                      "pack": 1.0, "vc": 40.0, "sigma_inc": 0.3, "sigma_abs": 0.2,
+                                      "lambda_ref": 1.8, "strain": 0.01}}}
+    a = oracle.run(_inst(tmp_path, comps, "a.json"), 20000, [4], keep_state=True)["state"][0]
+    comps["samp"]["scat_kernel"] = {"name": "SIncoherent", "sigma_s": 0.3, "sigma_a": 0.2, "lambda_ref": 1.8}
+    b = oracle.run(_inst(tmp_path, comps, "b.json"), 20000, [4], keep_state=True)["state"][0]
+    hit = b[:, 12] == 2.0
+    assert hit.sum() > 1000
+    # same draws for depth; one extra draw (channel) shifts the direction draws
+    assert np.allclose(a[hit, 9], b[hit, 9], rtol=2e-6)
