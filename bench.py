@@ -29,17 +29,23 @@ sys.path.insert(0, ROOT)
 METRIC = "neutron histories/s, 1 MI355X (LET-like 1e8); batched-scan & fwd+adjoint n/s"
 HBM_PEAK_GBS = 8000.0
 # gfx950 vector issue: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz = 39.3e12 32-bit
-# lane-ops/s, i.e. one wave64 VALU instruction per CU per clock
-VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
+# lane-ops/s at one wave64 VALU instruction per CU per clock; the SIMDs can
+# issue two VALU instructions per quad-cycle (the SQ_ACTIVE_INST_VALU2
+# counter; MI355X's 157.3 TFLOP/s FP32 vector peak = 2 x 39.3e12 FMA lanes),
+# so the VALU ceiling is 78.6e12 lane-ops/s.
+VALU_SINGLE_ISSUE_TOPS = 256 * 64 * 2.4e9 / 1e12
+VALU_PEAK_TOPS = 2 * VALU_SINGLE_ISSUE_TOPS
 
 # Per-history counts of the source-segment kernel (the dominant one), from
-# rocprofv3 --pmc runs of this script (files under profiles/):
+# rocprofv3 --pmc runs of this script (profiles/let_like_1e8_segment_kernels.md,
+# profiles/imaging_mesh50k_1e8_bvh.md):
 #   valu  = SQ_INSTS_VALU / histories (wave64 instructions, so x64 lane-ops)
+#   lanes = SQ_THREAD_CYCLES_VALU / (64 SQ_INSTS_VALU), the active-lane share
 #   fetch = FETCH_SIZE / histories, write = WRITE_SIZE / histories (bytes,
-#           after the plain-copy calibration of the counters)
+#           after the plain-copy calibration of the counters: x2.0, x1.0)
 PMC_PER_HISTORY = {
-    "let_like": None,
-    "imaging": None,
+    "let_like": {"valu": 80.07, "lanes": 0.628, "fetch": 0.0080, "write": 1.2633},
+    "imaging": {"valu": 454.59, "lanes": 0.176, "fetch": 52.84, "write": 63.93},
     "minimal": None,
 }
 
@@ -167,9 +173,12 @@ def main():
             "achieved": None, "peak": VALU_PEAK_TOPS, "unit": "TOP/s", "frac": None}
     pmc = PMC_PER_HISTORY.get(a.workload)
     if pmc and seg_avg_ms > 0:
+        # lane-op slots issued per second by the launch (measured instruction
+        # count per history x 64 x histories / measured launch time)
         ach = pmc["valu"] * 64 * N / (seg_avg_ms * 1e-3) / 1e12
         roof.update({"achieved": ach, "frac": ach / VALU_PEAK_TOPS,
-                     "valu_insts_per_history": pmc["valu"],
+                     "single_issue_peak": VALU_SINGLE_ISSUE_TOPS,
+                     "valu_insts_per_history": pmc["valu"], "active_lane_share": pmc["lanes"],
                      "traffic": (pmc["fetch"] + pmc["write"]) * N})
     out = {
         "metric": METRIC,
