@@ -45,12 +45,26 @@ RHD rf2 v2_norm(rf2 a)
  * rotate applies R, derotate applies R^T (geom.h:55-87). */
 typedef struct { float m[9]; } rmat3;
 
+/* sin and cos of an angle; rm_sin(+-0) is +0 and rm_cos(+-0) is 1 exactly
+ * (tested), so a zero angle takes no transcendental and gives the same bits */
+RHD void rm_sincos_z(float a, float *s, float *c)
+{
+    if (a == 0.0f) {
+        *s = 0.0f;
+        *c = 1.0f;
+    } else {
+        *s = rm_sin(a);
+        *c = rm_cos(a);
+    }
+}
+
 RHD rmat3 rmat3_euler(float xa, float ya, float za)
 {
     rmat3 R;
-    float cx = rm_cos(xa), sx = rm_sin(xa);
-    float cy = rm_cos(ya), sy = rm_sin(ya);
-    float cz = rm_cos(za), sz = rm_sin(za);
+    float cx, sx, cy, sy, cz, sz;
+    rm_sincos_z(xa, &sx, &cx);
+    rm_sincos_z(ya, &sy, &cy);
+    rm_sincos_z(za, &sz, &cz);
     R.m[0] = cy * cz;
     R.m[1] = -cx * sz + sx * sy * cz;
     R.m[2] = sx * sz + cx * sy * cz;

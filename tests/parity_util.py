@@ -43,7 +43,64 @@ def instrument_cases():
         "tas": (os.path.join(ROOT, "tests", "instruments", "tas.json"), {"Mono_angle": ang}),
         "powder_diff": (os.path.join(ROOT, "ramp_amd", "instruments", "powder_diff.json"),
                         _powder_diff_vars()),
+        "isis_isotropic": (os.path.join(ROOT, "tests", "instruments", "isis_isotropic.json"),
+                           {"spec": repr(table_files()["spectrum"]), "sqw": repr(table_files()["sqw"])}),
     }
+
+
+def isis_spectrum_file(path, n_e=40, n_t=60):
+    """A synthetic moderator spectrum in the McStas/MCNP tally layout the
+    MISIS loader reads (isis_mod.py:64-166): a TimeOffset comment, then per
+    energy bin (MeV) a time column (units of 1e-8 s) of
+    'value flux (error)' lines and a total line."""
+    e_edges = np.logspace(-9, -7, n_e + 1)           # 1 .. 100 meV
+    t = np.logspace(0, 5, n_t)                       # 1e-8 .. 1e-3 s
+    lines = ["1tally 5 nps 1000000", "c synthetic moderator spectrum", "c TimeOffset 100"]
+    for k in range(n_e):
+        e_mid = 0.5 * (e_edges[k] + e_edges[k + 1]) * 1e9
+        lines += [" surface 9999", "segment:  0", "user bin total",
+                  f"energy bin: {e_edges[k]:.6e} to {e_edges[k + 1]:.6e}", "cosine:   1", " time "]
+        tot = 0.0
+        for ti in t:
+            # Maxwellian-like in energy, Ikeda-Carpenter-like in time
+            f = 1e-9 * np.sqrt(e_mid) * np.exp(-e_mid / 25.0) * (ti / 100.0) * np.exp(-ti / (100.0 + 20 * e_mid))
+            tot += f
+            lines.append(f"{ti:.6e} {f:.6e} ({0.01 * f:.6e}) ")
+        lines.append(f"  total      {tot:.6e} ({0.01 * tot:.6e})")
+    lines.append("1end")
+    with open(path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+
+
+def sqw_file(path, rho=0.07, sigma_coh=2.0, sigma_abs=0.3):
+    """A synthetic S(q, w) table in the layout of SIsotropic's loader
+    (isotropic.py:171-208): '# key value' header lines, a q line, a w line,
+    then one row of S per q.  The loader consumes the first row when it
+    stops scanning and drops the last q, so the table it keeps is rows
+    1..nq-1 against q_0..q_{nq-2}."""
+    q = np.linspace(0.2, 6.0, 30)
+    w = np.linspace(-2.0, 2.0, 41)
+    S = np.exp(-0.5 * (w[None, :] / (0.2 + 0.05 * q[:, None])) ** 2) * np.exp(-0.05 * q[:, None] ** 2)
+    with open(path, "w") as fh:
+        fh.write(f"# V_rho {rho}\n# sigma_coh {sigma_coh}\n# sigma_inc 0.5\n# sigma_abs {sigma_abs}\n")
+        fh.write(" ".join(f"{x:.6f}" for x in q) + "\n")
+        fh.write(" ".join(f"{x:.6f}" for x in w) + "\n")
+        for row in S:
+            fh.write(" ".join(f"{x:.6e}" for x in row) + "\n")
+
+
+def table_files():
+    """Synthetic spectrum and S(q, w) files, written once per process."""
+    global _TAB_DIR
+    import tempfile
+    if _TAB_DIR is None:
+        _TAB_DIR = tempfile.mkdtemp(prefix="ramp_tables_")
+        isis_spectrum_file(os.path.join(_TAB_DIR, "spectrum.mcstas"))
+        sqw_file(os.path.join(_TAB_DIR, "inc.sqw"))
+    return {"spectrum": os.path.join(_TAB_DIR, "spectrum.mcstas"), "sqw": os.path.join(_TAB_DIR, "inc.sqw")}
+
+
+_TAB_DIR = None
 
 
 def _powder_diff_vars():

@@ -131,3 +131,17 @@ def test_device_bit_identical_to_host(lib, gctx, name):
     d = ev(lib, name, x, y if two else None, on_device=1, ctx=gctx)
     same = (h.view(np.uint32) == d.view(np.uint32)) | (np.isnan(h) & np.isnan(d))
     assert same.all(), (name, int((~same).sum()))
+
+
+def test_sin_cos_of_signed_zero_are_exact(lib):
+    # rmat3_euler skips the transcendentals for a zero angle on this basis
+    import ctypes as C
+    from ramp_amd import _capi
+    x = np.array([0.0, -0.0], np.float32)
+    y = np.zeros(2, np.float32)
+    for fn, want in ((_capi.RAMP_FN_SIN, 0), (_capi.RAMP_FN_COS, 0x3F800000)):
+        o = np.zeros(2, np.float32)
+        _capi.check(lib.ramp_math_eval(None, fn, x.ctypes.data_as(C.POINTER(C.c_float)),
+                                       y.ctypes.data_as(C.POINTER(C.c_float)),
+                                       o.ctypes.data_as(C.POINTER(C.c_float)), 2, 0))
+        assert (o.view(np.uint32) == want).all()
