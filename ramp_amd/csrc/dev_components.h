@@ -649,10 +649,30 @@ DEV void s_guide(NS &n, const DComp &c)
         float ch1 = -hhalf * l - n.z * hh, ch2 = n.y * l;
         float t1 = (l - n.z) / n.vz, t2;
         i = 0;
-        if (vdotn_v1 < 0.0f && (t2 = (cv1 - cv2) / vdotn_v1) < t1) { t1 = t2; i = 1; }
-        if (vdotn_v2 < 0.0f && (t2 = (cv1 + cv2) / vdotn_v2) < t1) { t1 = t2; i = 2; }
-        if (vdotn_h1 < 0.0f && (t2 = (ch1 - ch2) / vdotn_h1) < t1) { t1 = t2; i = 3; }
-        if (vdotn_h2 < 0.0f && (t2 = (ch1 + ch2) / vdotn_h2) < t1) { t1 = t2; i = 4; }
+        /* the reference tests walls 1..4 in order, each when vdotn < 0
+         * (guide.cl:67-86).  Of a pair of opposite walls at most one faces
+         * the neutron unless the guide widens, so a pair usually needs one
+         * division, not two; the order of the tests is unchanged. */
+        if (vdotn_v1 < 0.0f && vdotn_v2 < 0.0f) {
+            if ((t2 = (cv1 - cv2) / vdotn_v1) < t1) { t1 = t2; i = 1; }
+            if ((t2 = (cv1 + cv2) / vdotn_v2) < t1) { t1 = t2; i = 2; }
+        } else if (vdotn_v1 < 0.0f || vdotn_v2 < 0.0f) {
+            const bool w1 = vdotn_v1 < 0.0f;
+            if ((t2 = (w1 ? cv1 - cv2 : cv1 + cv2) / (w1 ? vdotn_v1 : vdotn_v2)) < t1) {
+                t1 = t2;
+                i = w1 ? 1 : 2;
+            }
+        }
+        if (vdotn_h1 < 0.0f && vdotn_h2 < 0.0f) {
+            if ((t2 = (ch1 - ch2) / vdotn_h1) < t1) { t1 = t2; i = 3; }
+            if ((t2 = (ch1 + ch2) / vdotn_h2) < t1) { t1 = t2; i = 4; }
+        } else if (vdotn_h1 < 0.0f || vdotn_h2 < 0.0f) {
+            const bool h1 = vdotn_h1 < 0.0f;
+            if ((t2 = (h1 ? ch1 - ch2 : ch1 + ch2) / (h1 ? vdotn_h1 : vdotn_h2)) < t1) {
+                t1 = t2;
+                i = h1 ? 3 : 4;
+            }
+        }
         if (i == 0) break;
         rf3 vel = nvel(n);
         n.x = n.x + t1 * vel.x;

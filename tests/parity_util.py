@@ -139,9 +139,11 @@ def compare_states(g, o):
     bits = (g16.view(np.uint32) != o16.view(np.uint32)) & ~(np.isnan(g16) & np.isnan(o16))
     out["bitdiff_values"] = int(bits.sum())
     out["bitdiff_rows"] = int(bits.any(axis=1).sum())
-    scale = np.maximum(np.nanmax(np.abs(o16), axis=0), 1e-30)
-    close = np.isclose(g16, o16, rtol=1e-5, atol=0) | (np.abs(g16 - o16) <= 1e-6 * scale) | \
-        (np.isnan(g16) & np.isnan(o16))
+    with np.errstate(invalid="ignore", over="ignore"):
+        scale = np.maximum(np.nanmax(np.abs(o16), axis=0), 1e-30)
+        # equal infinities and NaN pairs count as equal
+        close = np.isclose(g16, o16, rtol=1e-5, atol=0) | (np.abs(g16 - o16) <= 1e-6 * scale) | \
+            (np.isnan(g16) & np.isnan(o16)) | (g16 == o16)
     out["beyond_1e-5"] = int((~close).sum())
     out["beyond_rows"] = int((~close).any(axis=1).sum())
     return out
