@@ -206,6 +206,30 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def cpu_baseline_settings(path, settings, counts, target_s):
+    """Oracle on host cores over a bounded sample of the scan: the first
+    settings in order, each at its own count scaled so the sample takes
+    about target_s; the rate is histories/s over the sample."""
+    import ramp_amd
+    from oracle import pyoracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    n0 = 200_000
+    t = time.perf_counter()
+    pyoracle.run(ramp_amd.Instrument(path, **settings[0]), n0, [1], threads=threads)
+    rate0 = n0 / (time.perf_counter() - t)
+    budget = rate0 * target_s
+    done, used, k = 0, 0.0, 0
+    t0 = time.perf_counter()
+    while k < len(settings) and done < budget:
+        n = int(min(int(counts[k]), max(budget - done, 1e4)))
+        pyoracle.run(ramp_amd.Instrument(path, **settings[k]), n, [k + 1], threads=threads)
+        done += n
+        k += 1
+    used = time.perf_counter() - t0
+    return {"value": done / used, "unit": "histories/s", "cores": threads, "kind": "port",
+            "sample": f"{done} histories over the first {k} scan settings, {used:.1f} s"}
+
+
 def bench_tas_scan(a, wl, sync):
     """256-point triple-axis scan: one batched launch per step (graph
     replay) after the autotuner has picked the launch shape, against the
@@ -287,7 +311,35 @@ def bench_tas_scan(a, wl, sync):
         "roofline": None,
     }
     b.close()
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_settings(path, settings, counts, a.cpu_seconds)
     print(json.dumps(out), flush=True)
+
+
+def cpu_baseline_adjoint(di, t, t2, eps, target_s):
+    """Oracle forward + FP64 oracle adjoint of the same loss on host cores,
+    on a bounded number of histories; histories/s of the pair."""
+    import numpy as np
+    from oracle import pyoracle
+    from ramp_amd.diff import weighted_mse
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    inst = di.build(di.theta0)
+
+    def pair(n):
+        h = pyoracle.run(inst, n, [1], threads=threads)["hist"][di.det_slot][0]
+        _, dl = weighted_mse(h, t, t2, eps)
+        pyoracle.adjoint(inst, n, [1], di.det_slot, dl, di.n_params, threads=threads)
+
+    n0 = 200_000
+    t0 = time.perf_counter()
+    pair(n0)
+    r0 = n0 / (time.perf_counter() - t0)
+    n = int(min(max(r0 * target_s, 1e5), 3e7))
+    t0 = time.perf_counter()
+    pair(n)
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "histories/s", "cores": threads, "kind": "port",
+            "sample": f"{n} histories, forward + FP64 adjoint of the same instrument, {dt:.1f} s"}
 
 
 def bench_powder_adjoint(a, wl, sync):
@@ -356,6 +408,8 @@ def bench_powder_adjoint(a, wl, sync):
                                "adjoint_speedup": (2 * di.n_params * dt_fwd / K) / (dt / K)},
         "roofline": None,
     }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_adjoint(di, t, t2, eps, a.cpu_seconds)
     di.close()
     print(json.dumps(out), flush=True)
 
