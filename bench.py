@@ -58,6 +58,8 @@ WORKLOADS = {
     "tas_scan": dict(file="tas_scan.json", histories=None, name="tas_scan_256", bins=256 * 128),
     "powder_adjoint": dict(file="powder_diff.json", histories=3e7, name="powder_fwd_adjoint_3e7",
                            bins=2048 * 256),
+    "mcpl_out": dict(file="mcpl_out.json", histories=2e7, name="mcpl_out_guide_dump_2e7", bins=1000),
+    "mcpl_in": dict(file="mcpl_in.json", histories=2e7, name="mcpl_in_2e7_particles", bins=180 * 560),
 }
 
 
@@ -137,6 +139,8 @@ def main():
         return bench_tas_scan(a, wl, sync)
     if a.workload == "powder_adjoint":
         return bench_powder_adjoint(a, wl, sync)
+    if a.workload in ("mcpl_out", "mcpl_in"):
+        return bench_mcpl(a, wl, sync)
     tmpdir = tempfile.mkdtemp(prefix="ramp_bench_")
     kw = instrument_args(a.workload, tmpdir)
     path = os.path.join(ROOT, "ramp_amd", "instruments", wl["file"])
@@ -412,6 +416,123 @@ def bench_powder_adjoint(a, wl, sync):
         out["cpu_baseline"] = cpu_baseline_adjoint(di, t, t2, eps, a.cpu_seconds)
     di.close()
     print(json.dumps(out), flush=True)
+
+
+
+def bench_mcpl(a, wl, sync):
+    """MCPL particle I/O.  mcpl_out: the layout of the reference's
+    examples/MCPL_test.json (flat source, two straight guides, SMCPLOut at
+    the guide exit, energy monitor), one chunk per step; the records of the
+    ~97 % of neutrons alive at the dump are packed in the segment kernel and
+    compacted on the device (ramp_mcpl_read).  mcpl_in: SMCPLIn replaces
+    every history with a particle of a synthetic N-particle file (MCPL's
+    own packing, written before the timed region), then a banana detector.
+    value = histories / device time of the step (run + compaction); the
+    wall rate with the records' copy to the host is reported beside it."""
+    import tempfile
+    import numpy as np
+    import ramp_amd
+    from ramp_amd.mcpl import write_mcpl_full
+    N = int(a.histories if a.histories else wl["histories"])
+    path = os.path.join(ROOT, "ramp_amd", "instruments", wl["file"])
+    tmpdir = tempfile.mkdtemp(prefix="ramp_bench_")
+    if a.workload == "mcpl_out":
+        kw = {"mcpl_file": repr(""), "standard": 0}
+    else:
+        rng = np.random.default_rng(7)
+        rec = np.zeros((N, 11), np.float32)
+        u = rng.normal(size=(N, 3)) * [0.02, 0.02, 1.0]
+        u /= np.linalg.norm(u, axis=1)[:, None]
+        rec[:, 3:5] = rng.uniform(-2, 2, size=(N, 2))
+        rec[:, 6:8] = u[:, :2]
+        rec[:, 8] = rng.uniform(1.5e-9, 9e-9, size=N) * np.sign(u[:, 2])
+        rec[:, 9] = rng.uniform(20, 40, size=N)
+        rec[:, 10] = 1.0 / N
+        fn = os.path.join(tmpdir, "particles.mcpl")
+        write_mcpl_full(fn, rec)
+        del rec, u
+        kw = {"mcpl_file": repr(fn)}
+    inst = ramp_amd.Instrument(path, hist_mode=a.hist_mode, chunk=N, **kw)
+    seed = 20261203
+    for w in range(a.warmup):
+        inst.execute(N, seeds=[(seed + w) % 4097])
+    sync()
+    dev_ms, run_ms, sel_ms, wr_ms, recs = 0.0, 0.0, 0.0, 0.0, 0
+    t0 = time.perf_counter()
+    for k in range(a.steps):
+        inst.execute(N, seeds=[(seed + 100 + k) % 4097])
+        st = inst.stats()
+        run_ms += st["ms_total"]
+        if a.workload == "mcpl_out":
+            sel_ms += st["ms_mcpl_compact"]
+            wr_ms += st["ms_mcpl_write"]
+            recs += inst._mcpl_out[0].scat.records.shape[0]
+    sync()
+    dt = time.perf_counter() - t0
+    K = max(a.steps, 1)
+    dev_ms = (run_ms + sel_ms) / K
+    out = {
+        "metric": METRIC, "value": N / (dev_ms * 1e-3), "unit": "histories/s", "n_gpus": 1,
+        "steps": K, "warmup": a.warmup, "ms_per_step": dev_ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic: histories generated on the device" if a.workload == "mcpl_out"
+                else "synthetic MCPL file of N particles, resident in HBM",
+        "config": {"workload": wl["name"], "histories_per_step": N, "chunk": N,
+                   "instrument": "ramp_amd/instruments/" + wl["file"],
+                   "run_ms": run_ms / K, "wall_ms_per_step_with_host_copy": dt / K * 1e3,
+                   "wall_histories_per_s": N * K / dt},
+    }
+    if a.workload == "mcpl_out":
+        R = recs / K
+        w_ms = wr_ms / K
+        # k_sel_write: reads every tag (4 B) and each record (44 B), writes each record
+        alg = 4.0 * N + 88.0 * R
+        ach = alg / (w_ms * 1e-3) / 1e9 if w_ms > 0 else 0.0
+        out["config"].update({"records_per_step": R, "compaction_ms": sel_ms / K,
+                              "record_bytes_per_step": 44 * R})
+        out["roofline"] = {"bound": "hbm", "kernel": "k_sel_write (record compaction)",
+                           "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": ach / HBM_PEAK_GBS, "traffic": None, "launch_ms": w_ms,
+                           "algorithmic_bytes_per_launch": alg}
+    else:
+        # the source segment reads each particle's 44-byte record
+        seg_ms = inst.stats()["ms_source_segment"]
+        alg = 44.0 * min(N, inst._comp_list[0].scat.n_particles)
+        ach = alg / (seg_ms * 1e-3) / 1e9 if seg_ms > 0 else 0.0
+        out["roofline"] = {"bound": "hbm", "kernel": "k_segment<true,true> (SMCPLIn unpack + detector)",
+                           "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": ach / HBM_PEAK_GBS, "traffic": None, "launch_ms": seg_ms,
+                           "algorithmic_bytes_per_launch": alg}
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_mcpl(path, kw, a.cpu_seconds, wl["name"], a.workload)
+        out["cpu_baseline"]["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    inst.close()
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline_mcpl(path, kw, target_s, label, workload):
+    """Oracle on host cores: the same instrument on a bounded sample; for
+    mcpl_out the oracle's records then go through the same numpy
+    compaction the reference does on the host (alive filter in order)."""
+    import numpy as np
+    import ramp_amd
+    from oracle import pyoracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    inst = ramp_amd.Instrument(path, **kw)
+    n = 200_000
+    t = time.perf_counter()
+    pyoracle.run(inst, n, [1], threads=threads)
+    r0 = n / (time.perf_counter() - t)
+    n = int(min(max(r0 * target_s, 1e4), 5e7))
+    if workload == "mcpl_in":
+        n = min(n, inst._comp_list[0].scat.n_particles)
+    t = time.perf_counter()
+    r = pyoracle.run(inst, n, [2], threads=threads)
+    if workload == "mcpl_out":
+        np.ascontiguousarray(r["mcpl"][0])
+    dt = time.perf_counter() - t
+    return {"value": n / dt, "unit": "histories/s", "cores": threads, "kind": "port",
+            "sample": f"{n} histories of the same {label} instrument, one chunk, {dt:.1f} s"}
 
 
 if __name__ == "__main__":

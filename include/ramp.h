@@ -70,6 +70,8 @@ extern "C" {
 #define RAMP_S_ABSORBER 13   /* SAbsorber      Beer-Lambert absorber (no reference counterpart) */
 #define RAMP_S_DETPIX 14     /* SDetectorTubes tube x pixel x TOF bank (no reference counterpart) */
 #define RAMP_S_POWDERDIFF 15 /* SPowderDiff    differentiable powder + incoherent sample (no reference counterpart) */
+#define RAMP_S_MCPL_IN 16    /* SMCPLIn        scat/mcpl_in.py:9-76 (particles from an MCPL file) */
+#define RAMP_S_MCPL_OUT 17   /* SMCPLOut       scat/mcpl_out.py:7-117 (live particles to an MCPL file) */
 
 /* ---- plan descriptors --------------------------------------------------- */
 #define RAMP_NP 48  /* float parameters per component part */
@@ -153,6 +155,8 @@ typedef struct {
     uint64_t state_out_first;/* neutrons the first (source) segment wrote to its queue */
     double ms_total;         /* device time of the last run (HIP events) */
     double ms_source_segment;/* device time of the first (source) segment of the last run */
+    double ms_mcpl_compact;  /* last ramp_mcpl_read: device time of its three compaction kernels */
+    double ms_mcpl_write;    /* last ramp_mcpl_read: device time of the record-writing pass alone */
 } ramp_run_stats;
 
 typedef struct ramp_ctx ramp_ctx;
@@ -191,6 +195,20 @@ int ramp_hist_reset(ramp_plan *plan);
 int ramp_state_read(ramp_plan *plan, float *neutrons16, float *isect8, uint32_t *iidx, uint64_t n);
 
 int ramp_stats_read(const ramp_plan *plan, ramp_run_stats *out);
+
+/* ---- MCPL particle output -------------------------------------------------
+ * Records of SMCPLOut component `comp` (its index among the plan's
+ * SMCPLOut components, in plan order) from the last chunk of the last run:
+ * the neutrons alive at that component, in history order, each packed on
+ * the GPU into the 11-float record the reference writes (polarisation,
+ * position in cm, packed direction and kinetic energy, time in ms, weight;
+ * mcpl_out.py:55-73).  The records are compacted on the device; *count
+ * receives their number and min(count, cap) of them are copied to
+ * `records` (11 floats each; may be NULL with cap = 0 to query the count).
+ * Replaces the host loop of SMCPLOut.scatter_prg (mcpl_out.py:99-117),
+ * which copies the whole neutron buffer back and filters it in Python;
+ * the file header is written by the caller. */
+int ramp_mcpl_read(ramp_plan *plan, uint32_t comp, float *records, uint64_t cap, uint64_t *count);
 
 /* ---- gradients ----------------------------------------------------------
  * Gradient of a loss of one detector's histogram with respect to the model

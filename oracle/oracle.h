@@ -42,6 +42,12 @@ int oracle_adjoint(const ramp_plan_desc *desc, uint64_t n_hist, uint64_t chunk,
                    const uint32_t *seeds, uint32_t n_seeds, uint32_t det, const float *dL_dh,
                    double *grad, uint32_t n_grad, int n_threads);
 
+/* Where SMCPLOut components write their records: records[slot][cap][11]
+ * and tags[slot][cap] (chunk number + 1 of the chunk that wrote them); slot
+ * is the component's index among the plan's SMCPLOut components.  NULL
+ * records: SMCPLOut writes nothing. */
+void oracle_set_mcpl_dump(float *records, uint32_t *tags, uint64_t cap);
+
 const char *oracle_last_error(void);
 
 #ifdef __cplusplus

@@ -38,6 +38,8 @@ def load():
                                        C.POINTER(C.c_uint32), C.c_uint32, C.c_uint32,
                                        C.POINTER(C.c_float), C.POINTER(C.c_double), C.c_uint32,
                                        C.c_int]
+        lib.oracle_set_mcpl_dump.restype = None
+        lib.oracle_set_mcpl_dump.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_uint32), C.c_uint64]
         lib.oracle_last_error.restype = C.c_char_p
         lib.oracle_threefry4x32.restype = None
         lib.oracle_threefry4x32.argtypes = [C.POINTER(C.c_uint32)] * 3
@@ -49,10 +51,19 @@ def run(inst, n_hist, seeds, chunk=0, keep_state=False, threads=0):
     """Run `inst` (a ramp_amd.Instrument) on the oracle.
 
     Returns dict: 'hist' list of (w, w2) float64 per detector slot, 'oob',
-    and with keep_state the final AoS state (n16, isect8, iidx) of the
-    last chunk."""
+    with keep_state the final AoS state (n16, isect8, iidx) of the last
+    chunk, and 'mcpl': per SMCPLOut component the records (n, 11) float32
+    it wrote in the last chunk, in history order."""
     lib = load()
     desc = inst.plan_desc()
+    n_out = sum(1 for c in inst._comp_list if type(c.scat).__name__ == "SMCPLOut")
+    ch0 = int(chunk) if chunk else int(n_hist)
+    dump = tags = None
+    if n_out:
+        dump = np.zeros((n_out, ch0, 11), np.float32)
+        tags = np.zeros((n_out, ch0), np.uint32)
+        lib.oracle_set_mcpl_dump(dump.ctypes.data_as(C.POINTER(C.c_float)),
+                                 tags.ctypes.data_as(C.POINTER(C.c_uint32)), ch0)
     dets = [int(desc.dets[i].n_bins) for i in range(desc.n_dets)]
     hw = [np.zeros(n, np.float64) for n in dets]
     hw2 = [np.zeros(n, np.float64) for n in dets]
@@ -77,11 +88,16 @@ def run(inst, n_hist, seeds, chunk=0, keep_state=False, threads=0):
                         i8.ctypes.data_as(C.POINTER(C.c_float)) if keep_state else None,
                         ii.ctypes.data_as(C.POINTER(C.c_uint32)) if keep_state else None,
                         int(threads), C.byref(oob))
+    if n_out:
+        lib.oracle_set_mcpl_dump(None, None, 0)
     if rc != 0:
         raise RuntimeError(f"oracle error {rc}: {lib.oracle_last_error().decode()}")
     out = {"hist": list(zip(hw, hw2)), "oob": oob.value}
     if keep_state:
         out["state"] = (n16, i8, ii)
+    if n_out:
+        n_chunks = int(np.ceil(n_hist / ch0)) if n_hist else 0
+        out["mcpl"] = [dump[s][tags[s] == n_chunks] for s in range(n_out)]
     return out
 
 

@@ -20,6 +20,9 @@
  *   incoherent/absorber    Beer-Lambert closed forms
  *   detectors              binning arithmetic, weight conservation
  *   isotropic, mono, ISIS  parity unpinned beyond moments / conservation
+ *   MCPL in/out            header and record layout of the reference's
+ *                          files; unpack(pack(u)) = u on the McStas file's
+ *                          records; the reference writer's packing quirks
  */
 #include <math.h>
 #include <stdio.h>
@@ -109,6 +112,8 @@ typedef struct {
     double **hw, **hw2;
     uint64_t oob;
     oadj *adj;
+    int seed;        /* source seed of the running chunk */
+    uint32_t ck;     /* chunk number of the running chunk */
 } octx;
 
 /* the history a thread is running (the adjoint terms live in it) */
@@ -1422,6 +1427,116 @@ static void o_scatter(octx *cx, on_t *n, const ramp_component_desc *c)
     }
 }
 
+/* ======================= MCPL particle I/O =============================== */
+
+/* records written by SMCPLOut: [slot][cap][11] with the chunk number + 1 */
+static float *g_dump_rec;
+static uint32_t *g_dump_tag;
+static uint64_t g_dump_cap;
+
+void oracle_set_mcpl_dump(float *records, uint32_t *tags, uint64_t cap)
+{
+    g_dump_rec = records;
+    g_dump_tag = tags;
+    g_dump_cap = cap;
+}
+
+/* SMCPLIn.scatter_prg and MCPL_to_RAMP (mcpl_in.py:9-33, 57-76): slot i
+ * takes particle i of the file, every slot beyond the particles is zeroed
+ * and dead; iidx and the intersection are not touched.  The `mcpl` package
+ * returns ekin, a unit direction and the rest as Python floats; the
+ * direction comes from MCPL format 3's adaptive projection: of the stored
+ * (u1, u2), one with |.| > 1 is 1/uz in place of the dropped x (u1) or y
+ * (u2) component, otherwise z is dropped; the dropped component is
+ * sqrt(1 - sum of the squares of the other two) with the sign bit of the
+ * stored energy.  The particle's counter is float32(randint(0, 2^30) + i);
+ * the random integer is the low 30 bits of Threefry word 0 at key i and
+ * counter 2^31 + the chunk seed. */
+static void o_mcpl_in(const octx *cx, on_t *n, const ramp_part_desc *s)
+{
+    const uint64_t i = n->tid;
+    for (int k = 0; k < 16; k++) n->s[k] = 0.0f;
+    if (i >= s->ni[MCI_N]) {
+        n->N_DIE = 1.0f;
+        return;
+    }
+    double r[MCR_N];
+    const float *t = tab(cx, s, MCT_REC);
+    for (int k = 0; k < MCR_N; k++)
+        r[k] = s->ni[MCI_DOUBLE] ? ((const double *)t)[MCR_N * i + k] : (double)t[MCR_N * i + k];
+    double ekin = fabs(r[8]);
+    double sgn = signbit(r[8]) ? -1.0 : 1.0;
+    double dir[3];
+    if (fabs(r[6]) > 1.0) {
+        dir[2] = 1.0 / r[6];
+        dir[1] = r[7];
+        dir[0] = sgn * sqrt(fmax(0.0, 1.0 - (dir[1] * dir[1] + dir[2] * dir[2])));
+    } else if (fabs(r[7]) > 1.0) {
+        dir[2] = 1.0 / r[7];
+        dir[0] = r[6];
+        dir[1] = sgn * sqrt(fmax(0.0, 1.0 - (dir[0] * dir[0] + dir[2] * dir[2])));
+    } else {
+        dir[0] = r[6];
+        dir[1] = r[7];
+        dir[2] = sgn * sqrt(fmax(0.0, 1.0 - (dir[0] * dir[0] + dir[1] * dir[1])));
+    }
+    const double SE2V = 437.393377;
+    double KE = ekin * 1e9;
+    for (int k = 0; k < 3; k++) {
+        n->s[k] = (float)(r[3 + k] * 1e-2);
+        n->s[3 + k] = (float)(dir[k] * sqrt(KE) * SE2V);
+        n->s[6 + k] = (float)r[k];
+    }
+    n->N_P = (float)r[10];
+    n->N_TOF = (float)(r[9] * 1e-3);
+    uint32_t base = rr_word0(0x80000000u + (uint32_t)cx->seed, (uint32_t)i) & 0x3fffffffu;
+    n->N_SB = (float)(base + (uint32_t)i);
+}
+
+/* SMCPLOut.scatter_prg with write_particle, pack_vector and KE_from_v
+ * (mcpl_out.py:36-73, 99-107): every slot whose dead flag is not 1 is
+ * written, whatever its iidx; nothing in the state changes.  numpy of the
+ * time evaluates float32-scalar arithmetic with Python floats in double;
+ * the record rounds each value to float32.  `standard` (no reference
+ * counterpart) packs the unit direction the way MCPL format 3 does. */
+static void o_mcpl_out(const octx *cx, on_t *n, const ramp_part_desc *s)
+{
+    if (n->N_DIE == 1.0f || !g_dump_rec) return;
+    const uint64_t slot = (uint64_t)s->ni[MCO_SLOT] * g_dump_cap + n->tid;
+    float *o = g_dump_rec + (size_t)MCR_N * slot;
+    float x = n->N_VX, y = n->N_VY, z = n->N_VZ;
+    double KE = 5.22703725e-15 * (((double)x * x + (double)y * y) + (double)z * z);
+    double ux = 0.0, uy = 0.0, si = 1.0;
+    if (s->ni[MCO_STANDARD]) {
+        double v = sqrt(((double)x * x + (double)y * y) + (double)z * z);
+        double dx = x / v, dy = y / v, dz = z / v;
+        double ax = fabs(dx), ay = fabs(dy), az = fabs(dz), big;
+        if (az >= ax && az >= ay) { ux = dx; uy = dy; big = dz; }
+        else if (ax >= ay) { ux = 1.0 / dz; uy = dy; big = dx; }
+        else { ux = dx; uy = 1.0 / dz; big = dy; }
+        si = signbit(big) ? -1.0 : 1.0;
+    } else if (x > y && x > z) {
+        ux = 1.0 / (double)z;
+        si = (x > 0) - (x < 0);
+    } else if (y > x && y > z) {
+        ux = x;
+        uy = 1.0 / (double)z;
+        si = (y > 0) - (y < 0);
+    } else if (z > x && z > y) {
+        si = (z > 0) - (z < 0);
+    }
+    o[0] = n->s[6]; o[1] = n->s[7]; o[2] = n->s[8];
+    o[3] = (float)((double)n->N_X * 1e2);
+    o[4] = (float)((double)n->N_Y * 1e2);
+    o[5] = (float)((double)n->N_Z * 1e2);
+    o[6] = (float)ux;
+    o[7] = (float)uy;
+    o[8] = (float)(si * KE);
+    o[9] = (float)((double)n->N_TOF * 1e3);
+    o[10] = n->N_P;
+    g_dump_tag[slot] = cx->ck + 1;
+}
+
 /* ======================= host loop ====================================== */
 
 /* ExecutionBlock.execute: mcramp.py:118-220 */
@@ -1434,7 +1549,9 @@ static void o_block(octx *cx, on_t *n, const ramp_block_desc *b)
             o_transform(n, c);
             o_geom(cx, n, c);
             o_terminate(n, c->restore);
-            o_scatter(cx, n, c);
+            if (c->scat.type == RAMP_S_MCPL_IN) o_mcpl_in(cx, n, &c->scat);
+            else if (c->scat.type == RAMP_S_MCPL_OUT) o_mcpl_out(cx, n, &c->scat);
+            else o_scatter(cx, n, c);
             o_untransform(n, c);
         }
     } else {
@@ -1513,6 +1630,8 @@ int oracle_run(const ramp_plan_desc *desc, uint64_t n_hist, uint64_t chunk, cons
         remaining -= torun;
         int seed = (int)seeds[ck];
         int64_t nt = (int64_t)torun;
+        cx.seed = seed;
+        cx.ck = (uint32_t)ck;
 #pragma omp parallel for schedule(dynamic, 1024)
         for (int64_t i = 0; i < nt; i++) {
             on_t *n = &slots[i];

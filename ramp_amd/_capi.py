@@ -111,6 +111,8 @@ class RunStats(C.Structure):
         ("state_out_first", C.c_uint64),
         ("ms_total", C.c_double),
         ("ms_source_segment", C.c_double),
+        ("ms_mcpl_compact", C.c_double),
+        ("ms_mcpl_write", C.c_double),
     ]
 
 
@@ -139,6 +141,8 @@ _SIGS = {
     "ramp_state_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float),
                                   C.POINTER(C.c_uint32), C.c_uint64]),
     "ramp_stats_read": (C.c_int, [C.c_void_p, C.POINTER(RunStats)]),
+    "ramp_mcpl_read": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_float), C.c_uint64,
+                                 C.POINTER(C.c_uint64)]),
     "ramp_math_eval": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float),
                                  C.POINTER(C.c_float), C.c_uint64, C.c_int]),
     "ramp_rng_eval": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_float),

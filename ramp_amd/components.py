@@ -1038,16 +1038,72 @@ class SRescal(_OutOfScope):
     reason = "host-side resolution event list, not part of the MI355X neutron path"
 
 
-class SMCPLIn(_OutOfScope):
-    reason = "MCPL particle-file input is not implemented"
+class SMCPLIn(_Scat):
+    """scat/mcpl_in.py:35-76: the particles of an MCPL file (format 3,
+    optionally gzipped) replace the chunk's neutrons - slot i takes
+    particle i, the slots beyond the file's particles are dead - whatever
+    their state, iidx or intersection, which it leaves as they are.  The
+    file is parsed here (ramp_amd.mcpl); each GPU lane unpacks its own
+    record in FP64 as MCPL_to_RAMP does (:9-33).  The particle's RNG counter
+    is float32(r + i) with r a 30-bit random integer (:10), here the
+    Threefry stream of key i at counter 2^31 + chunk seed.  Extension: with
+    more particles in the file than histories in the chunk, the first
+    `chunk` particles are used (the reference's loop, :66-69, overruns its
+    buffer)."""
+    type_code = capi.RAMP_S_MCPL_IN
+
+    def __init__(self, filename="", idx=0, ctx=0, **kwargs):
+        super().__init__()
+        from .mcpl import read_mcpl
+        self.idx = np.uint32(idx)
+        self.filename = filename
+        self.header, rec = read_mcpl(filename)
+        n = rec.shape[0]
+        if n > 0xffffffff:
+            raise ValueError(f"{filename}: {n} particles, at most 2^32-1 are supported")
+        self.n_particles = n
+        self.max_weight = float(np.max(np.abs(rec[:, 10]))) if n else 0.0
+        self.ni[S.MCI_N] = n
+        self.ni[S.MCI_DOUBLE] = 0 if rec.dtype == np.float32 else 1
+        # doubles travel in the float32 table pool as pairs of words
+        self.tables[S.MCT_REC] = np.ascontiguousarray(rec).view(np.float32).ravel()
 
 
-class SMCPLOut(_OutOfScope):
-    reason = "MCPL particle-file output is not implemented"
+class SMCPLOut(_Scat):
+    """scat/mcpl_out.py:75-117: the neutrons alive at this component are
+    written to an MCPL file - the reference's header and 44-byte records
+    (polarisation, position in cm, pack_vector's direction with the kinetic
+    energy in MeV, time in ms, weight) in history order - leaving the state,
+    iidx and intersection unchanged.  As in the reference the file holds
+    the last chunk of the last execute().  The records are packed on the GPU
+    and compacted there (ramp_mcpl_read).  Extension: standard=True packs the
+    unit direction as MCPL itself does (the reference's pack_vector keeps the
+    velocity's components, not the unit vector's, and zeros where MCPL keeps
+    one: mcpl_out.py:36-50)."""
+    type_code = capi.RAMP_S_MCPL_OUT
+
+    def __init__(self, filename="", standard=False, idx=0, ctx=0, **kwargs):
+        super().__init__()
+        self.idx = np.uint32(idx)
+        self.filename = filename
+        self.ni[S.MCO_STANDARD] = 1 if standard else 0
+        self.records = np.zeros((0, 11), np.float32)
+
+    def _collect(self, lib, plan, slot):
+        import ctypes as C
+        from .mcpl import write_mcpl
+        cnt = C.c_uint64(0)
+        capi.check(lib.ramp_mcpl_read(plan, slot, None, 0, C.byref(cnt)))
+        rec = np.zeros((cnt.value, 11), np.float32)
+        capi.check(lib.ramp_mcpl_read(plan, slot, rec.ctypes.data_as(C.POINTER(C.c_float)),
+                                      cnt.value, C.byref(cnt)))
+        self.records = rec
+        if self.filename:
+            write_mcpl(self.filename, rec)
 
 
 SOURCES = ("MFlat", "MCont", "MISIS", "MLambda", "MParallel")
 GEOMS = ("GArm", "GPlane", "GBox", "GSphere", "GSphereInt", "GCylinderExt", "GBanana", "GMesh")
 SCATS = ("SArm", "SBeamstop", "SChopper", "SGuide", "SIsotropic", "SPowder1", "SPowderN",
          "SMonochromator", "SDetector1D", "SDetector2D", "SCounter", "SIncoherent", "SAbsorber",
-         "SDetectorTubes", "SPowderDiff")
+         "SDetectorTubes", "SPowderDiff", "SMCPLIn", "SMCPLOut")

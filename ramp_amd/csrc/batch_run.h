@@ -162,6 +162,8 @@ extern "C" int ramp_batch_create(ramp_ctx *ctx, ramp_plan *const *plans, uint32_
     for (uint32_t s = 0; s < n_settings; s++) {
         if (!plans[s]) return fail(RAMP_E_INVALID, "setting %u: null plan", s);
         if (plans[s]->ctx != ctx) return fail(RAMP_E_INVALID, "setting %u: plan of another context", s);
+        if (plans[s]->has_mcpl)
+            return fail(RAMP_E_INVALID, "setting %u: SMCPLIn/SMCPLOut plans run with ramp_run only", s);
         for (uint32_t r = 0; r < s; r++)
             if (plans[r] == plans[s]) return fail(RAMP_E_INVALID, "settings %u and %u share a plan", r, s);
         feat |= plans[s]->feat;
@@ -236,6 +238,7 @@ extern "C" int ramp_batch_run(ramp_batch *b, const uint64_t *counts, const uint3
         if (rc) return rc;
         SegArgs &a = b->hsets[s];
         std::memset(&a, 0, sizeof a);
+        a.revive_k = -1;
         a.comps = (const DComp *)p->comps.p;
         a.steps = (const DStep *)p->steps.p;
         a.src = (const DPart *)p->src.p;

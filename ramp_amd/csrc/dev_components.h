@@ -762,6 +762,7 @@ DEV void s_absorber(NS &n, const DComp &c)
 }
 
 #include "dev_samples.h"
+#include "dev_mcpl.h"
 
 template <bool FULL, uint32_t F, class HC>
 DEV void scat_dispatch(NS &n, const DComp &c, HC &hc)

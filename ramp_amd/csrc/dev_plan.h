@@ -60,6 +60,7 @@ struct DQueue {
     float *x, *y, *z, *vx, *vy, *vz, *w, *t, *ctr;
     uint32_t *key, *status;
     float *px, *py, *pz, *sc, *sd, *se; /* full-state mode only */
+    float *is[8];                       /* full-state mode only: the intersection */
 };
 
 /* counters written by the kernels */

@@ -1,0 +1,136 @@
+/* mcpl_run.h - ramp_mcpl_read: the records of one SMCPLOut component,
+ * compacted on the device in history order.
+ *
+ * The reference copies the whole neutron buffer of the chunk to the host
+ * and filters it particle by particle in Python (mcpl_out.py:99-107).
+ * Here lane i of the last chunk wrote record i and tagged it with the
+ * chunk's tag; a three-kernel stream compaction keeps the tagged records in
+ * order: per-wave counts over contiguous ranges, one exclusive scan, and a
+ * second pass in which every wave writes its records at its offset (skipped
+ * when the caller only asks for the count).  Only the compacted records
+ * cross PCIe.  Included by ramp_gpu.hip.
+ */
+
+#define SEL_WAVES_MAX 16384u
+
+/* waves over the n tags of one slot; each takes a contiguous range */
+DEV void sel_range(uint64_t n, uint32_t G, uint32_t g, uint64_t &b, uint64_t &e)
+{
+    const uint64_t per = ((n + G - 1) / G + 63) & ~(uint64_t)63;
+    b = (uint64_t)g * per;
+    e = b + per < n ? b + per : n;
+    if (b > n) b = n;
+}
+
+__global__ __launch_bounds__(256) void k_sel_count(const uint32_t *tags, uint64_t n, uint32_t tag,
+                                                   uint32_t G, uint32_t *counts)
+{
+    const uint32_t g = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (g >= G) return;
+    uint64_t b, e;
+    sel_range(n, G, g, b, e);
+    uint32_t c = 0;
+    for (uint64_t i = b + __lane_id(); i < e; i += 64) c += tags[i] == tag ? 1u : 0u;
+    c = wave_sum(c);
+    if (__lane_id() == 0) counts[g] = c;
+}
+
+/* exclusive prefix of G counts in one block; prefix[G] is the total */
+__global__ __launch_bounds__(1024) void k_sel_scan(const uint32_t *counts, uint32_t G, uint64_t *prefix)
+{
+    __shared__ uint64_t part[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (G + 1023) / 1024;
+    const uint32_t b = t * per, e = b + per < G ? b + per : G;
+    uint64_t sum = 0;
+    for (uint32_t i = b; i < e; i++) sum += counts[i];
+    part[t] = sum;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        uint64_t v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint64_t run = part[t] - sum;
+    for (uint32_t i = b; i < e; i++) {
+        prefix[i] = run;
+        run += counts[i];
+    }
+    if (t == 1023) prefix[G] = part[1023];
+}
+
+__global__ __launch_bounds__(256) void k_sel_write(const uint32_t *tags, const float *recs, uint64_t n,
+                                                   uint32_t tag, uint32_t G, const uint64_t *prefix,
+                                                   float *out)
+{
+    const uint32_t g = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (g >= G) return;
+    uint64_t b, e;
+    sel_range(n, G, g, b, e);
+    uint64_t o = prefix[g];
+    const uint32_t lane = __lane_id();
+    for (uint64_t base = b; base < e; base += 64) {
+        const uint64_t i = base + lane;
+        const bool hit = i < e && tags[i] == tag;
+        const unsigned long long m = __ballot(hit);
+        if (hit) {
+            const uint64_t d = o + (uint64_t)__popcll(m & ((1ull << lane) - 1ull));
+            const float *r = recs + (size_t)MCR_N * i;
+            float *w = out + (size_t)MCR_N * d;
+            for (int k = 0; k < MCR_N; k++) w[k] = r[k];
+        }
+        o += (uint64_t)__popcll(m);
+    }
+}
+
+extern "C" int ramp_mcpl_read(ramp_plan *p, uint32_t comp, float *records, uint64_t cap, uint64_t *count)
+{
+    if (!p || !count) return fail(RAMP_E_INVALID, "null argument");
+    if (comp >= p->n_mcpl_out) return fail(RAMP_E_INVALID, "SMCPLOut %u of %u", comp, p->n_mcpl_out);
+    if (cap && !records) return fail(RAMP_E_INVALID, "null record buffer");
+    *count = 0;
+    if (!p->dump.p || p->dump_n == 0) return 0;
+    HIPCHK(hipSetDevice(p->ctx->device));
+    hipStream_t s = p->ctx->stream;
+    const uint64_t n = p->dump_n;
+    const uint32_t G = (uint32_t)std::min<uint64_t>(SEL_WAVES_MAX, std::max<uint64_t>(1, (n + 4095) / 4096));
+    int rc = dev_alloc(p->sel_counts, G * sizeof(uint32_t));
+    if (!rc) rc = dev_alloc(p->sel_prefix, (G + 1) * sizeof(uint64_t));
+    if (rc) return rc;
+    const uint32_t *tags = (const uint32_t *)p->dump_tag.p + (size_t)comp * p->dump_cap;
+    const float *recs = (const float *)p->dump.p + (size_t)comp * p->dump_cap * MCR_N;
+    const uint32_t tag = p->dump_chunk_tag;
+    const unsigned blocks = (G + 3) / 4;
+    ramp_ctx *ctx = p->ctx;
+    HIPCHK(hipEventRecord(ctx->ev[0], s));
+    hipLaunchKernelGGL(k_sel_count, dim3(blocks), dim3(256), 0, s, tags, n, tag, G,
+                       (uint32_t *)p->sel_counts.p);
+    hipLaunchKernelGGL(k_sel_scan, dim3(1), dim3(1024), 0, s, (const uint32_t *)p->sel_counts.p, G,
+                       (uint64_t *)p->sel_prefix.p);
+    HIPCHK(hipEventRecord(ctx->ev[1], s));
+    HIPCHK(hipGetLastError());
+    uint64_t total = 0;
+    HIPCHK(hipMemcpyAsync(&total, (const uint64_t *)p->sel_prefix.p + G, sizeof total, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    *count = total;
+    if (total && cap) {
+        rc = dev_alloc(p->sel_out, total * MCR_N * sizeof(float));
+        if (rc) return rc;
+        HIPCHK(hipEventRecord(ctx->ev[2], s));
+        hipLaunchKernelGGL(k_sel_write, dim3(blocks), dim3(256), 0, s, tags, recs, n, tag, G,
+                           (const uint64_t *)p->sel_prefix.p, (float *)p->sel_out.p);
+        HIPCHK(hipEventRecord(ctx->ev[3], s));
+        HIPCHK(hipGetLastError());
+    }
+    const uint64_t ncopy = std::min(total, cap);
+    if (ncopy)
+        HIPCHK(hipMemcpyAsync(records, p->sel_out.p, ncopy * MCR_N * sizeof(float), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    float ms = 0, ms_w = 0;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+    if (total && cap) HIPCHK(hipEventElapsedTime(&ms_w, ctx->ev[2], ctx->ev[3]));
+    p->stats.ms_mcpl_compact = ms + ms_w;
+    p->stats.ms_mcpl_write = ms_w;
+    return 0;
+}

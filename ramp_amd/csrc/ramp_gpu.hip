@@ -86,10 +86,20 @@ struct SegArgs {
     uint32_t *slot_iidx;
     float *st16, *st8;
     uint32_t *stii;
+    /* MCPL steps: SMCPLOut records [slot][dump_cap][MCR_N] with their chunk
+     * tags; revive_k is the last SMCPLIn step (-1: none), which every lane
+     * must reach even when all of its neutrons are dead */
+    float *dump;
+    uint32_t *dump_tag;
+    uint64_t dump_cap;
+    uint32_t chunk_tag;
+    int revive_k;
 };
 
 /* status word of the queue: iidx in bits 0..23, bit 24: intersection is all
- * zeros (GBox miss with restore) instead of the sentinel */
+ * zeros (GBox miss with restore) instead of the sentinel.  The full-state
+ * mode stores the whole intersection instead (components that leave it as
+ * their geometry set it, such as the MCPL steps, need that). */
 #define ST_ZERO_ISECT (1u << 24)
 
 template <bool FULL>
@@ -104,7 +114,9 @@ DEV void queue_load(NS &n, const DQueue &q, uint64_t i)
     n.dead = false;
     n.pend = false;
     n.aux = 0u;
-    if (st & ST_ZERO_ISECT) {
+    if (FULL) {
+        for (int k = 0; k < 8; k++) n.is[k] = q.is[k][i];
+    } else if (st & ST_ZERO_ISECT) {
         for (int k = 0; k < 8; k++) n.is[k] = 0.0f;
     } else {
         isect_reset(n);
@@ -129,6 +141,7 @@ DEV void queue_store(const NS &n, const DQueue &q, uint64_t i)
     if (FULL) {
         q.px[i] = n.px; q.py[i] = n.py; q.pz[i] = n.pz;
         q.sc[i] = n.sc; q.sd[i] = n.sd; q.se[i] = n.se;
+        for (int k = 0; k < 8; k++) q.is[k][i] = n.is[k];
     }
 }
 
@@ -199,13 +212,22 @@ template <bool FULL, uint32_t F>
 DEV void run_range(NS &n, const SegArgs &a, HistCtx &hc, int k0, int k1, DeferQ &dq, bool defer)
 {
     for (int k = k0; k < k1; k++) {
-        if (!a.pilot && !__any(!n.dead)) break;
+        if (!a.pilot && k > a.revive_k && !__any(!n.dead)) break;
         const DStep st = a.steps[k];
         if (st.kind == 0) {
             const DComp &c = a.comps[st.first];
             tr_in<FULL>(n, c);
             geom_dispatch<F>(n, c);
             terminate(n, c.restore);
+            if (c.s.type == RAMP_S_MCPL_IN || c.s.type == RAMP_S_MCPL_OUT) {
+                /* plans holding these run in full-state mode only */
+                if constexpr (FULL) {
+                    if (c.s.type == RAMP_S_MCPL_IN) mcpl_in_step(n, c, (uint32_t)a.seed);
+                    else if (!a.pilot) mcpl_out_step(n, c, a.dump, a.dump_tag, a.dump_cap, a.chunk_tag);
+                }
+                tr_out<FULL>(n, c);
+                continue;
+            }
             if constexpr (!FULL && (F & FEAT_MONO) != 0) {
                 if (defer && k == a.defer_k) {
                     bool park = false;
@@ -409,7 +431,7 @@ void k_segment(SegArgs a)
         } else {
             n.dead = true;
             n.pend = false;
-            n.key = 0;
+            n.key = 0xffffffffu; /* no history on this lane */
             n.iidx = 0;
         }
         run_steps<FULL, F>(n, a, hc, dq, defer);
@@ -594,7 +616,7 @@ static void dev_free(DevBuf &b)
 }
 
 struct QueueBufs {
-    DevBuf f[15];   /* x y z vx vy vz w t ctr px py pz sc sd se */
+    DevBuf f[23];   /* x y z vx vy vz w t ctr px py pz sc sd se, intersection[8] */
     DevBuf key, status;
 };
 
@@ -623,6 +645,13 @@ struct ramp_plan {
     bool has_atomic = false, has_det = false;
     std::vector<int> bounds;  /* segment boundaries (step indices) */
     int defer_k = -1;         /* first linear monochromator step (reflections deferred) */
+    int revive_k = -1;        /* last SMCPLIn step: segment 0 runs at least through it */
+    bool has_mcpl = false;    /* SMCPLIn/SMCPLOut present: full-state execution */
+    uint32_t n_mcpl_out = 0;  /* SMCPLOut components (dump slots) */
+    DevBuf dump, dump_tag, sel_counts, sel_prefix, sel_out; /* SMCPLOut records and their compaction */
+    uint64_t dump_cap = 0;
+    uint32_t dump_chunk_tag = 0; /* tag of the last chunk of the last run */
+    uint64_t dump_n = 0;         /* histories of that chunk */
     std::vector<float> pd_lines; /* host copy of the SPowderDiff line table (adjoint) */
     bool bounds_set = false;
     int variant = kNumVariants - 1;
@@ -919,12 +948,16 @@ static int validate_plan(const ramp_plan_desc *d)
         if ((uint64_t)bl.first + bl.count > d->n_comps)
             return fail(RAMP_E_INVALID, "block %u out of range", b);
         if (!bl.linear && bl.multi > 100000) return fail(RAMP_E_RANGE, "block %u: multi too large", b);
+        if (!bl.linear)
+            for (uint32_t k = bl.first; k < bl.first + bl.count; k++)
+                if (d->comps[k].scat.type == RAMP_S_MCPL_IN || d->comps[k].scat.type == RAMP_S_MCPL_OUT)
+                    return fail(RAMP_E_INVALID, "component %u: SMCPLIn/SMCPLOut must be in a linear block", k);
     }
     for (uint32_t k = 0; k < d->n_comps; k++) {
         const ramp_component_desc &c = d->comps[k];
         if (c.geom.type < RAMP_G_ARM || c.geom.type > RAMP_G_MESH)
             return fail(RAMP_E_INVALID, "component %u: unknown geometry type %u", k, c.geom.type);
-        if (c.scat.type < RAMP_S_ARM || c.scat.type > RAMP_S_POWDERDIFF)
+        if (c.scat.type < RAMP_S_ARM || c.scat.type > RAMP_S_MCPL_OUT)
             return fail(RAMP_E_INVALID, "component %u: unknown scatter type %u", k, c.scat.type);
         if (c.comp_idx == 0 || c.comp_idx > 0x00ffffffu)
             return fail(RAMP_E_INVALID, "component %u: index %u out of range", k, c.comp_idx);
@@ -961,6 +994,12 @@ static int validate_plan(const ramp_plan_desc *d)
                 if (nq * nw && s.tab_len[ch] < nq + 2 * nw + nw * nq)
                     return fail(RAMP_E_INVALID, "component %u: S(Q,w) table %d too short", k, ch);
             }
+        if (s.type == RAMP_S_MCPL_IN) {
+            const uint64_t words = (uint64_t)s.ni[MCI_N] * MCR_N * (s.ni[MCI_DOUBLE] ? 2u : 1u);
+            if (s.tab_len[MCT_REC] < words || (s.ni[MCI_DOUBLE] && (s.tab_off[MCT_REC] & 1u)))
+                return fail(RAMP_E_INVALID, "component %u: MCPL record table shorter than %u particles "
+                            "or misaligned", k, s.ni[MCI_N]);
+        }
         if (s.type == RAMP_S_DETPIX && (uint64_t)s.ni[DTI_NTUBE] * s.ni[DTI_NPIX] * s.ni[DTI_NTOF] >
                                            (c.det >= 0 ? d->dets[c.det].n_bins : 0))
             return fail(RAMP_E_INVALID, "component %u: tube bank larger than its histogram", k);
@@ -1010,6 +1049,8 @@ extern "C" int ramp_plan_create(ramp_ctx *ctx, const ramp_plan_desc *d, ramp_pla
         for (int i = 0; i < 3; i++) dc.pos[i] = c.pos[i];
         rotation_matrix(c.rot, dc.R);
         dc.pad = (c.rot[0] == 0.0f && c.rot[1] == 0.0f && c.rot[2] == 0.0f) ? 1u : 0u;
+        if (c.scat.type == RAMP_S_MCPL_OUT) dc.s.ni[MCO_SLOT] = p->n_mcpl_out++;
+        if (c.scat.type == RAMP_S_MCPL_IN || c.scat.type == RAMP_S_MCPL_OUT) p->has_mcpl = true;
     }
     uint32_t feat = 0;
     if (d->source.type == RAMP_SRC_ISIS) feat |= FEAT_ISIS;
@@ -1042,6 +1083,9 @@ extern "C" int ramp_plan_create(ramp_ctx *ctx, const ramp_plan_desc *d, ramp_pla
     for (size_t k = 0; k < p->hsteps.size() && p->defer_k < 0; k++)
         if (p->hsteps[k].kind == 0 && p->hcomps[p->hsteps[k].first].s.type == RAMP_S_MONO)
             p->defer_k = (int)k;
+    for (size_t k = 0; k < p->hsteps.size(); k++)
+        if (p->hsteps[k].kind == 0 && p->hcomps[p->hsteps[k].first].s.type == RAMP_S_MCPL_IN)
+            p->revive_k = (int)k;
     p->det_desc.assign(d->dets, d->dets + d->n_dets);
     p->hdets.resize(d->n_dets);
     p->hw.resize(d->n_dets);
@@ -1077,7 +1121,8 @@ extern "C" int ramp_plan_destroy(ramp_plan *p)
     }
     DevBuf *single[] = {&p->comps, &p->steps, &p->src, &p->dets, &p->tables, &p->cnt, &p->count,
                         &p->slot_ctr, &p->slot_iidx, &p->st16, &p->st8, &p->stii,
-                        &p->rcounts, &p->prefix, &p->comps_brute};
+                        &p->rcounts, &p->prefix, &p->comps_brute, &p->dump, &p->dump_tag,
+                        &p->sel_counts, &p->sel_prefix, &p->sel_out};
     for (DevBuf *b : single) dev_free(*b);
     for (auto *v : {&p->hw, &p->hw2, &p->hq, &p->hq2, &p->mesh_nodes})
         for (auto &b : *v) dev_free(b);
@@ -1167,7 +1212,7 @@ static int ensure_queues(ramp_plan *p, uint64_t cap, bool full)
 {
     if (cap <= p->qcap && (!full || p->qfull)) return 0;
     uint64_t c = std::max(cap, p->qcap);
-    int nf = full ? 15 : 9;
+    int nf = full ? 23 : 9;
     for (int i = 0; i < 2; i++) {
         for (int f = 0; f < nf; f++) {
             int rc = dev_alloc(p->q[i].f[f], c * sizeof(float));
@@ -1185,9 +1230,10 @@ static int ensure_queues(ramp_plan *p, uint64_t cap, bool full)
 static DQueue queue_view(ramp_plan *p, int i)
 {
     DQueue q;
-    float **f[15] = {&q.x, &q.y, &q.z, &q.vx, &q.vy, &q.vz, &q.w, &q.t, &q.ctr,
-                     &q.px, &q.py, &q.pz, &q.sc, &q.sd, &q.se};
-    for (int k = 0; k < 15; k++) *f[k] = (float *)p->q[i].f[k].p;
+    float **f[23] = {&q.x, &q.y, &q.z, &q.vx, &q.vy, &q.vz, &q.w, &q.t, &q.ctr,
+                     &q.px, &q.py, &q.pz, &q.sc, &q.sd, &q.se,
+                     &q.is[0], &q.is[1], &q.is[2], &q.is[3], &q.is[4], &q.is[5], &q.is[6], &q.is[7]};
+    for (int k = 0; k < 23; k++) *f[k] = (float *)p->q[i].f[k].p;
     q.key = (uint32_t *)p->q[i].key.p;
     q.status = (uint32_t *)p->q[i].status.p;
     return q;
@@ -1258,14 +1304,15 @@ static int choose_bounds(ramp_plan *p, const SegArgs &base, uint64_t chunk, unsi
         a.num_sim = (int)pn;
         launch_fill_u64(s, p->cnt.p, CNT_N);
         unsigned grid = (unsigned)std::max<uint64_t>(1, (pn + wg - 1) / wg);
-        launch_segment(a, false, true, p->variant, grid, wg, s);
+        launch_segment(a, p->has_mcpl, true, p->variant, grid, wg, s);
         HIPCHK(hipGetLastError());
         std::vector<unsigned long long> cnt(CNT_N);
         HIPCHK(hipMemcpyAsync(cnt.data(), p->cnt.p, CNT_N * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
         double start = (double)pn;
         int lim = max_seg > 0 ? max_seg : 16;
-        for (int k = 0; k < ns - 1 && (int)p->bounds.size() < lim; k++) {
+        /* SMCPLIn revives dead slots, which only the first segment holds */
+        for (int k = std::max(0, p->revive_k); k < ns - 1 && (int)p->bounds.size() < lim; k++) {
             double alive = (double)cnt[CNT_NSTEP0 + k];
             if (alive < 0.5 * start) {
                 p->bounds.push_back(k + 1);
@@ -1302,7 +1349,9 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
     hipStream_t s = ctx->stream;
     unsigned wg = o.wg ? o.wg : 256;
     if (wg != 64 && wg != 128 && wg != 256) return fail(RAMP_E_INVALID, "workgroup size %u", wg);
-    bool full = o.keep_state != 0;
+    const bool keep = o.keep_state != 0;
+    /* the MCPL steps need the polarisation and the whole intersection */
+    const bool full = keep || p->has_mcpl;
     int rc = prepare_dets(p, (int)o.hist_mode, o.weight_bound, n_hist, chunk);
     if (rc) return rc;
     bool carry = n_chunks > 1;
@@ -1313,11 +1362,20 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
         launch_fill_f32(s, p->slot_ctr.p, chunk);
         launch_fill_u32(s, p->slot_iidx.p, chunk);
     }
-    if (full) {
+    if (keep) {
         rc = dev_alloc(p->st16, chunk * 16 * sizeof(float));
         if (!rc) rc = dev_alloc(p->st8, chunk * 8 * sizeof(float));
         if (!rc) rc = dev_alloc(p->stii, chunk * sizeof(uint32_t));
         if (rc) return rc;
+    }
+    if (p->n_mcpl_out && p->dump_cap < chunk) {
+        const uint64_t nslot = (uint64_t)p->n_mcpl_out * chunk;
+        rc = dev_alloc(p->dump, nslot * MCR_N * sizeof(float));
+        if (!rc) rc = dev_alloc(p->dump_tag, nslot * sizeof(uint32_t));
+        if (rc) return rc;
+        launch_fill_u32(s, (uint32_t *)p->dump_tag.p, nslot);
+        p->dump_cap = chunk;
+        p->dump_chunk_tag = 0;
     }
     SegArgs a;
     std::memset(&a, 0, sizeof a);
@@ -1339,8 +1397,12 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
     a.cnt = (unsigned long long *)p->cnt.p;
     a.hist_mode = (int)o.hist_mode;
     a.carry = carry ? 1 : 0;
-    a.keep = full ? 1 : 0;
+    a.keep = keep ? 1 : 0;
     a.defer_k = full ? -1 : p->defer_k;
+    a.revive_k = p->revive_k;
+    a.dump = (float *)p->dump.p;
+    a.dump_tag = (uint32_t *)p->dump_tag.p;
+    a.dump_cap = p->dump_cap;
     a.slot_ctr = (float *)p->slot_ctr.p;
     a.slot_iidx = (uint32_t *)p->slot_iidx.p;
     a.st16 = (float *)p->st16.p;
@@ -1374,6 +1436,7 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
         a.num_sim = (int)torun;
         a.seed = (int)seeds[ck];
         a.n = torun;
+        a.chunk_tag = ++p->dump_chunk_tag ? p->dump_chunk_tag : ++p->dump_chunk_tag;
         for (int sg = 0; sg < nseg; sg++) {
             a.s0 = p->bounds[sg];
             a.s1 = p->bounds[sg + 1];
@@ -1416,7 +1479,8 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
     p->stats.state_in = cnt[CNT_QIN];
     p->stats.ms_total = ms;
     p->stats.ms_source_segment = ms0;
-    p->st_n = full ? std::min(chunk, n_hist - (n_chunks - 1) * chunk) : 0;
+    p->st_n = keep ? std::min(chunk, n_hist - (n_chunks - 1) * chunk) : 0;
+    p->dump_n = std::min(chunk, n_hist - (n_chunks - 1) * chunk);
     return 0;
 }
 
@@ -1543,3 +1607,4 @@ extern "C" int ramp_rng_eval(ramp_ctx *ctx, const uint32_t *tid, const float *ct
 
 #include "batch_run.h"
 #include "adjoint_run.h"
+#include "mcpl_run.h"

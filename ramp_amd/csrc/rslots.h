@@ -171,5 +171,20 @@
 #define DTI_NPIX 1
 #define DTI_NTOF 2
 #define DTI_RESTORE 3
+/* SMCPLIn: particle records in the canonical layout MCR_* (floats, or
+ * doubles when MCI_DOUBLE is set), converted from the file on the host
+ * without arithmetic */
+#define MCI_N 0               /* particles in the file (at most 2^32 - 1) */
+#define MCI_DOUBLE 1          /* 1: records hold doubles (double-precision file) */
+#define MCT_REC 0
+#define MCR_PX 0              /* 0..2 polarisation */
+#define MCR_X 3               /* 3..5 position, cm */
+#define MCR_U 6               /* 6..8 packed direction (u1, u2, +-ekin in MeV) */
+#define MCR_T 9               /* time, ms */
+#define MCR_W 10              /* weight */
+#define MCR_N 11
+/* SMCPLOut */
+#define MCO_SLOT 0            /* index among the plan's SMCPLOut components */
+#define MCO_STANDARD 1        /* 1: MCPL's own direction packing instead of the reference's */
 
 #endif

@@ -21,6 +21,7 @@ from collections import OrderedDict
 import numpy as np
 
 from . import _capi as capi
+from . import _slots as S
 from . import components as comps_mod
 from .components import TablePool
 
@@ -223,6 +224,11 @@ class Instrument:
                                len(comp_list) - first))
         cds = (capi.ComponentDesc * max(1, len(comp_list)))()
         self._det_of = {}
+        self._mcpl_out = []
+        for c in comp_list:
+            if c.scat.type_code == capi.RAMP_S_MCPL_OUT:
+                c.scat.ni[S.MCO_SLOT] = len(self._mcpl_out)
+                self._mcpl_out.append(c)
         for k, c in enumerate(comp_list):
             d = cds[k]
             c.geom.fill(d.geom, pool)
@@ -260,6 +266,17 @@ class Instrument:
         self._det_bins = det_list
         self._desc_keep = (plan, cds, bds, dds, tables)
         return plan
+
+    def weight_bound(self, chunk):
+        """Upper bound of one history's weight at a detector (sets the
+        fixed-point scale of deterministic histograms): the source's, or
+        the largest particle weight of an SMCPLIn file if that is larger."""
+        wb = self.source.weight_bound(chunk)
+        for b in self.blocks:
+            for c in b.components.values():
+                if hasattr(c.scat, "max_weight"):
+                    wb = max(wb, c.scat.max_weight)
+        return wb
 
     def chunking(self, N, buf_max_mod=None):
         """Chunk size of a run of N histories (mcramp.py:415-420, 443-445)."""
@@ -312,7 +329,7 @@ class Instrument:
         opts.hist_mode = (capi.RAMP_HIST_DETERMINISTIC if self.hist_mode == "deterministic"
                           else capi.RAMP_HIST_ATOMIC)
         opts.keep_state = 1 if (keep_state or debug) else 0
-        opts.weight_bound = self.source.weight_bound(chunk) if N else 0.0
+        opts.weight_bound = self.weight_bound(chunk) if N else 0.0
         opts.flags = capi.RAMP_RUN_MESH_BRUTE_FORCE if mesh_brute_force else 0
         opts.wg = int(wg)
         opts.npt = int(npt)
@@ -323,6 +340,11 @@ class Instrument:
         self.last_chunk = chunk
         self.last_stats = self.stats()
         self._hist_stale = True
+        if N:
+            # SMCPLOut writes its file at every chunk (mcpl_out.py:99-117); the
+            # last chunk's records are what remains
+            for slot, c in enumerate(self._mcpl_out):
+                c.scat._collect(lib, self._plan, slot)
         if debug:
             n16, _, _ = self.state(min(chunk, N))
             live = n16[n16[:, 15] == 0.0]

@@ -103,6 +103,43 @@ def table_files():
 _TAB_DIR = None
 
 
+_MCPL_DIR = None
+
+
+def mcpl_fixtures():
+    return np.load(os.path.join(ROOT, "tests", "golden", "mcpl_fixtures.npz"))
+
+
+def mcpl_files():
+    """MCPL files written once per process from the fixtures: 'back', the
+    McStas-written particles of the reference's MCPLtestback.mcpl, in a
+    file with the same header structure (source name, one comment, two
+    blobs of the recorded lengths with filler bytes)."""
+    global _MCPL_DIR
+    import tempfile
+    from ramp_amd.mcpl import write_mcpl_full
+    if _MCPL_DIR is None:
+        _MCPL_DIR = tempfile.mkdtemp(prefix="ramp_mcpl_")
+        fx = mcpl_fixtures()
+        keys = bytes(fx["back_blob_keys"]).split(b"\0")
+        blobs = {k.decode(): b"#" * int(n) for k, n in zip(keys, fx["back_blob_lengths"])}
+        write_mcpl_full(os.path.join(_MCPL_DIR, "back.mcpl"), fx["back_records"],
+                        source_name=bytes(fx["back_source"]).decode(),
+                        comments=[bytes(fx["back_comment"]).decode()], blobs=blobs)
+    return {"back": os.path.join(_MCPL_DIR, "back.mcpl"), "dir": _MCPL_DIR}
+
+
+def mcpl_cases():
+    f = mcpl_files()
+    return {
+        "mcpl_in": (os.path.join(ROOT, "ramp_amd", "instruments", "mcpl_in.json"), {"mcpl_file": repr(f["back"])}),
+        "mcpl_out": (os.path.join(ROOT, "ramp_amd", "instruments", "mcpl_out.json"),
+                     {"mcpl_file": repr(os.path.join(f["dir"], "out_ref.mcpl")), "standard": 0}),
+        "mcpl_out_std": (os.path.join(ROOT, "ramp_amd", "instruments", "mcpl_out.json"),
+                         {"mcpl_file": repr(os.path.join(f["dir"], "out_std.mcpl")), "standard": 1}),
+    }
+
+
 def _powder_diff_vars():
     sys.path.insert(0, os.path.join(ROOT, "ramp_amd", "instruments"))
     import powder_diff
