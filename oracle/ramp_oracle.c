@@ -1462,8 +1462,9 @@ static void o_mcpl_in(const octx *cx, on_t *n, const ramp_part_desc *s)
     }
     double r[MCR_N];
     const float *t = tab(cx, s, MCT_REC);
+    const uint64_t np_ = s->ni[MCI_N];  /* the table is field-major */
     for (int k = 0; k < MCR_N; k++)
-        r[k] = s->ni[MCI_DOUBLE] ? ((const double *)t)[MCR_N * i + k] : (double)t[MCR_N * i + k];
+        r[k] = s->ni[MCI_DOUBLE] ? ((const double *)t)[k * np_ + i] : (double)t[k * np_ + i];
     double ekin = fabs(r[8]);
     double sgn = signbit(r[8]) ? -1.0 : 1.0;
     double dir[3];

@@ -1065,8 +1065,9 @@ class SMCPLIn(_Scat):
         self.max_weight = float(np.max(np.abs(rec[:, 10]))) if n else 0.0
         self.ni[S.MCI_N] = n
         self.ni[S.MCI_DOUBLE] = 0 if rec.dtype == np.float32 else 1
+        # field-major (each lane of a wave then reads contiguous words);
         # doubles travel in the float32 table pool as pairs of words
-        self.tables[S.MCT_REC] = np.ascontiguousarray(rec).view(np.float32).ravel()
+        self.tables[S.MCT_REC] = np.ascontiguousarray(rec.T).view(np.float32).ravel()
 
 
 class SMCPLOut(_Scat):

@@ -14,8 +14,9 @@
  * SMCPLOut (scat/mcpl_out.py:55-117).  Every live neutron of the chunk is
  * written as an 11-float record (write_particle, :55-73), whatever its
  * iidx; the state, iidx and intersection are left unchanged.  Lane i
- * writes record i of the component's dump and tags it with the chunk
- * number, so the records of the last chunk are found in history order.
+ * writes record i of the component's dump (field-major: field k of record
+ * i at k * cap + i) and tags it with the chunk number, so the records of
+ * the last chunk are found in history order.
  */
 #ifndef RAMP_DEV_MCPL_H
 #define RAMP_DEV_MCPL_H
@@ -140,13 +141,15 @@ DEV void mcpl_in_step(NS &n, const DComp &c, uint32_t seed)
     const uint32_t i = n.key;
     n.sc = n.sd = n.se = 0.0f;
     if (i < s.ni[MCI_N]) {
+        /* field-major table: field k of particle i at k * N + i */
+        const size_t N = s.ni[MCI_N];
         double rec[MCR_N];
         if (s.ni[MCI_DOUBLE]) {
-            const double *r = (const double *)s.tab[MCT_REC] + (size_t)MCR_N * i;
-            for (int k = 0; k < MCR_N; k++) rec[k] = r[k];
+            const double *r = (const double *)s.tab[MCT_REC] + i;
+            for (int k = 0; k < MCR_N; k++) rec[k] = r[k * N];
         } else {
-            const float *r = s.tab[MCT_REC] + (size_t)MCR_N * i;
-            for (int k = 0; k < MCR_N; k++) rec[k] = (double)r[k];
+            const float *r = s.tab[MCT_REC] + i;
+            for (int k = 0; k < MCR_N; k++) rec[k] = (double)r[k * N];
         }
         float o[12];
         mcpl_to_state(rec, i, seed, o);
@@ -166,13 +169,14 @@ DEV void mcpl_out_step(const NS &n, const DComp &c, float *dump, uint32_t *tags,
                        uint32_t tag)
 {
     if (n.dead || n.key == 0xffffffffu) return;
-    const uint64_t slot = (uint64_t)c.s.ni[MCO_SLOT] * cap + n.key;
+    const uint64_t s = c.s.ni[MCO_SLOT];
     const float st[11] = {n.x, n.y, n.z, n.vx, n.vy, n.vz, n.px, n.py, n.pz, n.w, n.t};
     float rec[MCR_N];
     mcpl_record(st, (int)c.s.ni[MCO_STANDARD], rec);
-    float *o = dump + (size_t)MCR_N * slot;
-    for (int k = 0; k < MCR_N; k++) o[k] = rec[k];
-    tags[slot] = tag;
+    /* field-major, so that the lanes of a wave store contiguously */
+    float *o = dump + (size_t)s * MCR_N * cap + n.key;
+    for (int k = 0; k < MCR_N; k++) o[(size_t)k * cap] = rec[k];
+    tags[s * cap + n.key] = tag;
 }
 #endif
 

@@ -60,11 +60,18 @@ __global__ __launch_bounds__(1024) void k_sel_scan(const uint32_t *counts, uint3
     if (t == 1023) prefix[G] = part[1023];
 }
 
+/* The records sit field-major in the dump (field k of record i at
+ * k * cap + i); the output is the file's record-major layout.  Per group of
+ * 64 candidates the wave ranks its hits, keeps their lanes in LDS, and then
+ * all 64 lanes copy the group's cnt * 11 output floats in order, so every
+ * store instruction writes 256 contiguous bytes. */
 __global__ __launch_bounds__(256) void k_sel_write(const uint32_t *tags, const float *recs, uint64_t n,
-                                                   uint32_t tag, uint32_t G, const uint64_t *prefix,
-                                                   float *out)
+                                                   uint64_t cap, uint32_t tag, uint32_t G,
+                                                   const uint64_t *prefix, float *out)
 {
-    const uint32_t g = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    __shared__ uint32_t src[4][64];
+    const uint32_t wv = threadIdx.x >> 6;
+    const uint32_t g = blockIdx.x * (blockDim.x >> 6) + wv;
     if (g >= G) return;
     uint64_t b, e;
     sel_range(n, G, g, b, e);
@@ -74,13 +81,16 @@ __global__ __launch_bounds__(256) void k_sel_write(const uint32_t *tags, const f
         const uint64_t i = base + lane;
         const bool hit = i < e && tags[i] == tag;
         const unsigned long long m = __ballot(hit);
-        if (hit) {
-            const uint64_t d = o + (uint64_t)__popcll(m & ((1ull << lane) - 1ull));
-            const float *r = recs + (size_t)MCR_N * i;
-            float *w = out + (size_t)MCR_N * d;
-            for (int k = 0; k < MCR_N; k++) w[k] = r[k];
+        const uint32_t cnt = (uint32_t)__popcll(m);
+        if (hit) src[wv][__popcll(m & ((1ull << lane) - 1ull))] = lane;
+        wave_lds_sync();
+        float *w = out + (size_t)MCR_N * o;
+        for (uint32_t j = lane; j < cnt * MCR_N; j += 64) {
+            const uint32_t r = j / MCR_N, k = j - r * MCR_N;
+            w[j] = recs[(size_t)k * cap + base + src[wv][r]];
         }
-        o += (uint64_t)__popcll(m);
+        wave_lds_sync();
+        o += cnt;
     }
 }
 
@@ -99,7 +109,7 @@ extern "C" int ramp_mcpl_read(ramp_plan *p, uint32_t comp, float *records, uint6
     if (!rc) rc = dev_alloc(p->sel_prefix, (G + 1) * sizeof(uint64_t));
     if (rc) return rc;
     const uint32_t *tags = (const uint32_t *)p->dump_tag.p + (size_t)comp * p->dump_cap;
-    const float *recs = (const float *)p->dump.p + (size_t)comp * p->dump_cap * MCR_N;
+    const float *recs = (const float *)p->dump.p + (size_t)comp * MCR_N * p->dump_cap;
     const uint32_t tag = p->dump_chunk_tag;
     const unsigned blocks = (G + 3) / 4;
     ramp_ctx *ctx = p->ctx;
@@ -118,8 +128,8 @@ extern "C" int ramp_mcpl_read(ramp_plan *p, uint32_t comp, float *records, uint6
         rc = dev_alloc(p->sel_out, total * MCR_N * sizeof(float));
         if (rc) return rc;
         HIPCHK(hipEventRecord(ctx->ev[2], s));
-        hipLaunchKernelGGL(k_sel_write, dim3(blocks), dim3(256), 0, s, tags, recs, n, tag, G,
-                           (const uint64_t *)p->sel_prefix.p, (float *)p->sel_out.p);
+        hipLaunchKernelGGL(k_sel_write, dim3(blocks), dim3(256), 0, s, tags, recs, n, p->dump_cap, tag,
+                           G, (const uint64_t *)p->sel_prefix.p, (float *)p->sel_out.p);
         HIPCHK(hipEventRecord(ctx->ev[3], s));
         HIPCHK(hipGetLastError());
     }

@@ -173,7 +173,8 @@
 #define DTI_RESTORE 3
 /* SMCPLIn: particle records in the canonical layout MCR_* (floats, or
  * doubles when MCI_DOUBLE is set), converted from the file on the host
- * without arithmetic */
+ * without arithmetic and stored field-major (field k of particle i at
+ * k * MCI_N + i) */
 #define MCI_N 0               /* particles in the file (at most 2^32 - 1) */
 #define MCI_DOUBLE 1          /* 1: records hold doubles (double-precision file) */
 #define MCT_REC 0
