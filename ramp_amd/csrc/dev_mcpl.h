@@ -26,19 +26,16 @@
  * dropped, |u2| > 1 means u2 = 1/uz with y dropped, else z is dropped. */
 RHD void mcpl_unpack_dir(double p0, double p1, double s, double *ux, double *uy, double *uz)
 {
-    if (fabs(p0) > 1.0) {
-        *uz = 1.0 / p0;
-        *uy = p1;
-        *ux = s * sqrt(fmax(0.0, 1.0 - (*uy * *uy + *uz * *uz)));
-    } else if (fabs(p1) > 1.0) {
-        *uz = 1.0 / p1;
-        *ux = p0;
-        *uy = s * sqrt(fmax(0.0, 1.0 - (*ux * *ux + *uz * *uz)));
-    } else {
-        *ux = p0;
-        *uy = p1;
-        *uz = s * sqrt(fmax(0.0, 1.0 - (*ux * *ux + *uy * *uy)));
-    }
+    /* written with selects: stores through branch-dependent pointers would
+     * keep the outputs in scratch memory */
+    const bool xd = fabs(p0) > 1.0;
+    const bool yd = !xd && fabs(p1) > 1.0;
+    const double a = xd ? p1 : p0;                           /* uy if x is dropped, else ux */
+    const double b = xd ? 1.0 / p0 : (yd ? 1.0 / p1 : p1);   /* uz if x or y is dropped, else uy */
+    const double d = s * sqrt(fmax(0.0, 1.0 - (a * a + b * b)));
+    *ux = xd ? d : a;
+    *uy = xd ? a : (yd ? d : b);
+    *uz = (xd || yd) ? b : d;
 }
 
 /* the inverse, for a unit vector (z kept as the dropped component on ties) */
@@ -139,30 +136,32 @@ DEV void mcpl_in_step(NS &n, const DComp &c, uint32_t seed)
     if (n.key == 0xffffffffu) return; /* no history on this lane */
     const DPart &s = c.s;
     const uint32_t i = n.key;
-    n.sc = n.sd = n.se = 0.0f;
-    if (i < s.ni[MCI_N]) {
+    const bool have = i < s.ni[MCI_N];
+    double rec[MCR_N];
+#pragma unroll
+    for (int k = 0; k < MCR_N; k++) rec[k] = 0.0;
+    if (have) {
         /* field-major table: field k of particle i at k * N + i */
         const size_t N = s.ni[MCI_N];
-        double rec[MCR_N];
         if (s.ni[MCI_DOUBLE]) {
             const double *r = (const double *)s.tab[MCT_REC] + i;
+#pragma unroll
             for (int k = 0; k < MCR_N; k++) rec[k] = r[k * N];
         } else {
             const float *r = s.tab[MCT_REC] + i;
+#pragma unroll
             for (int k = 0; k < MCR_N; k++) rec[k] = (double)r[k * N];
         }
-        float o[12];
-        mcpl_to_state(rec, i, seed, o);
-        n.x = o[0]; n.y = o[1]; n.z = o[2];
-        n.vx = o[3]; n.vy = o[4]; n.vz = o[5];
-        n.px = o[6]; n.py = o[7]; n.pz = o[8];
-        n.w = o[9]; n.t = o[10]; n.ctr = o[11];
-        n.dead = false;
-    } else {
-        n.x = n.y = n.z = n.vx = n.vy = n.vz = n.px = n.py = n.pz = 0.0f;
-        n.w = n.t = n.ctr = 0.0f;
-        n.dead = true;
     }
+    float o[12];
+    mcpl_to_state(rec, i, seed, o);
+    /* beyond the file's particles every field is zero and the slot dead */
+    n.x = have ? o[0] : 0.0f; n.y = have ? o[1] : 0.0f; n.z = have ? o[2] : 0.0f;
+    n.vx = have ? o[3] : 0.0f; n.vy = have ? o[4] : 0.0f; n.vz = have ? o[5] : 0.0f;
+    n.px = have ? o[6] : 0.0f; n.py = have ? o[7] : 0.0f; n.pz = have ? o[8] : 0.0f;
+    n.w = have ? o[9] : 0.0f; n.t = have ? o[10] : 0.0f; n.ctr = have ? o[11] : 0.0f;
+    n.sc = n.sd = n.se = 0.0f;
+    n.dead = !have;
 }
 
 DEV void mcpl_out_step(const NS &n, const DComp &c, float *dump, uint32_t *tags, uint64_t cap,

@@ -212,21 +212,22 @@ template <bool FULL, uint32_t F>
 DEV void run_range(NS &n, const SegArgs &a, HistCtx &hc, int k0, int k1, DeferQ &dq, bool defer)
 {
     for (int k = k0; k < k1; k++) {
-        if (!a.pilot && k > a.revive_k && !__any(!n.dead)) break;
+        /* SMCPLIn revives dead slots; its plans run in full-state mode only */
+        if (!a.pilot && (!FULL || k > a.revive_k) && !__any(!n.dead)) break;
         const DStep st = a.steps[k];
         if (st.kind == 0) {
             const DComp &c = a.comps[st.first];
             tr_in<FULL>(n, c);
             geom_dispatch<F>(n, c);
             terminate(n, c.restore);
-            if (c.s.type == RAMP_S_MCPL_IN || c.s.type == RAMP_S_MCPL_OUT) {
-                /* plans holding these run in full-state mode only */
-                if constexpr (FULL) {
+            if constexpr (FULL) {
+                /* plans holding MCPL components run in full-state mode only */
+                if (c.s.type == RAMP_S_MCPL_IN || c.s.type == RAMP_S_MCPL_OUT) {
                     if (c.s.type == RAMP_S_MCPL_IN) mcpl_in_step(n, c, (uint32_t)a.seed);
                     else if (!a.pilot) mcpl_out_step(n, c, a.dump, a.dump_tag, a.dump_cap, a.chunk_tag);
+                    tr_out<FULL>(n, c);
+                    continue;
                 }
-                tr_out<FULL>(n, c);
-                continue;
             }
             if constexpr (!FULL && (F & FEAT_MONO) != 0) {
                 if (defer && k == a.defer_k) {
