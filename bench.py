@@ -60,6 +60,7 @@ WORKLOADS = {
                            bins=2048 * 256),
     "mcpl_out": dict(file="mcpl_out.json", histories=2e7, name="mcpl_out_guide_dump_2e7", bins=1000),
     "mcpl_in": dict(file="mcpl_in.json", histories=2e7, name="mcpl_in_2e7_particles", bins=180 * 560),
+    "rescal": dict(file="rescal.json", histories=5e7, name="rescal_events_5e7", bins=1),
 }
 
 
@@ -139,7 +140,7 @@ def main():
         return bench_tas_scan(a, wl, sync)
     if a.workload == "powder_adjoint":
         return bench_powder_adjoint(a, wl, sync)
-    if a.workload in ("mcpl_out", "mcpl_in"):
+    if a.workload in ("mcpl_out", "mcpl_in", "rescal"):
         return bench_mcpl(a, wl, sync)
     tmpdir = tempfile.mkdtemp(prefix="ramp_bench_")
     kw = instrument_args(a.workload, tmpdir)
@@ -438,6 +439,8 @@ def bench_mcpl(a, wl, sync):
     tmpdir = tempfile.mkdtemp(prefix="ramp_bench_")
     if a.workload == "mcpl_out":
         kw = {"mcpl_file": repr(""), "standard": 0}
+    elif a.workload == "rescal":
+        kw = {"tx": 0.3, "tz": 1.0, "arm_rot": math.atan2(0.3, 1.0)}
     else:
         rng = np.random.default_rng(7)
         rec = np.zeros((N, 11), np.float32)
@@ -457,7 +460,8 @@ def bench_mcpl(a, wl, sync):
     for w in range(a.warmup):
         inst.execute(N, seeds=[(seed + w) % 4097])
     sync()
-    dev_ms, run_ms, sel_ms, wr_ms, recs = 0.0, 0.0, 0.0, 0.0, 0
+    dev_ms, run_ms, sel_ms, wr_ms, recs, app_ms = 0.0, 0.0, 0.0, 0.0, 0, 0.0
+    ev0 = 0
     t0 = time.perf_counter()
     for k in range(a.steps):
         inst.execute(N, seeds=[(seed + 100 + k) % 4097])
@@ -467,6 +471,8 @@ def bench_mcpl(a, wl, sync):
             sel_ms += st["ms_mcpl_compact"]
             wr_ms += st["ms_mcpl_write"]
             recs += inst._mcpl_out[0].scat.records.shape[0]
+        if a.workload == "rescal":
+            app_ms += st["ms_rescal_append"]
     sync()
     dt = time.perf_counter() - t0
     K = max(a.steps, 1)
@@ -475,8 +481,8 @@ def bench_mcpl(a, wl, sync):
         "metric": METRIC, "value": N / (dev_ms * 1e-3), "unit": "histories/s", "n_gpus": 1,
         "steps": K, "warmup": a.warmup, "ms_per_step": dev_ms, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic: histories generated on the device" if a.workload == "mcpl_out"
-                else "synthetic MCPL file of N particles, resident in HBM",
+        "data": "synthetic MCPL file of N particles, resident in HBM" if a.workload == "mcpl_in"
+                else "synthetic: histories generated on the device",
         "config": {"workload": wl["name"], "histories_per_step": N, "chunk": N,
                    "instrument": "ramp_amd/instruments/" + wl["file"],
                    "run_ms": run_ms / K, "wall_ms_per_step_with_host_copy": dt / K * 1e3,
@@ -493,6 +499,22 @@ def bench_mcpl(a, wl, sync):
         out["roofline"] = {"bound": "hbm", "kernel": "k_sel_write (record compaction)",
                            "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": ach / HBM_PEAK_GBS, "traffic": None, "launch_ms": w_ms,
+                           "algorithmic_bytes_per_launch": alg}
+    elif a.workload == "rescal":
+        import ctypes as C
+        cnt = C.c_uint64(0)
+        lib = ramp_amd._capi.load_library()
+        lib.ramp_rescal_read(inst._plan, 0, None, 0, C.byref(cnt))
+        E = cnt.value / (a.warmup + K)  # events per step (the list accumulates)
+        a_ms = app_ms / K
+        # k_ev_append and k_sel_count: 4 B tag per history read twice, 20 B
+        # event read and 20 B written per kept event
+        alg = 8.0 * N + 40.0 * E
+        ach = alg / (a_ms * 1e-3) / 1e9 if a_ms > 0 else 0.0
+        out["config"].update({"events_per_step": E, "append_ms": a_ms})
+        out["roofline"] = {"bound": "hbm", "kernel": "SRescal event append (k_sel_count, k_sel_scan, "
+                           "k_ev_append, k_ev_advance)", "achieved": ach, "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None, "launch_ms": a_ms,
                            "algorithmic_bytes_per_launch": alg}
     else:
         # the source segment reads each particle's 44-byte record

@@ -72,6 +72,7 @@ extern "C" {
 #define RAMP_S_POWDERDIFF 15 /* SPowderDiff    differentiable powder + incoherent sample (no reference counterpart) */
 #define RAMP_S_MCPL_IN 16    /* SMCPLIn        scat/mcpl_in.py:9-76 (particles from an MCPL file) */
 #define RAMP_S_MCPL_OUT 17   /* SMCPLOut       scat/mcpl_out.py:7-117 (live particles to an MCPL file) */
+#define RAMP_S_RESCAL 18     /* SRescal        scat/rescal.cl:5-77, rescal.py:9-66 (TAS resolution events) */
 
 /* ---- plan descriptors --------------------------------------------------- */
 #define RAMP_NP 48  /* float parameters per component part */
@@ -157,6 +158,7 @@ typedef struct {
     double ms_source_segment;/* device time of the first (source) segment of the last run */
     double ms_mcpl_compact;  /* last ramp_mcpl_read: device time of its three compaction kernels */
     double ms_mcpl_write;    /* last ramp_mcpl_read: device time of the record-writing pass alone */
+    double ms_rescal_append; /* device time of the first chunk's SRescal event append (count, scan, append) */
 } ramp_run_stats;
 
 typedef struct ramp_ctx ramp_ctx;
@@ -209,6 +211,21 @@ int ramp_stats_read(const ramp_plan *plan, ramp_run_stats *out);
  * which copies the whole neutron buffer back and filters it in Python;
  * the file header is written by the caller. */
 int ramp_mcpl_read(ramp_plan *plan, uint32_t comp, float *records, uint64_t cap, uint64_t *count);
+
+/* ---- SRescal event lists --------------------------------------------------
+ * Events of SRescal component `comp` (its index among the plan's SRescal
+ * components, in plan order), accumulated over every chunk of every run
+ * since the plan was created or ramp_hist_reset was called: for each
+ * history that scattered there and ended its block detected (se > 0),
+ * (qx, qy, qz) = ki - kf, the energy transfer and the weight, in history
+ * order within a chunk and chunk order.  The device appends each chunk's
+ * events after the chunk (a compaction kernel), so nothing crosses PCIe
+ * during a run.  *count receives the number of events; min(count, cap) of
+ * them are copied field-major to `events`: qx[cap], qy[cap], qz[cap],
+ * dE[cap], p[cap] (events may be NULL with cap = 0).  Replaces
+ * SRescal.data_reduce (rescal.py:51-62), which copies the event and neutron
+ * buffers of every chunk back and filters them on the host. */
+int ramp_rescal_read(ramp_plan *plan, uint32_t comp, float *events, uint64_t cap, uint64_t *count);
 
 /* ---- gradients ----------------------------------------------------------
  * Gradient of a loss of one detector's histogram with respect to the model

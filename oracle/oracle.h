@@ -48,6 +48,13 @@ int oracle_adjoint(const ramp_plan_desc *desc, uint64_t n_hist, uint64_t chunk,
  * records: SMCPLOut writes nothing. */
 void oracle_set_mcpl_dump(float *records, uint32_t *tags, uint64_t cap);
 
+/* Where SRescal event lists go: events[slot][5][cap] (qx, qy, qz, dE, p),
+ * appended at counts[slot] (not cleared), in chunk and history order. */
+void oracle_set_rescal_out(float *events, uint64_t cap, uint64_t *counts);
+
+/* Test hook: the shared rotate-about-axis restatement (geom.h:1-19). */
+void oracle_rotate_axis(float angle, const float *axis, const float *v, float *out);
+
 const char *oracle_last_error(void);
 
 #ifdef __cplusplus

@@ -38,6 +38,8 @@ def load():
                                        C.POINTER(C.c_uint32), C.c_uint32, C.c_uint32,
                                        C.POINTER(C.c_float), C.POINTER(C.c_double), C.c_uint32,
                                        C.c_int]
+        lib.oracle_set_rescal_out.restype = None
+        lib.oracle_set_rescal_out.argtypes = [C.POINTER(C.c_float), C.c_uint64, C.POINTER(C.c_uint64)]
         lib.oracle_set_mcpl_dump.restype = None
         lib.oracle_set_mcpl_dump.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_uint32), C.c_uint64]
         lib.oracle_last_error.restype = C.c_char_p
@@ -64,6 +66,12 @@ def run(inst, n_hist, seeds, chunk=0, keep_state=False, threads=0):
         tags = np.zeros((n_out, ch0), np.uint32)
         lib.oracle_set_mcpl_dump(dump.ctypes.data_as(C.POINTER(C.c_float)),
                                  tags.ctypes.data_as(C.POINTER(C.c_uint32)), ch0)
+    n_rs = sum(1 for c in inst._comp_list if type(c.scat).__name__ == "SRescal")
+    if n_rs:
+        rs_ev = np.zeros((n_rs, 5, max(int(n_hist), 1)), np.float32)
+        rs_cnt = np.zeros(n_rs, np.uint64)
+        lib.oracle_set_rescal_out(rs_ev.ctypes.data_as(C.POINTER(C.c_float)), max(int(n_hist), 1),
+                                  rs_cnt.ctypes.data_as(C.POINTER(C.c_uint64)))
     dets = [int(desc.dets[i].n_bins) for i in range(desc.n_dets)]
     hw = [np.zeros(n, np.float64) for n in dets]
     hw2 = [np.zeros(n, np.float64) for n in dets]
@@ -90,6 +98,8 @@ def run(inst, n_hist, seeds, chunk=0, keep_state=False, threads=0):
                         int(threads), C.byref(oob))
     if n_out:
         lib.oracle_set_mcpl_dump(None, None, 0)
+    if n_rs:
+        lib.oracle_set_rescal_out(None, 0, None)
     if rc != 0:
         raise RuntimeError(f"oracle error {rc}: {lib.oracle_last_error().decode()}")
     out = {"hist": list(zip(hw, hw2)), "oob": oob.value}
@@ -98,6 +108,8 @@ def run(inst, n_hist, seeds, chunk=0, keep_state=False, threads=0):
     if n_out:
         n_chunks = int(np.ceil(n_hist / ch0)) if n_hist else 0
         out["mcpl"] = [dump[s][tags[s] == n_chunks] for s in range(n_out)]
+    if n_rs:
+        out["rescal"] = [rs_ev[r][:, :int(rs_cnt[r])] for r in range(n_rs)]
     return out
 
 

@@ -1402,6 +1402,48 @@ static void s_detpix(octx *cx, on_t *n, const ramp_component_desc *c)
     s_done(n, c->comp_idx);
 }
 
+/* SRescal events of the running chunk: [slot][history][5], zeroed per chunk */
+static float *g_rs_ev;
+static uint64_t g_rs_cap;
+
+/* SRescal (scat/rescal.cl:5-77).  The event is ki - kf, the weight and the
+ * energy transfer; rescal.py:51-62 keeps it when the history ends its block
+ * with se > 0 (done by oracle_run after each chunk). */
+static void s_rescal(on_t *n, const ramp_part_desc *s, uint32_t ci)
+{
+    rf3 pos1 = is_pos1(n);
+    rf3 path = v3_sub(is_pos2(n), pos1);
+    float u = ornd(n);
+    set_pos(n, rf3_make(pos1.x + u * path.x, pos1.y + u * path.y, pos1.z + u * path.z));
+    rf3 v = n_vel(n);
+    rf3 ki = rf3_make(RC_V2K * v.x, RC_V2K * v.y, RC_V2K * v.z);
+    rf3 target = rf3_make(s->p[RS_T], s->p[RS_T + 1], s->p[RS_T + 2]);
+    float l2 = v3_len(target) * v3_len(target);
+    float fr = s->p[RS_FOCUS_R];
+    float costhetamax = rm_sqrt(l2 / (fr * fr + l2));
+    float theta = rm_acos(1.0f - ornd(n) * (1.0f - costhetamax));
+    float phi = (float)((double)(ornd(n) * 2.0f) * RC_M_PI_D); /* FP64 (M_PI) */
+    rf3 tperp = (target.x == 0.0f && target.z == 0.0f) ? rf3_make(1.0f, 0.0f, 0.0f)
+                                                       : rf3_make(-target.z, 0.0f, target.x);
+    rf3 tpp = v3_cross(target, tperp);
+    rf3 tf = v3_rotate_axis(theta, tpp, target);
+    tf = v3_rotate_axis(phi, target, tf);
+    tf = v3_norm(tf);
+    float Ef = s->p[RS_E0] + s->p[RS_DE] * (1.0f - 2.0f * ornd(n));
+    float vf = RC_SE2V * rm_sqrt(Ef);
+    set_vel(n, rf3_make(tf.x * vf, tf.y * vf, tf.z * vf));
+    v = n_vel(n);
+    rf3 kf = rf3_make(RC_V2K * v.x, RC_V2K * v.y, RC_V2K * v.z);
+    rf3 kv = rf3_make(RC_K2V * ki.x, RC_K2V * ki.y, RC_K2V * ki.z);
+    float *e = g_rs_ev + ((size_t)s->ni[RSI_SLOT] * g_rs_cap + n->tid) * 5;
+    e[0] = ki.x - kf.x;
+    e[1] = ki.y - kf.y;
+    e[2] = ki.z - kf.z;
+    e[3] = RC_VS2E * rm_pow(v3_len(kv), 2.0f) - Ef;
+    e[4] = n->N_P;
+    s_done(n, ci);
+}
+
 static void o_scatter(octx *cx, on_t *n, const ramp_component_desc *c)
 {
     if (n->iidx != c->comp_idx) return;
@@ -1423,6 +1465,7 @@ static void o_scatter(octx *cx, on_t *n, const ramp_component_desc *c)
     case RAMP_S_ABSORBER: s_absorber(n, s, c->comp_idx); break;
     case RAMP_S_DETPIX: s_detpix(cx, n, c); break;
     case RAMP_S_POWDERDIFF: s_powderdiff(n, tab(cx, s, PDT_LINES), s, c->comp_idx); break;
+    case RAMP_S_RESCAL: s_rescal(n, s, c->comp_idx); break;
     default: break;
     }
 }
@@ -1433,6 +1476,17 @@ static void o_scatter(octx *cx, on_t *n, const ramp_component_desc *c)
 static float *g_dump_rec;
 static uint32_t *g_dump_tag;
 static uint64_t g_dump_cap;
+
+/* SRescal event lists of a run: [slot][5][cap] field-major, counts[slot] */
+static float *g_rs_out;
+static uint64_t *g_rs_count, g_rs_out_cap;
+
+void oracle_set_rescal_out(float *events, uint64_t cap, uint64_t *counts)
+{
+    g_rs_out = events;
+    g_rs_out_cap = cap;
+    g_rs_count = counts;
+}
 
 void oracle_set_mcpl_dump(float *records, uint32_t *tags, uint64_t cap)
 {
@@ -1614,6 +1668,18 @@ int oracle_run(const ramp_plan_desc *desc, uint64_t n_hist, uint64_t chunk, cons
         snprintf(g_err, sizeof g_err, "out of memory");
         return RAMP_E_NOMEM;
     }
+    uint32_t n_rs = 0;
+    for (uint32_t k = 0; k < desc->n_comps; k++)
+        if (desc->comps[k].scat.type == RAMP_S_RESCAL) n_rs++;
+    if (n_rs) {
+        g_rs_cap = chunk;
+        g_rs_ev = (float *)malloc((size_t)n_rs * chunk * 5 * sizeof(float));
+        if (!g_rs_ev) {
+            free(slots);
+            snprintf(g_err, sizeof g_err, "out of memory");
+            return RAMP_E_NOMEM;
+        }
+    }
     octx cx;
     cx.d = desc;
     cx.hw = hist_w;
@@ -1633,6 +1699,7 @@ int oracle_run(const ramp_plan_desc *desc, uint64_t n_hist, uint64_t chunk, cons
         int64_t nt = (int64_t)torun;
         cx.seed = seed;
         cx.ck = (uint32_t)ck;
+        if (n_rs) memset(g_rs_ev, 0, (size_t)n_rs * chunk * 5 * sizeof(float));
 #pragma omp parallel for schedule(dynamic, 1024)
         for (int64_t i = 0; i < nt; i++) {
             on_t *n = &slots[i];
@@ -1644,6 +1711,15 @@ int oracle_run(const ramp_plan_desc *desc, uint64_t n_hist, uint64_t chunk, cons
             o_source(&cx, n, (int)torun, seed);
             for (uint32_t b = 0; b < desc->n_blocks; b++) o_block(&cx, n, &desc->blocks[b]);
         }
+        /* rescal.py:51-62: the events of histories detected (se > 0) at the
+         * end of the block, appended in slot order */
+        for (uint32_t r = 0; r < n_rs && g_rs_out; r++)
+            for (int64_t i = 0; i < nt; i++) {
+                if (!(slots[i].N_SE > 0.0f) || g_rs_count[r] >= g_rs_out_cap) continue;
+                const float *e = g_rs_ev + ((size_t)r * chunk + i) * 5;
+                for (int f = 0; f < 5; f++) g_rs_out[((size_t)r * 5 + f) * g_rs_out_cap + g_rs_count[r]] = e[f];
+                g_rs_count[r]++;
+            }
     }
     if (neutrons16 || isect8 || iidx) {
         for (uint64_t i = 0; i < torun; i++) {
@@ -1654,6 +1730,8 @@ int oracle_run(const ramp_plan_desc *desc, uint64_t n_hist, uint64_t chunk, cons
     }
     if (n_oob_out) *n_oob_out = cx.oob;
     free(slots);
+    free(g_rs_ev);
+    g_rs_ev = NULL;
     return 0;
 }
 
@@ -1694,6 +1772,12 @@ int oracle_adjoint(const ramp_plan_desc *desc, uint64_t n_hist, uint64_t chunk, 
 
 /* Test hook: the full 4-word Threefry-4x32-20 block of the shared RNG
  * header, compared with the reference's Random123 output by the tests. */
+void oracle_rotate_axis(float angle, const float *axis, const float *v, float *out)
+{
+    rf3 r = v3_rotate_axis(angle, rf3_make(axis[0], axis[1], axis[2]), rf3_make(v[0], v[1], v[2]));
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+
 void oracle_threefry4x32(const uint32_t *in, const uint32_t *key, uint32_t *out)
 {
     rr_threefry4x32_20(in, key, out);

@@ -113,6 +113,7 @@ class RunStats(C.Structure):
         ("ms_source_segment", C.c_double),
         ("ms_mcpl_compact", C.c_double),
         ("ms_mcpl_write", C.c_double),
+        ("ms_rescal_append", C.c_double),
     ]
 
 
@@ -143,6 +144,8 @@ _SIGS = {
     "ramp_stats_read": (C.c_int, [C.c_void_p, C.POINTER(RunStats)]),
     "ramp_mcpl_read": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_float), C.c_uint64,
                                  C.POINTER(C.c_uint64)]),
+    "ramp_rescal_read": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_float), C.c_uint64,
+                                   C.POINTER(C.c_uint64)]),
     "ramp_math_eval": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float),
                                  C.POINTER(C.c_float), C.c_uint64, C.c_int]),
     "ramp_rng_eval": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_float),

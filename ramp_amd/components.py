@@ -1034,8 +1034,43 @@ class SCollimatorLin(_OutOfScope):
     reason = "the reference kernel does not compile (collimator_lin.cl:30)"
 
 
-class SRescal(_OutOfScope):
-    reason = "host-side resolution event list, not part of the MI355X neutron path"
+class SRescal(_Scat):
+    """scat/rescal.cl:5-77 + rescal.py:9-66: scatters at a random depth
+    into the cone of radius focus_r about `target` with a final energy
+    uniform in E0 +- dE, and records the event (Q = ki - kf, energy
+    transfer, weight) of every history that ends its block detected
+    (se > 0, as an SCounter sets it).  data() returns (qx, qy, qz, dE, p),
+    accumulated over all chunks and runs as the reference's lists are.
+    The events stay on the GPU and are appended there after each chunk;
+    they cross to the host only when data() is read.  The component must
+    sit in the instrument's last block."""
+    type_code = capi.RAMP_S_RESCAL
+
+    def __init__(self, target=(0, 0, 0), E0=0, dE=0, focus_r=0, idx=0, ctx=None, **kwargs):
+        super().__init__()
+        self.idx = np.uint32(idx)
+        for k in range(3):
+            self.p[S.RS_T + k] = np.float32(target[k])
+        self.E0 = np.float32(E0)
+        self.dE = np.float32(dE)
+        self.focus_r = np.float32(focus_r)
+        self.p[S.RS_E0], self.p[S.RS_DE], self.p[S.RS_FOCUS_R] = self.E0, self.dE, self.focus_r
+        self._events = tuple(np.zeros(0, np.float64) for _ in range(5))
+
+    def _collect(self, lib, plan, slot):
+        import ctypes as C
+        cnt = C.c_uint64(0)
+        capi.check(lib.ramp_rescal_read(plan, slot, None, 0, C.byref(cnt)))
+        n = cnt.value
+        ev = np.zeros((5, max(n, 1)), np.float32)
+        if n:
+            capi.check(lib.ramp_rescal_read(plan, slot, ev.ctypes.data_as(C.POINTER(C.c_float)), n,
+                                            C.byref(cnt)))
+        qx, qy, qz, dE, w = (ev[k, :n].astype(np.float64) for k in range(5))
+        self._events = (qx, qy, qz, dE, w)
+
+    def data(self, queue=None):
+        return self._events
 
 
 class SMCPLIn(_Scat):
@@ -1107,4 +1142,4 @@ SOURCES = ("MFlat", "MCont", "MISIS", "MLambda", "MParallel")
 GEOMS = ("GArm", "GPlane", "GBox", "GSphere", "GSphereInt", "GCylinderExt", "GBanana", "GMesh")
 SCATS = ("SArm", "SBeamstop", "SChopper", "SGuide", "SIsotropic", "SPowder1", "SPowderN",
          "SMonochromator", "SDetector1D", "SDetector2D", "SCounter", "SIncoherent", "SAbsorber",
-         "SDetectorTubes", "SPowderDiff", "SMCPLIn", "SMCPLOut")
+         "SDetectorTubes", "SPowderDiff", "SMCPLIn", "SMCPLOut", "SRescal")

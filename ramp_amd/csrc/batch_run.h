@@ -57,6 +57,8 @@ void k_batch(BatchArgs b)
     hc.events = hc.oob = hc.overflow = 0;
     hc.dets = nullptr;
     hc.mode = 0;
+    hc.acc_f = nullptr;  /* settings change per tile: global atomics */
+    hc.acc_q = nullptr;
     const uint32_t lane = __lane_id();
     DeferQ dq;
     dq.b = batch_lds;
@@ -162,8 +164,8 @@ extern "C" int ramp_batch_create(ramp_ctx *ctx, ramp_plan *const *plans, uint32_
     for (uint32_t s = 0; s < n_settings; s++) {
         if (!plans[s]) return fail(RAMP_E_INVALID, "setting %u: null plan", s);
         if (plans[s]->ctx != ctx) return fail(RAMP_E_INVALID, "setting %u: plan of another context", s);
-        if (plans[s]->has_mcpl)
-            return fail(RAMP_E_INVALID, "setting %u: SMCPLIn/SMCPLOut plans run with ramp_run only", s);
+        if (plans[s]->need_full)
+            return fail(RAMP_E_INVALID, "setting %u: SMCPLIn/SMCPLOut/SRescal plans run with ramp_run only", s);
         for (uint32_t r = 0; r < s; r++)
             if (plans[r] == plans[s]) return fail(RAMP_E_INVALID, "settings %u and %u share a plan", r, s);
         feat |= plans[s]->feat;

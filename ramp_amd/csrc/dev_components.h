@@ -443,6 +443,9 @@ struct HistCtx {
     int mode;       /* RAMP_HIST_* */
     int enabled;    /* 0 in pilot runs */
     uint32_t events, oob, overflow;
+    /* LDS accumulators of the single-bin detectors (null: global atomics) */
+    float *acc_f;                 /* [DET_ACC_MAX][2] atomic mode */
+    unsigned long long *acc_q;    /* [DET_ACC_MAX][2] deterministic mode */
 };
 
 DEV void hist_add(HistCtx &hc, int det, uint64_t idx, float w)
@@ -461,13 +464,50 @@ DEV void hist_add(HistCtx &hc, int det, uint64_t idx, float w)
             hc.overflow++;
             return;
         }
-        atomicAdd(&d.qw[idx], (unsigned long long)(long long)a);
-        atomicAdd(&d.qw2[idx], (unsigned long long)(long long)b);
+        if (d.acc >= 0 && hc.acc_q) {
+            atomicAdd(&hc.acc_q[2 * d.acc], (unsigned long long)(long long)a);
+            atomicAdd(&hc.acc_q[2 * d.acc + 1], (unsigned long long)(long long)b);
+        } else {
+            atomicAdd(&d.qw[idx], (unsigned long long)(long long)a);
+            atomicAdd(&d.qw2[idx], (unsigned long long)(long long)b);
+        }
+    } else if (d.acc >= 0 && hc.acc_f) {
+        atomicAdd(&hc.acc_f[2 * d.acc], w);
+        atomicAdd(&hc.acc_f[2 * d.acc + 1], w2);
     } else {
         unsafeAtomicAdd(&d.w[idx], w);
         unsafeAtomicAdd(&d.w2[idx], w2);
     }
     hc.events++;
+}
+
+/* start and end of a workgroup's LDS accumulation (all threads call both;
+ * the end adds each accumulator to its detector's only bin) */
+DEV void hist_acc_begin(HistCtx &hc, float *f, unsigned long long *q)
+{
+    if (threadIdx.x < 2 * DET_ACC_MAX) {
+        f[threadIdx.x] = 0.0f;
+        q[threadIdx.x] = 0ull;
+    }
+    __syncthreads();
+    hc.acc_f = f;
+    hc.acc_q = q;
+}
+
+DEV void hist_acc_end(const HistCtx &hc, int n_det)
+{
+    __syncthreads();
+    const uint32_t t = threadIdx.x;
+    if (t >= 2 * DET_ACC_MAX) return;
+    for (int k = 0; k < n_det; k++) {
+        const DDet &d = hc.dets[k];
+        if (d.acc < 0 || (uint32_t)d.acc != t / 2) continue;
+        if (hc.mode == RAMP_HIST_DETERMINISTIC) {
+            if (hc.acc_q[t]) atomicAdd(t & 1 ? d.qw2 : d.qw, hc.acc_q[t]);
+        } else if (hc.acc_f[t] != 0.0f) {
+            unsafeAtomicAdd(t & 1 ? d.w2 : d.w, hc.acc_f[t]);
+        }
+    }
 }
 
 #include "dev_adjoint.h"
@@ -800,6 +840,10 @@ DEV void scat_dispatch(NS &n, const DComp &c, HC &hc)
     case RAMP_S_DETPIX: s_detpix(n, c, hc); break;
     case RAMP_S_POWDERDIFF:
         if constexpr ((F & FEAT_SAMPLE) != 0) s_powderdiff(n, c, hc);
+        break;
+    case RAMP_S_RESCAL:
+        /* plans holding SRescal run in full-state mode only */
+        if constexpr (FULL) s_rescal(n, c, hc);
         break;
     default: break;
     }

@@ -1,4 +1,5 @@
-/* dev_mcpl.h - the MCPL particle components as device steps.
+/* dev_mcpl.h - the MCPL particle components as device steps, and SRescal,
+ * whose event list uses the same tag-and-compact scheme.
  *
  * SMCPLIn (scat/mcpl_in.py:9-76).  The reference overwrites the whole
  * neutron buffer of a chunk on the host: slot i gets particle i of the file
@@ -176,6 +177,42 @@ DEV void mcpl_out_step(const NS &n, const DComp &c, float *dump, uint32_t *tags,
     float *o = dump + (size_t)s * MCR_N * cap + n.key;
     for (int k = 0; k < MCR_N; k++) o[(size_t)k * cap] = rec[k];
     tags[s * cap + n.key] = tag;
+}
+/* SRescal (scat/rescal.cl:5-77): a scattering point on the path through
+ * the sample, a final direction in the cone of half-angle acos(d[0])
+ * about the target (rotations about the unnormalised axes d[1..3] =
+ * target x target_perp and the target itself, as the reference does), a
+ * final energy uniform in E0 +- dE; the event (ki - kf, energy transfer,
+ * weight) goes field-major to the component's buffer (DPart.aux) at the
+ * history's index.  phi = rand*2*M_PI promotes to double (rescal.cl:40). */
+template <class HC>
+DEV void s_rescal(NS &n, const DComp &c, HC &hc)
+{
+    const DPart &s = c.s;
+    rf3 p1 = ipos1(n);
+    rf3 path = v3_sub(ipos2(n), p1);
+    setpos(n, v3_add(p1, v3_scale(drw(n), path)));
+    rf3 ki = v3_scale(RC_V2K, nvel(n));
+    float theta = rm_acos(1.0f - drw(n) * (1.0f - s.d[0]));
+    float phi = (float)((double)(drw(n) * 2.0f) * RC_M_PI_D);
+    rf3 tgt = rf3_make(s.p[RS_T], s.p[RS_T + 1], s.p[RS_T + 2]);
+    rf3 tf = v3_rotate_axis(theta, rf3_make(s.d[1], s.d[2], s.d[3]), tgt);
+    tf = v3_norm(v3_rotate_axis(phi, tgt, tf));
+    float Ef = s.p[RS_E0] + s.p[RS_DE] * (1.0f - 2.0f * drw(n));
+    float vf = RC_SE2V * rm_sqrt(Ef);
+    setvel(n, v3_mulv(tf, vf));
+    rf3 kf = v3_scale(RC_V2K, nvel(n));
+    float dE = RC_VS2E * rm_pow(v3_len(v3_scale(RC_K2V, ki)), 2.0f) - Ef;
+    if (hc.enabled) {
+        const uint64_t cap = (uint64_t)s.ni[RSI_CAP_LO] | ((uint64_t)s.ni[RSI_CAP_HI] << 32);
+        float *e = (float *)s.aux + n.key;
+        e[0] = ki.x - kf.x;
+        e[cap] = ki.y - kf.y;
+        e[2 * cap] = ki.z - kf.z;
+        e[3 * cap] = dE;
+        e[4 * cap] = n.w;
+    }
+    s_done(n, c.comp_idx);
 }
 #endif
 

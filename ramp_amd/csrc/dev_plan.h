@@ -53,7 +53,14 @@ struct DDet {
     uint64_t n;
     int32_t sh_w;                /* fixed-point exponents: q = rint(w * 2^sh) */
     int32_t sh_w2;
+    int32_t acc;                 /* single-bin detector: its LDS accumulator slot (0..3), else -1 */
+    int32_t pad;
 };
+
+/* single-bin detectors (an SCounter: every event hits the same address)
+ * accumulate per workgroup in LDS and flush one global atomic per
+ * workgroup; at most this many per plan */
+#define DET_ACC_MAX 4
 
 /* SoA neutron queue */
 struct DQueue {

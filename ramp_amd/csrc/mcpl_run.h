@@ -1,5 +1,6 @@
 /* mcpl_run.h - ramp_mcpl_read: the records of one SMCPLOut component,
- * compacted on the device in history order.
+ * compacted on the device in history order; and the per-chunk append of
+ * SRescal's detected events to their device-resident lists.
  *
  * The reference copies the whole neutron buffer of the chunk to the host
  * and filters it particle by particle in Python (mcpl_out.py:99-107).
@@ -8,10 +9,18 @@
  * order: per-wave counts over contiguous ranges, one exclusive scan, and a
  * second pass in which every wave writes its records at its offset (skipped
  * when the caller only asks for the count).  Only the compacted records
- * cross PCIe.  Included by ramp_gpu.hip.
+ * cross PCIe.  Included by ramp_gpu.hip ahead of ramp_run.
  */
 
-#define SEL_WAVES_MAX 16384u
+/* waves of a compaction pass: one per SEL_PER_WAVE candidates, so that
+ * several rounds of waves keep HBM busy; the scan takes up to 2^20 counts */
+#define SEL_PER_WAVE 1024u
+#define SEL_WAVES_MAX (1u << 20)
+
+static uint32_t sel_waves(uint64_t n)
+{
+    return (uint32_t)std::min<uint64_t>(SEL_WAVES_MAX, std::max<uint64_t>(1, (n + SEL_PER_WAVE - 1) / SEL_PER_WAVE));
+}
 
 /* waves over the n tags of one slot; each takes a contiguous range */
 DEV void sel_range(uint64_t n, uint32_t G, uint32_t g, uint64_t &b, uint64_t &e)
@@ -94,6 +103,125 @@ __global__ __launch_bounds__(256) void k_sel_write(const uint32_t *tags, const f
     }
 }
 
+/* SRescal: after each chunk, the events of its histories tagged as
+ * detected are appended, in history order, to the slot's field-major list
+ * [5][list_cap] at the device-held count *list_n (rescal.py:51-62). */
+__global__ __launch_bounds__(256) void k_ev_append(const uint32_t *tags, const float *ev, uint64_t n,
+                                                   uint64_t cap, uint32_t tag, uint32_t G,
+                                                   const uint64_t *prefix, float *list, uint64_t list_cap,
+                                                   const unsigned long long *list_n)
+{
+    const uint32_t g = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (g >= G) return;
+    uint64_t b, e;
+    sel_range(n, G, g, b, e);
+    uint64_t o = *list_n + prefix[g];
+    const uint32_t lane = __lane_id();
+    for (uint64_t base = b; base < e; base += 64) {
+        const uint64_t i = base + lane;
+        const bool hit = i < e && tags[i] == tag;
+        const unsigned long long m = __ballot(hit);
+        if (hit) {
+            const uint64_t d = o + (uint64_t)__popcll(m & ((1ull << lane) - 1ull));
+            for (int f = 0; f < RS_NEV; f++) list[(size_t)f * list_cap + d] = ev[(size_t)f * cap + i];
+        }
+        o += (uint64_t)__popcll(m);
+    }
+}
+
+__global__ void k_ev_advance(unsigned long long *list_n, const uint64_t *total)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *list_n += *total;
+}
+
+/* the append of one chunk's events, for every SRescal slot (stream order) */
+static int rescal_append(ramp_plan *p, uint64_t n, uint32_t tag, hipStream_t s)
+{
+    const uint32_t G = sel_waves(n);
+    int rc = dev_alloc(p->sel_counts, G * sizeof(uint32_t));
+    if (!rc) rc = dev_alloc(p->sel_prefix, (G + 1) * sizeof(uint64_t));
+    if (rc) return rc;
+    const unsigned blocks = (G + 3) / 4;
+    for (uint32_t sl = 0; sl < p->n_rescal; sl++) {
+        const uint32_t *tags = (const uint32_t *)p->ev_tag.p + (size_t)sl * p->ev_cap;
+        const float *ev = (const float *)p->ev.p + (size_t)sl * RS_NEV * p->ev_cap;
+        float *list = (float *)p->ev_list.p + (size_t)sl * RS_NEV * p->ev_list_cap;
+        unsigned long long *ln = (unsigned long long *)p->ev_list_n.p + sl;
+        hipLaunchKernelGGL(k_sel_count, dim3(blocks), dim3(256), 0, s, tags, n, tag, G,
+                           (uint32_t *)p->sel_counts.p);
+        hipLaunchKernelGGL(k_sel_scan, dim3(1), dim3(1024), 0, s, (const uint32_t *)p->sel_counts.p, G,
+                           (uint64_t *)p->sel_prefix.p);
+        hipLaunchKernelGGL(k_ev_append, dim3(blocks), dim3(256), 0, s, tags, ev, n, p->ev_cap, tag, G,
+                           (const uint64_t *)p->sel_prefix.p, list, p->ev_list_cap, ln);
+        hipLaunchKernelGGL(k_ev_advance, dim3(1), dim3(64), 0, s, ln, (const uint64_t *)p->sel_prefix.p + G);
+        HIPCHK(hipGetLastError());
+    }
+    return 0;
+}
+
+/* event buffers of a run: per-chunk events and tags (stride `chunk`), and
+ * lists with room for every history of this run on top of what they hold */
+static int rescal_prepare(ramp_plan *p, uint64_t chunk, uint64_t n_hist, hipStream_t s)
+{
+    const uint32_t ns = p->n_rescal;
+    if (p->ev_cap < chunk) {
+        int rc = dev_alloc(p->ev, (size_t)ns * RS_NEV * chunk * sizeof(float));
+        if (!rc) rc = dev_alloc(p->ev_tag, (size_t)ns * chunk * sizeof(uint32_t));
+        if (rc) return rc;
+        launch_fill_u32(s, p->ev_tag.p, (uint64_t)ns * chunk);
+        p->ev_cap = chunk;
+        for (uint32_t sl = 0; sl < ns; sl++) {
+            DPart &d = p->hcomps[p->rescal_comp[sl]].s;
+            d.aux = (const float *)p->ev.p + (size_t)sl * RS_NEV * chunk;
+            d.ni[RSI_CAP_LO] = (uint32_t)chunk;
+            d.ni[RSI_CAP_HI] = (uint32_t)(chunk >> 32);
+        }
+        HIPCHK(hipMemcpyAsync(p->comps.p, p->hcomps.data(), p->hcomps.size() * sizeof(DComp),
+                              hipMemcpyHostToDevice, s));
+        dev_free(p->comps_brute); /* rebuilt from hcomps when asked for */
+    }
+    if (!p->ev_list_n.p) {
+        int rc = dev_alloc(p->ev_list_n, ns * sizeof(unsigned long long));
+        if (rc) return rc;
+        launch_fill_u64(s, p->ev_list_n.p, ns);
+    }
+    uint64_t need = n_hist;
+    for (uint64_t c : p->ev_count) need = std::max(need, c + n_hist);
+    if (p->ev_list_cap < need) {
+        const uint64_t cap = std::max(need, 2 * p->ev_list_cap);
+        DevBuf nb;
+        int rc = dev_alloc(nb, (size_t)ns * RS_NEV * cap * sizeof(float));
+        if (rc) return rc;
+        for (uint32_t sl = 0; sl < ns; sl++)
+            for (int f = 0; f < RS_NEV && p->ev_count[sl]; f++)
+                HIPCHK(hipMemcpyAsync((float *)nb.p + ((size_t)sl * RS_NEV + f) * cap,
+                                      (const float *)p->ev_list.p + ((size_t)sl * RS_NEV + f) * p->ev_list_cap,
+                                      p->ev_count[sl] * sizeof(float), hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+        dev_free(p->ev_list);
+        p->ev_list = nb;
+        nb.p = nullptr;
+        p->ev_list_cap = cap;
+    }
+    return 0;
+}
+
+extern "C" int ramp_rescal_read(ramp_plan *p, uint32_t comp, float *events, uint64_t cap, uint64_t *count)
+{
+    if (!p || !count) return fail(RAMP_E_INVALID, "null argument");
+    if (comp >= p->n_rescal) return fail(RAMP_E_INVALID, "SRescal %u of %u", comp, p->n_rescal);
+    if (cap && !events) return fail(RAMP_E_INVALID, "null event buffer");
+    HIPCHK(hipSetDevice(p->ctx->device));
+    HIPCHK(hipStreamSynchronize(p->ctx->stream));
+    *count = p->ev_count[comp];
+    const uint64_t nc = std::min(cap, *count);
+    for (int f = 0; f < RS_NEV && nc; f++)
+        HIPCHK(hipMemcpy(events + (size_t)f * cap,
+                         (const float *)p->ev_list.p + ((size_t)comp * RS_NEV + f) * p->ev_list_cap,
+                         nc * sizeof(float), hipMemcpyDeviceToHost));
+    return 0;
+}
+
 extern "C" int ramp_mcpl_read(ramp_plan *p, uint32_t comp, float *records, uint64_t cap, uint64_t *count)
 {
     if (!p || !count) return fail(RAMP_E_INVALID, "null argument");
@@ -104,7 +232,7 @@ extern "C" int ramp_mcpl_read(ramp_plan *p, uint32_t comp, float *records, uint6
     HIPCHK(hipSetDevice(p->ctx->device));
     hipStream_t s = p->ctx->stream;
     const uint64_t n = p->dump_n;
-    const uint32_t G = (uint32_t)std::min<uint64_t>(SEL_WAVES_MAX, std::max<uint64_t>(1, (n + 4095) / 4096));
+    const uint32_t G = sel_waves(n);
     int rc = dev_alloc(p->sel_counts, G * sizeof(uint32_t));
     if (!rc) rc = dev_alloc(p->sel_prefix, (G + 1) * sizeof(uint64_t));
     if (rc) return rc;

@@ -94,6 +94,12 @@ struct SegArgs {
     uint64_t dump_cap;
     uint32_t chunk_tag;
     int revive_k;
+    /* SRescal: a finished neutron with se > 0 tags its events [slot][ev_cap] */
+    uint32_t *ev_tag;
+    uint64_t ev_cap;
+    uint32_t n_rescal;
+    /* single-bin detectors accumulated in LDS (DDet.acc >= 0), detector count */
+    int n_acc, n_dets;
 };
 
 /* status word of the queue: iidx in bits 0..23, bit 24: intersection is all
@@ -321,6 +327,8 @@ DEV void defer_finish(NS &r, const SegArgs &a, HistCtx &hc, int k1, DeferQ &dq)
 template <bool FULL>
 DEV void finalize(const NS &n, const SegArgs &a)
 {
+    if (FULL && a.n_rescal && n.se > 0.0f)
+        for (uint32_t s = 0; s < a.n_rescal; s++) a.ev_tag[s * a.ev_cap + n.key] = a.chunk_tag;
     if (a.carry) {
         a.slot_ctr[n.key] = n.ctr;
         a.slot_iidx[n.key] = n.iidx;
@@ -387,11 +395,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SEG_MIN_WAV
 void k_segment(SegArgs a)
 {
     extern __shared__ float seg_lds[];
+    __shared__ float acc_f[2 * DET_ACC_MAX];
+    __shared__ unsigned long long acc_q[2 * DET_ACC_MAX];
     HistCtx hc;
     hc.dets = a.dets;
     hc.mode = a.hist_mode;
     hc.enabled = !a.pilot;
     hc.events = hc.oob = hc.overflow = 0;
+    hc.acc_f = nullptr;
+    hc.acc_q = nullptr;
+    if (a.n_acc) hist_acc_begin(hc, acc_f, acc_q);
     const uint32_t lane = __lane_id();
     const uint64_t G = (uint64_t)gridDim.x * (blockDim.x >> 6);
     const uint64_t g = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -457,6 +470,7 @@ void k_segment(SegArgs a)
         }
     }
     if (!a.last && lane == 0) a.out_counts[g] = wcount;
+    if (a.n_acc) hist_acc_end(hc, a.n_dets);
     uint32_t ev = wave_sum(hc.events), oob = wave_sum(hc.oob), ovf = wave_sum(hc.overflow);
     if (lane == 0) {
         if (ev) atomicAdd(&a.cnt[CNT_EVENTS], (unsigned long long)ev);
@@ -588,7 +602,7 @@ struct ramp_ctx {
     int device;
     int n_cu;
     hipStream_t stream;
-    hipEvent_t ev[4];
+    hipEvent_t ev[6];
 };
 
 struct DevBuf {
@@ -647,7 +661,14 @@ struct ramp_plan {
     std::vector<int> bounds;  /* segment boundaries (step indices) */
     int defer_k = -1;         /* first linear monochromator step (reflections deferred) */
     int revive_k = -1;        /* last SMCPLIn step: segment 0 runs at least through it */
-    bool has_mcpl = false;    /* SMCPLIn/SMCPLOut present: full-state execution */
+    bool has_mcpl = false;    /* SMCPLIn/SMCPLOut present */
+    bool need_full = false;   /* MCPL or SRescal present: full-state execution */
+    uint32_t n_rescal = 0;    /* SRescal components (event slots) */
+    int n_acc = 0;            /* single-bin detectors accumulated in LDS */
+    std::vector<uint32_t> rescal_comp; /* their component numbers */
+    DevBuf ev, ev_tag, ev_list, ev_list_n; /* per-chunk events, tags; accumulated lists [slot][5][list_cap], counts */
+    uint64_t ev_cap = 0, ev_list_cap = 0;
+    std::vector<uint64_t> ev_count; /* host copy of the list counts after the last run */
     uint32_t n_mcpl_out = 0;  /* SMCPLOut components (dump slots) */
     DevBuf dump, dump_tag, sel_counts, sel_prefix, sel_out; /* SMCPLOut records and their compaction */
     uint64_t dump_cap = 0;
@@ -673,7 +694,7 @@ extern "C" int ramp_ctx_create(int device, ramp_ctx **out)
     HIPCHK(hipGetDeviceProperties(&prop, device));
     c->n_cu = prop.multiProcessorCount;
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    for (int i = 0; i < 4; i++) HIPCHK(hipEventCreate(&c->ev[i]));
+    for (int i = 0; i < 6; i++) HIPCHK(hipEventCreate(&c->ev[i]));
     *out = c;
     return 0;
 }
@@ -683,7 +704,7 @@ extern "C" int ramp_ctx_destroy(ramp_ctx *ctx)
     if (!ctx) return 0;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    for (int i = 0; i < 4; i++) (void)hipEventDestroy(ctx->ev[i]);
+    for (int i = 0; i < 6; i++) (void)hipEventDestroy(ctx->ev[i]);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return 0;
@@ -829,6 +850,18 @@ static void derive_scat(DPart &s)
     case RAMP_S_DET2D:
         set_df(s.d, 0, 2 * RC_M_PI_D);
         break;
+    case RAMP_S_RESCAL: {
+        /* rescal.cl:36-50, the parameter-only part: cos of the cone
+         * half-angle and target x target_perp */
+        rf3 t = rf3_make(s.p[RS_T], s.p[RS_T + 1], s.p[RS_T + 2]);
+        float fr = s.p[RS_FOCUS_R];
+        float l2 = v3_len(t) * v3_len(t);
+        s.d[0] = rm_sqrt(l2 / (fr * fr + l2));
+        rf3 tp = (t.x == 0.0f && t.z == 0.0f) ? rf3_make(1.0f, 0.0f, 0.0f) : rf3_make(-t.z, 0.0f, t.x);
+        rf3 tpp = v3_cross(t, tp);
+        s.d[1] = tpp.x; s.d[2] = tpp.y; s.d[3] = tpp.z;
+        break;
+    }
     default: break;
     }
 }
@@ -953,12 +986,18 @@ static int validate_plan(const ramp_plan_desc *d)
             for (uint32_t k = bl.first; k < bl.first + bl.count; k++)
                 if (d->comps[k].scat.type == RAMP_S_MCPL_IN || d->comps[k].scat.type == RAMP_S_MCPL_OUT)
                     return fail(RAMP_E_INVALID, "component %u: SMCPLIn/SMCPLOut must be in a linear block", k);
+        /* the event filter reads the detected flag at the end of the block
+         * (rescal.py:51-55 is called when its block ends) */
+        if (b + 1 < d->n_blocks)
+            for (uint32_t k = bl.first; k < bl.first + bl.count; k++)
+                if (d->comps[k].scat.type == RAMP_S_RESCAL)
+                    return fail(RAMP_E_INVALID, "component %u: SRescal must be in the last block", k);
     }
     for (uint32_t k = 0; k < d->n_comps; k++) {
         const ramp_component_desc &c = d->comps[k];
         if (c.geom.type < RAMP_G_ARM || c.geom.type > RAMP_G_MESH)
             return fail(RAMP_E_INVALID, "component %u: unknown geometry type %u", k, c.geom.type);
-        if (c.scat.type < RAMP_S_ARM || c.scat.type > RAMP_S_MCPL_OUT)
+        if (c.scat.type < RAMP_S_ARM || c.scat.type > RAMP_S_RESCAL)
             return fail(RAMP_E_INVALID, "component %u: unknown scatter type %u", k, c.scat.type);
         if (c.comp_idx == 0 || c.comp_idx > 0x00ffffffu)
             return fail(RAMP_E_INVALID, "component %u: index %u out of range", k, c.comp_idx);
@@ -1052,6 +1091,10 @@ extern "C" int ramp_plan_create(ramp_ctx *ctx, const ramp_plan_desc *d, ramp_pla
         dc.pad = (c.rot[0] == 0.0f && c.rot[1] == 0.0f && c.rot[2] == 0.0f) ? 1u : 0u;
         if (c.scat.type == RAMP_S_MCPL_OUT) dc.s.ni[MCO_SLOT] = p->n_mcpl_out++;
         if (c.scat.type == RAMP_S_MCPL_IN || c.scat.type == RAMP_S_MCPL_OUT) p->has_mcpl = true;
+        if (c.scat.type == RAMP_S_RESCAL) {
+            dc.s.ni[RSI_SLOT] = p->n_rescal++;
+            p->rescal_comp.push_back(k);
+        }
     }
     uint32_t feat = 0;
     if (d->source.type == RAMP_SRC_ISIS) feat |= FEAT_ISIS;
@@ -1087,6 +1130,8 @@ extern "C" int ramp_plan_create(ramp_ctx *ctx, const ramp_plan_desc *d, ramp_pla
     for (size_t k = 0; k < p->hsteps.size(); k++)
         if (p->hsteps[k].kind == 0 && p->hcomps[p->hsteps[k].first].s.type == RAMP_S_MCPL_IN)
             p->revive_k = (int)k;
+    p->need_full = p->has_mcpl || p->n_rescal > 0;
+    p->ev_count.assign(p->n_rescal, 0);
     p->det_desc.assign(d->dets, d->dets + d->n_dets);
     p->hdets.resize(d->n_dets);
     p->hw.resize(d->n_dets);
@@ -1096,6 +1141,8 @@ extern "C" int ramp_plan_create(ramp_ctx *ctx, const ramp_plan_desc *d, ramp_pla
     for (uint32_t k = 0; k < d->n_dets; k++) {
         std::memset(&p->hdets[k], 0, sizeof(DDet));
         p->hdets[k].n = d->dets[k].n_bins;
+        p->hdets[k].acc = -1;
+        if (d->dets[k].n_bins == 1 && p->n_acc < DET_ACC_MAX) p->hdets[k].acc = p->n_acc++;
     }
     PCHK(dev_alloc(p->comps, std::max<size_t>(1, p->hcomps.size()) * sizeof(DComp)));
     PCHK(dev_alloc(p->steps, std::max<size_t>(1, p->hsteps.size()) * sizeof(DStep)));
@@ -1123,7 +1170,8 @@ extern "C" int ramp_plan_destroy(ramp_plan *p)
     DevBuf *single[] = {&p->comps, &p->steps, &p->src, &p->dets, &p->tables, &p->cnt, &p->count,
                         &p->slot_ctr, &p->slot_iidx, &p->st16, &p->st8, &p->stii,
                         &p->rcounts, &p->prefix, &p->comps_brute, &p->dump, &p->dump_tag,
-                        &p->sel_counts, &p->sel_prefix, &p->sel_out};
+                        &p->sel_counts, &p->sel_prefix, &p->sel_out, &p->ev, &p->ev_tag,
+                        &p->ev_list, &p->ev_list_n};
     for (DevBuf *b : single) dev_free(*b);
     for (auto *v : {&p->hw, &p->hw2, &p->hq, &p->hq2, &p->mesh_nodes})
         for (auto &b : *v) dev_free(b);
@@ -1305,7 +1353,7 @@ static int choose_bounds(ramp_plan *p, const SegArgs &base, uint64_t chunk, unsi
         a.num_sim = (int)pn;
         launch_fill_u64(s, p->cnt.p, CNT_N);
         unsigned grid = (unsigned)std::max<uint64_t>(1, (pn + wg - 1) / wg);
-        launch_segment(a, p->has_mcpl, true, p->variant, grid, wg, s);
+        launch_segment(a, p->need_full, true, p->variant, grid, wg, s);
         HIPCHK(hipGetLastError());
         std::vector<unsigned long long> cnt(CNT_N);
         HIPCHK(hipMemcpyAsync(cnt.data(), p->cnt.p, CNT_N * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
@@ -1325,6 +1373,8 @@ static int choose_bounds(ramp_plan *p, const SegArgs &base, uint64_t chunk, unsi
     p->bounds_set = true;
     return 0;
 }
+
+#include "mcpl_run.h"
 
 extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uint32_t *seeds,
                         uint32_t n_seeds, const ramp_run_opts *opts)
@@ -1351,8 +1401,9 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
     unsigned wg = o.wg ? o.wg : 256;
     if (wg != 64 && wg != 128 && wg != 256) return fail(RAMP_E_INVALID, "workgroup size %u", wg);
     const bool keep = o.keep_state != 0;
-    /* the MCPL steps need the polarisation and the whole intersection */
-    const bool full = keep || p->has_mcpl;
+    /* the MCPL steps need the polarisation and the whole intersection,
+     * SRescal's filter the detected flag */
+    const bool full = keep || p->need_full;
     int rc = prepare_dets(p, (int)o.hist_mode, o.weight_bound, n_hist, chunk);
     if (rc) return rc;
     bool carry = n_chunks > 1;
@@ -1377,6 +1428,10 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
         launch_fill_u32(s, (uint32_t *)p->dump_tag.p, nslot);
         p->dump_cap = chunk;
         p->dump_chunk_tag = 0;
+    }
+    if (p->n_rescal) {
+        rc = rescal_prepare(p, chunk, n_hist, s);
+        if (rc) return rc;
     }
     SegArgs a;
     std::memset(&a, 0, sizeof a);
@@ -1404,6 +1459,11 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
     a.dump = (float *)p->dump.p;
     a.dump_tag = (uint32_t *)p->dump_tag.p;
     a.dump_cap = p->dump_cap;
+    a.ev_tag = (uint32_t *)p->ev_tag.p;
+    a.ev_cap = p->ev_cap;
+    a.n_rescal = p->n_rescal;
+    a.n_acc = p->n_acc;
+    a.n_dets = (int)p->hdets.size();
     a.slot_ctr = (float *)p->slot_ctr.p;
     a.slot_iidx = (uint32_t *)p->slot_iidx.p;
     a.st16 = (float *)p->st16.p;
@@ -1438,6 +1498,7 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
         a.seed = (int)seeds[ck];
         a.n = torun;
         a.chunk_tag = ++p->dump_chunk_tag ? p->dump_chunk_tag : ++p->dump_chunk_tag;
+        if (p->n_rescal) launch_fill_f32(s, p->ev.p, (uint64_t)p->n_rescal * RS_NEV * p->ev_cap);
         for (int sg = 0; sg < nseg; sg++) {
             a.s0 = p->bounds[sg];
             a.s1 = p->bounds[sg + 1];
@@ -1461,6 +1522,12 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
                 HIPCHK(hipGetLastError());
             }
         }
+        if (p->n_rescal) {
+            if (ck == 0) HIPCHK(hipEventRecord(ctx->ev[4], s));
+            rc = rescal_append(p, torun, a.chunk_tag, s);
+            if (rc) return rc;
+            if (ck == 0) HIPCHK(hipEventRecord(ctx->ev[5], s));
+        }
     }
     HIPCHK(hipEventRecord(ctx->ev[1], s));
     HIPCHK(hipEventSynchronize(ctx->ev[1]));
@@ -1480,6 +1547,12 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
     p->stats.state_in = cnt[CNT_QIN];
     p->stats.ms_total = ms;
     p->stats.ms_source_segment = ms0;
+    float ms_ev = 0;
+    if (p->n_rescal) HIPCHK(hipEventElapsedTime(&ms_ev, ctx->ev[4], ctx->ev[5]));
+    p->stats.ms_rescal_append = ms_ev;
+    if (p->n_rescal)
+        HIPCHK(hipMemcpy(p->ev_count.data(), p->ev_list_n.p, p->n_rescal * sizeof(uint64_t),
+                         hipMemcpyDeviceToHost));
     p->st_n = keep ? std::min(chunk, n_hist - (n_chunks - 1) * chunk) : 0;
     p->dump_n = std::min(chunk, n_hist - (n_chunks - 1) * chunk);
     return 0;
@@ -1523,6 +1596,8 @@ extern "C" int ramp_hist_reset(ramp_plan *p)
         if (p->hw[k].p) { launch_fill_f32(s, p->hw[k].p, p->hdets[k].n); launch_fill_f32(s, p->hw2[k].p, p->hdets[k].n); }
         if (p->hq[k].p) { launch_fill_u64(s, p->hq[k].p, p->hdets[k].n); launch_fill_u64(s, p->hq2[k].p, p->hdets[k].n); }
     }
+    if (p->ev_list_n.p) launch_fill_u64(s, p->ev_list_n.p, p->n_rescal);
+    std::fill(p->ev_count.begin(), p->ev_count.end(), 0);
     HIPCHK(hipStreamSynchronize(s));
     return 0;
 }
@@ -1608,4 +1683,3 @@ extern "C" int ramp_rng_eval(ramp_ctx *ctx, const uint32_t *tid, const float *ct
 
 #include "batch_run.h"
 #include "adjoint_run.h"
-#include "mcpl_run.h"

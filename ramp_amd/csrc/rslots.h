@@ -187,5 +187,14 @@
 /* SMCPLOut */
 #define MCO_SLOT 0            /* index among the plan's SMCPLOut components */
 #define MCO_STANDARD 1        /* 1: MCPL's own direction packing instead of the reference's */
+/* SRescal */
+#define RS_T 0                /* 0..2 target (sample frame) */
+#define RS_E0 3               /* meV */
+#define RS_DE 4               /* meV */
+#define RS_FOCUS_R 5          /* m */
+#define RSI_SLOT 0            /* index among the plan's SRescal components */
+#define RSI_CAP_LO 6          /* set by the library: event buffer stride (histories), */
+#define RSI_CAP_HI 7          /* low and high words; the buffer itself is DPart.aux */
+#define RS_NEV 5              /* event fields: qx qy qz dE p */
 
 #endif

@@ -40,6 +40,19 @@ RHD rf2 v2_norm(rf2 a)
     return rf2_make(a.x / l, a.y / l);
 }
 
+/* rotate_about_axis (mcramp/include/geom.h:1-19): the Rodrigues matrix of
+ * `angle` about `a`, entries formed as the reference forms them, applied
+ * to v.  The axis is used as given; the reference does not normalise it. */
+RHD rf3 v3_rotate_axis(float angle, rf3 a, rf3 v)
+{
+    float c = rm_cos(angle), s = rm_sin(angle), t = 1 - c;
+    float rxx = c + a.x * a.x * t, rxy = a.x * a.y * t - a.z * s, rxz = a.x * a.z * t + a.y * s;
+    float ryx = a.y * a.x * t + a.z * s, ryy = c + a.y * a.y * t, ryz = a.y * a.z * t - a.x * s;
+    float rzx = a.z * a.x * t - a.y * s, rzy = a.z * a.y * t + a.x * s, rzz = c + a.z * a.z * t;
+    return rf3_make((rxx * v.x + rxy * v.y) + rxz * v.z, (ryx * v.x + ryy * v.y) + ryz * v.z,
+                    (rzx * v.x + rzy * v.y) + rzz * v.z);
+}
+
 /* Euler-angle frame rotation R = Rz(g) Ry(b) Rx(a) (mcramp/include/geom.h:
  * 21-53).  The nine entries are formed exactly as the reference forms them;
  * rotate applies R, derotate applies R^T (geom.h:55-87). */

@@ -225,10 +225,14 @@ class Instrument:
         cds = (capi.ComponentDesc * max(1, len(comp_list)))()
         self._det_of = {}
         self._mcpl_out = []
+        self._rescal = []
         for c in comp_list:
             if c.scat.type_code == capi.RAMP_S_MCPL_OUT:
                 c.scat.ni[S.MCO_SLOT] = len(self._mcpl_out)
                 self._mcpl_out.append(c)
+            if c.scat.type_code == capi.RAMP_S_RESCAL:
+                c.scat.ni[S.RSI_SLOT] = len(self._rescal)
+                self._rescal.append(c)
         for k, c in enumerate(comp_list):
             d = cds[k]
             c.geom.fill(d.geom, pool)
@@ -355,6 +359,8 @@ class Instrument:
             return
         self._hist_stale = False
         lib = capi.load_library()
+        for slot, c in enumerate(getattr(self, "_rescal", [])):
+            c.scat._collect(lib, self._plan, slot)
         for k, c in enumerate(self._comp_list):
             if k in self._det_of:
                 d = self._det_of[k]

@@ -8,12 +8,15 @@ moments, detector bin placement.  A dropped term, a wrong sign or a swapped
 axis in the restated kernels moves these numbers."""
 import json
 import math
+import os
 
 import numpy as np
 import pytest
 from scipy import integrate
 
 import ramp_amd
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def _inst(tmp_path, comps, name="t.json", **kw):
@@ -328,3 +331,81 @@ def test_powderdiff_at_the_proposal_is_forced_scattering(oracle, tmp_path):
     assert hit.sum() > 1000
     # same draws for depth; one extra draw (channel) shifts the direction draws
     assert np.allclose(a[hit, 9], b[hit, 9], rtol=2e-6)
+
+
+
+def test_rotate_about_axis_closed_form():
+    """v3_rotate_axis, the shared restatement of geom.h:1-19: quarter turns
+    about the coordinate axes, lengths and the axis kept for unit axes, and
+    for a non-unit axis the matrix cI + (1-c) a a^T + s [a]x the reference
+    forms (it does not normalise)."""
+    import ctypes as C
+    from oracle import pyoracle
+    lib = pyoracle.load()
+    f = lib.oracle_rotate_axis
+    f.restype = None
+    f.argtypes = [C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]
+
+    def rot(angle, a, v):
+        a = np.ascontiguousarray(a, np.float32)
+        v = np.ascontiguousarray(v, np.float32)
+        o = np.zeros(3, np.float32)
+        f(angle, a.ctypes.data_as(C.POINTER(C.c_float)), v.ctypes.data_as(C.POINTER(C.c_float)),
+          o.ctypes.data_as(C.POINTER(C.c_float)))
+        return o.astype(np.float64)
+
+    q = np.float32(np.pi / 2)
+    assert np.allclose(rot(q, [0, 0, 1], [1, 0, 0]), [0, 1, 0], atol=1e-7)
+    assert np.allclose(rot(q, [1, 0, 0], [0, 1, 0]), [0, 0, 1], atol=1e-7)
+    assert np.allclose(rot(q, [0, 1, 0], [0, 0, 1]), [1, 0, 0], atol=1e-7)
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        a = rng.normal(size=3)
+        a /= np.linalg.norm(a)
+        v = rng.normal(size=3)
+        ang = rng.uniform(-np.pi, np.pi)
+        r = rot(ang, a, v)
+        assert abs(np.linalg.norm(r) - np.linalg.norm(v)) < 1e-5 * np.linalg.norm(v)
+        assert np.allclose(rot(ang, a, a), a, atol=1e-6)
+        b = a * rng.uniform(0.2, 3.0)  # non-unit axis
+        c, s_ = np.cos(np.float32(ang)), np.sin(np.float32(ang))
+        K = np.array([[0, -b[2], b[1]], [b[2], 0, -b[0]], [-b[1], b[0], 0]])
+        M = c * np.eye(3) + (1 - c) * np.outer(b, b) + s_ * K
+        assert np.allclose(rot(ang, b, v), M @ v, rtol=1e-5, atol=1e-5 * np.abs(M @ v).max())
+
+
+def test_rescal_cone_and_energy_closed_form():
+    """SRescal (rescal.cl:5-77) on the oracle with a unit-length target:
+    the final direction's cosine to the target is uniform on
+    [cos(atan(focus_r)), 1], the final energy is uniform on E0 +- dE, the
+    energy transfer of the kept events has the variance of U[4,6] - U[4.8,5.2],
+    and each event carries the history's weight 1/N."""
+    from scipy import stats
+    from ramp_amd import Instrument
+    from oracle import pyoracle
+    n = 100000
+    inst = Instrument(os.path.join(ROOT, "ramp_amd", "instruments", "rescal.json"), None, None,
+                      tx=0.6, tz=0.8, arm_rot=float(np.arctan2(0.6, 0.8)))
+    r = pyoracle.run(inst, n, [9], keep_state=True)
+    n16 = r["state"][0]
+    ev = r["rescal"][0].astype(np.float64)
+    det = n16[:, 14] > 0
+    assert ev.shape[1] == det.sum() > n // 2
+    # every history that scattered in the sample (prevcomp = sample 2, or the
+    # counter 4 behind it) keeps the velocity SRescal gave it
+    sc = n16[:, 12] >= 2
+    v = n16[sc, 3:6].astype(np.float64)
+    sp = np.linalg.norm(v, axis=1)
+    cosang = v @ np.array([0.6, 0.0, 0.8]) / sp
+    ctm = np.cos(np.arctan(0.05))
+    u = (cosang - ctm) / (1 - ctm)
+    assert u.min() > -1e-4 and u.max() < 1 + 1e-4
+    assert stats.kstest(np.clip(u, 0, 1), "uniform").pvalue > 1e-3
+    Ef = 5.22703725e-6 * sp ** 2
+    assert Ef.min() > 4.8 - 1e-4 and Ef.max() < 5.2 + 1e-4
+    assert stats.kstest((Ef - 4.8) / 0.4, "uniform").pvalue > 1e-3
+    dE = ev[3]
+    var = 1 / 3 + 0.16 / 12
+    assert abs(dE.mean()) < 5 * np.sqrt(var / len(dE))
+    assert abs(dE.var() / var - 1) < 0.03
+    assert np.allclose(ev[4], 1.0 / n, rtol=1e-6)
