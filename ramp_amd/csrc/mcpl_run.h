@@ -13,9 +13,11 @@
  */
 
 /* waves of a compaction pass: one per SEL_PER_WAVE candidates, so that
- * several rounds of waves keep HBM busy; the scan takes up to 2^20 counts */
-#define SEL_PER_WAVE 1024u
+ * several rounds of waves keep HBM busy; the scan takes up to 2^20 counts.
+ * A wave takes SEL_GROUP = 256 candidates per step, four per lane. */
+#define SEL_PER_WAVE 2048u
 #define SEL_WAVES_MAX (1u << 20)
+#define SEL_GROUP 256u
 
 static uint32_t sel_waves(uint64_t n)
 {
@@ -25,10 +27,38 @@ static uint32_t sel_waves(uint64_t n)
 /* waves over the n tags of one slot; each takes a contiguous range */
 DEV void sel_range(uint64_t n, uint32_t G, uint32_t g, uint64_t &b, uint64_t &e)
 {
-    const uint64_t per = ((n + G - 1) / G + 63) & ~(uint64_t)63;
+    const uint64_t per = ((n + G - 1) / G + SEL_GROUP - 1) & ~(uint64_t)(SEL_GROUP - 1);
     b = (uint64_t)g * per;
     e = b + per < n ? b + per : n;
     if (b > n) b = n;
+}
+
+/* one group [i0, i0 + 64 * PER) of a wave's range: the offsets (from i0)
+ * of its tagged entries, in order, go to src[0..cnt) in LDS; returns cnt */
+template <int PER>
+DEV uint32_t sel_group(const uint32_t *tags, uint64_t i0, uint64_t e, uint32_t tag, uint32_t *src)
+{
+    const uint32_t lane = __lane_id();
+    bool h[PER];
+    uint32_t c = 0;
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const uint64_t i = i0 + PER * lane + k;
+        h[k] = i < e && tags[i] == tag;
+        c += h[k] ? 1u : 0u;
+    }
+    uint32_t x = c; /* inclusive prefix over the lanes */
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    uint32_t r = x - c;
+#pragma unroll
+    for (int k = 0; k < PER; k++)
+        if (h[k]) src[r++] = PER * lane + k;
+    wave_lds_sync();
+    return __shfl(x, 63);
 }
 
 __global__ __launch_bounds__(256) void k_sel_count(const uint32_t *tags, uint64_t n, uint32_t tag,
@@ -39,7 +69,9 @@ __global__ __launch_bounds__(256) void k_sel_count(const uint32_t *tags, uint64_
     uint64_t b, e;
     sel_range(n, G, g, b, e);
     uint32_t c = 0;
-    for (uint64_t i = b + __lane_id(); i < e; i += 64) c += tags[i] == tag ? 1u : 0u;
+    for (uint64_t i = b + 4 * __lane_id(); i < e; i += SEL_GROUP)
+#pragma unroll
+        for (int k = 0; k < 4; k++) c += (i + k < e && tags[i + k] == tag) ? 1u : 0u;
     c = wave_sum(c);
     if (__lane_id() == 0) counts[g] = c;
 }
@@ -70,15 +102,18 @@ __global__ __launch_bounds__(1024) void k_sel_scan(const uint32_t *counts, uint3
 }
 
 /* The records sit field-major in the dump (field k of record i at
- * k * cap + i); the output is the file's record-major layout.  Per group of
- * 64 candidates the wave ranks its hits, keeps their lanes in LDS, and then
- * all 64 lanes copy the group's cnt * 11 output floats in order, so every
- * store instruction writes 256 contiguous bytes. */
+ * k * cap + i); the output is the file's record-major layout.  Per group
+ * of 128 candidates the wave ranks its hits into LDS (sel_group), gathers
+ * their fields with coalesced loads into an LDS tile in record-major
+ * order, and then streams the tile out, so that every load and store
+ * instruction touches contiguous memory. */
+#define SEL_WGROUP 128u
 __global__ __launch_bounds__(256) void k_sel_write(const uint32_t *tags, const float *recs, uint64_t n,
                                                    uint64_t cap, uint32_t tag, uint32_t G,
                                                    const uint64_t *prefix, float *out)
 {
-    __shared__ uint32_t src[4][64];
+    __shared__ uint32_t src[4][SEL_WGROUP];
+    __shared__ float tile[4][SEL_WGROUP * MCR_N];
     const uint32_t wv = threadIdx.x >> 6;
     const uint32_t g = blockIdx.x * (blockDim.x >> 6) + wv;
     if (g >= G) return;
@@ -86,18 +121,15 @@ __global__ __launch_bounds__(256) void k_sel_write(const uint32_t *tags, const f
     sel_range(n, G, g, b, e);
     uint64_t o = prefix[g];
     const uint32_t lane = __lane_id();
-    for (uint64_t base = b; base < e; base += 64) {
-        const uint64_t i = base + lane;
-        const bool hit = i < e && tags[i] == tag;
-        const unsigned long long m = __ballot(hit);
-        const uint32_t cnt = (uint32_t)__popcll(m);
-        if (hit) src[wv][__popcll(m & ((1ull << lane) - 1ull))] = lane;
+    for (uint64_t base = b; base < e; base += SEL_WGROUP) {
+        const uint32_t cnt = sel_group<2>(tags, base, e, tag, src[wv]);
+#pragma unroll
+        for (int k = 0; k < MCR_N; k++)
+            for (uint32_t r = lane; r < cnt; r += 64)
+                tile[wv][r * MCR_N + k] = recs[(size_t)k * cap + base + src[wv][r]];
         wave_lds_sync();
         float *w = out + (size_t)MCR_N * o;
-        for (uint32_t j = lane; j < cnt * MCR_N; j += 64) {
-            const uint32_t r = j / MCR_N, k = j - r * MCR_N;
-            w[j] = recs[(size_t)k * cap + base + src[wv][r]];
-        }
+        for (uint32_t j = lane; j < cnt * MCR_N; j += 64) w[j] = tile[wv][j];
         wave_lds_sync();
         o += cnt;
     }
@@ -111,21 +143,21 @@ __global__ __launch_bounds__(256) void k_ev_append(const uint32_t *tags, const f
                                                    const uint64_t *prefix, float *list, uint64_t list_cap,
                                                    const unsigned long long *list_n)
 {
-    const uint32_t g = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    __shared__ uint32_t src[4][SEL_GROUP];
+    const uint32_t wv = threadIdx.x >> 6;
+    const uint32_t g = blockIdx.x * (blockDim.x >> 6) + wv;
     if (g >= G) return;
     uint64_t b, e;
     sel_range(n, G, g, b, e);
     uint64_t o = *list_n + prefix[g];
     const uint32_t lane = __lane_id();
-    for (uint64_t base = b; base < e; base += 64) {
-        const uint64_t i = base + lane;
-        const bool hit = i < e && tags[i] == tag;
-        const unsigned long long m = __ballot(hit);
-        if (hit) {
-            const uint64_t d = o + (uint64_t)__popcll(m & ((1ull << lane) - 1ull));
-            for (int f = 0; f < RS_NEV; f++) list[(size_t)f * list_cap + d] = ev[(size_t)f * cap + i];
-        }
-        o += (uint64_t)__popcll(m);
+    for (uint64_t base = b; base < e; base += SEL_GROUP) {
+        const uint32_t cnt = sel_group<4>(tags, base, e, tag, src[wv]);
+        for (int f = 0; f < RS_NEV; f++)
+            for (uint32_t r = lane; r < cnt; r += 64)
+                list[(size_t)f * list_cap + o + r] = ev[(size_t)f * cap + base + src[wv][r]];
+        wave_lds_sync();
+        o += cnt;
     }
 }
 
