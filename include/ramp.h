@@ -73,6 +73,9 @@ extern "C" {
 #define RAMP_S_MCPL_IN 16    /* SMCPLIn        scat/mcpl_in.py:9-76 (particles from an MCPL file) */
 #define RAMP_S_MCPL_OUT 17   /* SMCPLOut       scat/mcpl_out.py:7-117 (live particles to an MCPL file) */
 #define RAMP_S_RESCAL 18     /* SRescal        scat/rescal.cl:5-77, rescal.py:9-66 (TAS resolution events) */
+#define RAMP_S_POLARISER 19  /* SPolariser     scat/polariser.cl:3-32 */
+#define RAMP_S_ANALYSER 20   /* SAnalyser      scat/analyser.cl:1-31 (with the consts.h include it lacks) */
+#define RAMP_S_COLLIMATOR 21 /* SLinearCollimator scat/collimator_lin.cl:3-55 (with the ';' it lacks at :30) */
 
 /* ---- plan descriptors --------------------------------------------------- */
 #define RAMP_NP 48  /* float parameters per component part */

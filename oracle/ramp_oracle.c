@@ -1444,6 +1444,40 @@ static void s_rescal(on_t *n, const ramp_part_desc *s, uint32_t ci)
     s_done(n, ci);
 }
 
+/* scat/polariser.cl:3-32 */
+static void s_polariser(on_t *n, const ramp_part_desc *s, uint32_t ci)
+{
+    set_pol(n, rf3_make(s->p[PO_P], s->p[PO_P + 1], s->p[PO_P + 2]));
+    s_done(n, ci);
+}
+
+/* scat/analyser.cl:1-31 (the kernel lacks its consts.h include) */
+static void s_analyser(on_t *n, const ramp_part_desc *s, uint32_t ci)
+{
+    rf3 pol = rf3_make(s->p[PO_P], s->p[PO_P + 1], s->p[PO_P + 2]);
+    n->N_P *= (1.0f + v3_dot(pol, n_pol(n))) / 2.0f;
+    set_pol(n, pol);
+    s_done(n, ci);
+}
+
+/* scat/collimator_lin.cl:3-55 (the kernel lacks a ';' at :30) */
+static void s_collimator(on_t *n, const ramp_part_desc *s, uint32_t ci)
+{
+    float slope_H = s->p[CO_SLOPE_H], slope_V = s->p[CO_SLOPE_V], transmission = s->p[CO_TRANS];
+    float phi;
+    if (slope_H > 0.0f) {
+        phi = rm_fabs(n->N_VX / n->N_VZ);
+        if (phi > slope_H) n->N_DIE = 1.0f;
+        else n->N_P *= transmission * (1.0f - phi / slope_H);
+    }
+    if (slope_V > 0.0f) {
+        phi = rm_fabs(n->N_VY / n->N_VZ);
+        if (phi > slope_V) n->N_DIE = 1.0f;
+        else n->N_P *= transmission * (1.0f - phi / slope_V);
+    }
+    s_done(n, ci);
+}
+
 static void o_scatter(octx *cx, on_t *n, const ramp_component_desc *c)
 {
     if (n->iidx != c->comp_idx) return;
@@ -1466,6 +1500,9 @@ static void o_scatter(octx *cx, on_t *n, const ramp_component_desc *c)
     case RAMP_S_DETPIX: s_detpix(cx, n, c); break;
     case RAMP_S_POWDERDIFF: s_powderdiff(n, tab(cx, s, PDT_LINES), s, c->comp_idx); break;
     case RAMP_S_RESCAL: s_rescal(n, s, c->comp_idx); break;
+    case RAMP_S_POLARISER: s_polariser(n, s, c->comp_idx); break;
+    case RAMP_S_ANALYSER: s_analyser(n, s, c->comp_idx); break;
+    case RAMP_S_COLLIMATOR: s_collimator(n, s, c->comp_idx); break;
     default: break;
     }
 }

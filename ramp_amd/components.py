@@ -1015,23 +1015,47 @@ class SDetectorTubes(_Hist):
             np.save(self.filename + "E", e)
 
 
-class _OutOfScope:
-    reason = ""
+class SPolariser(_Scat):
+    """scat/polariser.py:6-47 + polariser.cl:3-32: an ideal polariser - the
+    neutron's polarisation becomes `polarisation`."""
+    type_code = capi.RAMP_S_POLARISER
 
-    def __init__(self, *a, **k):
-        raise NotImplementedError(f"{type(self).__name__}: {self.reason}")
-
-
-class SPolariser(_OutOfScope):
-    reason = "polarisation components are not part of the MI355X neutron path"
-
-
-class SAnalyser(_OutOfScope):
-    reason = "polarisation components are not part of the MI355X neutron path"
+    def __init__(self, polarisation=(0, 0, 0), idx=0, ctx=None, filename=None, **kwargs):
+        super().__init__()
+        self.idx = np.uint32(idx)
+        for k in range(3):
+            self.p[S.PO_P + k] = np.float32(polarisation[k])
 
 
-class SCollimatorLin(_OutOfScope):
-    reason = "the reference kernel does not compile (collimator_lin.cl:30)"
+class SAnalyser(SPolariser):
+    """scat/analyser.py + analyser.cl:1-31 (which lacks its consts.h
+    include): an ideal analyser - the weight is multiplied by
+    (1 + P.pol)/2 and the polarisation becomes P.  Plans holding one run in
+    full-state mode, which tracks the polarisation."""
+    type_code = capi.RAMP_S_ANALYSER
+
+
+class SLinearCollimator(_Scat):
+    """scat/collimator_lin.py:7-63 + collimator_lin.cl:3-55 (which lacks a
+    ';' at :30): neutrons whose |vx/vz| or |vy/vz| exceeds tan(divergence)
+    (arcminutes) die, the others are weighted by transmission*(1 - phi/slope)
+    per direction."""
+    type_code = capi.RAMP_S_COLLIMATOR
+
+    def __init__(self, length=0.0, divergence_H=0.0, divergence_V=0.0, transmission=1.0, idx=0, ctx=0,
+                 **kwargs):
+        super().__init__()
+
+        def min2rad(x):
+            return x * np.pi / (60.0 * 180.0)
+
+        self.idx = np.uint32(idx)
+        self.length = np.float32(length)
+        self.slope_H = np.float32(np.tan(min2rad(divergence_H)))
+        self.slope_V = np.float32(np.tan(min2rad(divergence_V)))
+        self.transmission = np.float32(transmission)
+        self.p[S.CO_LEN], self.p[S.CO_SLOPE_H] = self.length, self.slope_H
+        self.p[S.CO_SLOPE_V], self.p[S.CO_TRANS] = self.slope_V, self.transmission
 
 
 class SRescal(_Scat):
@@ -1142,4 +1166,5 @@ SOURCES = ("MFlat", "MCont", "MISIS", "MLambda", "MParallel")
 GEOMS = ("GArm", "GPlane", "GBox", "GSphere", "GSphereInt", "GCylinderExt", "GBanana", "GMesh")
 SCATS = ("SArm", "SBeamstop", "SChopper", "SGuide", "SIsotropic", "SPowder1", "SPowderN",
          "SMonochromator", "SDetector1D", "SDetector2D", "SCounter", "SIncoherent", "SAbsorber",
-         "SDetectorTubes", "SPowderDiff", "SMCPLIn", "SMCPLOut", "SRescal")
+         "SDetectorTubes", "SPowderDiff", "SMCPLIn", "SMCPLOut", "SRescal", "SPolariser", "SAnalyser",
+         "SLinearCollimator")

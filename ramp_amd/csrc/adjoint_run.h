@@ -155,7 +155,7 @@ extern "C" int ramp_adjoint(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const
 {
     if (!p) return fail(RAMP_E_INVALID, "null plan");
     if (!dL_dh || !grad) return fail(RAMP_E_INVALID, "null dL/dh or gradient buffer");
-    if (p->need_full) return fail(RAMP_E_INVALID, "SMCPLIn/SMCPLOut/SRescal plans run with ramp_run only");
+    if (p->need_full) return fail(RAMP_E_INVALID, "SMCPLIn/SMCPLOut/SRescal/SAnalyser plans run with ramp_run only");
     ramp_run_opts o;
     std::memset(&o, 0, sizeof o);
     if (opts) o = *opts;

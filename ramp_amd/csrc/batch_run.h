@@ -165,7 +165,7 @@ extern "C" int ramp_batch_create(ramp_ctx *ctx, ramp_plan *const *plans, uint32_
         if (!plans[s]) return fail(RAMP_E_INVALID, "setting %u: null plan", s);
         if (plans[s]->ctx != ctx) return fail(RAMP_E_INVALID, "setting %u: plan of another context", s);
         if (plans[s]->need_full)
-            return fail(RAMP_E_INVALID, "setting %u: SMCPLIn/SMCPLOut/SRescal plans run with ramp_run only", s);
+            return fail(RAMP_E_INVALID, "setting %u: SMCPLIn/SMCPLOut/SRescal/SAnalyser plans run with ramp_run only", s);
         for (uint32_t r = 0; r < s; r++)
             if (plans[r] == plans[s]) return fail(RAMP_E_INVALID, "settings %u and %u share a plan", r, s);
         feat |= plans[s]->feat;

@@ -804,6 +804,41 @@ DEV void s_absorber(NS &n, const DComp &c)
 #include "dev_samples.h"
 #include "dev_mcpl.h"
 
+/* scat/polariser.cl:3-32: the polarisation becomes the parameter vector */
+DEV void s_polariser(NS &n, const DComp &c)
+{
+    n.px = c.s.p[PO_P]; n.py = c.s.p[PO_P + 1]; n.pz = c.s.p[PO_P + 2];
+    s_done(n, c.comp_idx);
+}
+
+/* scat/analyser.cl:1-31: w *= (1 + P.pol)/2, then pol = P */
+DEV void s_analyser(NS &n, const DComp &c)
+{
+    rf3 P = rf3_make(c.s.p[PO_P], c.s.p[PO_P + 1], c.s.p[PO_P + 2]);
+    n.w *= (1.0f + v3_dot(P, rf3_make(n.px, n.py, n.pz))) / 2.0f;
+    n.px = P.x; n.py = P.y; n.pz = P.z;
+    s_done(n, c.comp_idx);
+}
+
+/* scat/collimator_lin.cl:3-55: kill above the slope, else attenuate
+ * linearly; the vertical test runs even after the horizontal one killed */
+DEV void s_collimator(NS &n, const DComp &c)
+{
+    const DPart &s = c.s;
+    float sh = s.p[CO_SLOPE_H], sv = s.p[CO_SLOPE_V], tr = s.p[CO_TRANS];
+    if (sh > 0.0f) {
+        float phi = rm_fabs(n.vx / n.vz);
+        if (phi > sh) n.dead = true;
+        else n.w *= tr * (1.0f - phi / sh);
+    }
+    if (sv > 0.0f) {
+        float phi = rm_fabs(n.vy / n.vz);
+        if (phi > sv) n.dead = true;
+        else n.w *= tr * (1.0f - phi / sv);
+    }
+    s_done(n, c.comp_idx);
+}
+
 template <bool FULL, uint32_t F, class HC>
 DEV void scat_dispatch(NS &n, const DComp &c, HC &hc)
 {
@@ -845,6 +880,12 @@ DEV void scat_dispatch(NS &n, const DComp &c, HC &hc)
         /* plans holding SRescal run in full-state mode only */
         if constexpr (FULL) s_rescal(n, c, hc);
         break;
+    case RAMP_S_POLARISER: s_polariser(n, c); break;
+    case RAMP_S_ANALYSER:
+        /* plans holding an analyser run in full-state mode (it reads pol) */
+        if constexpr (FULL) s_analyser(n, c);
+        break;
+    case RAMP_S_COLLIMATOR: s_collimator(n, c); break;
     default: break;
     }
 }

@@ -662,7 +662,7 @@ struct ramp_plan {
     int defer_k = -1;         /* first linear monochromator step (reflections deferred) */
     int revive_k = -1;        /* last SMCPLIn step: segment 0 runs at least through it */
     bool has_mcpl = false;    /* SMCPLIn/SMCPLOut present */
-    bool need_full = false;   /* MCPL or SRescal present: full-state execution */
+    bool need_full = false;   /* MCPL, SRescal or SAnalyser present: full-state execution */
     uint32_t n_rescal = 0;    /* SRescal components (event slots) */
     int n_acc = 0;            /* single-bin detectors accumulated in LDS */
     std::vector<uint32_t> rescal_comp; /* their component numbers */
@@ -997,7 +997,7 @@ static int validate_plan(const ramp_plan_desc *d)
         const ramp_component_desc &c = d->comps[k];
         if (c.geom.type < RAMP_G_ARM || c.geom.type > RAMP_G_MESH)
             return fail(RAMP_E_INVALID, "component %u: unknown geometry type %u", k, c.geom.type);
-        if (c.scat.type < RAMP_S_ARM || c.scat.type > RAMP_S_RESCAL)
+        if (c.scat.type < RAMP_S_ARM || c.scat.type > RAMP_S_COLLIMATOR)
             return fail(RAMP_E_INVALID, "component %u: unknown scatter type %u", k, c.scat.type);
         if (c.comp_idx == 0 || c.comp_idx > 0x00ffffffu)
             return fail(RAMP_E_INVALID, "component %u: index %u out of range", k, c.comp_idx);
@@ -1130,7 +1130,9 @@ extern "C" int ramp_plan_create(ramp_ctx *ctx, const ramp_plan_desc *d, ramp_pla
     for (size_t k = 0; k < p->hsteps.size(); k++)
         if (p->hsteps[k].kind == 0 && p->hcomps[p->hsteps[k].first].s.type == RAMP_S_MCPL_IN)
             p->revive_k = (int)k;
-    p->need_full = p->has_mcpl || p->n_rescal > 0;
+    bool analyser = false;
+    for (const DComp &c : p->hcomps) analyser = analyser || c.s.type == RAMP_S_ANALYSER;
+    p->need_full = p->has_mcpl || p->n_rescal > 0 || analyser;
     p->ev_count.assign(p->n_rescal, 0);
     p->det_desc.assign(d->dets, d->dets + d->n_dets);
     p->hdets.resize(d->n_dets);

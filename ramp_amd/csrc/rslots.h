@@ -196,5 +196,12 @@
 #define RSI_CAP_LO 6          /* set by the library: event buffer stride (histories), */
 #define RSI_CAP_HI 7          /* low and high words; the buffer itself is DPart.aux */
 #define RS_NEV 5              /* event fields: qx qy qz dE p */
+/* SPolariser, SAnalyser */
+#define PO_P 0                /* 0..2 polarisation vector */
+/* SLinearCollimator (slopes = tan of the divergences, as collimator_lin.py forms them) */
+#define CO_LEN 0              /* m (passed, unused by the kernel) */
+#define CO_SLOPE_H 1
+#define CO_SLOPE_V 2
+#define CO_TRANS 3
 
 #endif

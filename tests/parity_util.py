@@ -43,6 +43,8 @@ def instrument_cases():
         "tas": (os.path.join(ROOT, "tests", "instruments", "tas.json"), {"Mono_angle": ang}),
         "powder_diff": (os.path.join(ROOT, "ramp_amd", "instruments", "powder_diff.json"),
                         _powder_diff_vars()),
+        "polarisation": (os.path.join(ROOT, "ramp_amd", "instruments", "polarisation.json"),
+                         {"py": 1.0, "ax": 0.6, "ay": 0.8, "az": 0.0}),
         "isis_isotropic": (os.path.join(ROOT, "tests", "instruments", "isis_isotropic.json"),
                            {"spec": repr(table_files()["spectrum"]), "sqw": repr(table_files()["sqw"])}),
     }

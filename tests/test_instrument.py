@@ -120,7 +120,7 @@ def test_unknown_and_out_of_scope_components(tmp_path):
     with pytest.raises(ModuleNotFoundError):
         make(tmp_path, bad, Ei=10.0, ang=0.25)
     with pytest.raises(NotImplementedError):
-        ramp_amd.SPolariser()
+        make(tmp_path, IDF, Ei=10.0, ang=0.25).visualise()
     with pytest.raises(ValueError):
         ramp_amd.SIsotropic()
 

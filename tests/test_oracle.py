@@ -409,3 +409,50 @@ def test_rescal_cone_and_energy_closed_form():
     assert abs(dE.mean()) < 5 * np.sqrt(var / len(dE))
     assert abs(dE.var() / var - 1) < 0.03
     assert np.allclose(ev[4], 1.0 / n, rtol=1e-6)
+
+
+def _pol_run(n, **kw):
+    from oracle import pyoracle
+    inst = ramp_amd.Instrument(os.path.join(ROOT, "ramp_amd", "instruments", "polarisation.json"),
+                               None, None, **kw)
+    return pyoracle.run(inst, n, [3], keep_state=True)
+
+
+def test_polariser_analyser_closed_form():
+    """Ideal polariser then analyser (polariser.cl, analyser.cl): the same
+    histories reach the monitor with weight factor (1 + P.P')/2 - 1 for
+    parallel, 0 for antiparallel, 1/2 for orthogonal axes and for a
+    polariser that zeroes the polarisation."""
+    n = 50000
+    par = _pol_run(n, py=1.0, ax=0, ay=1, az=0)["hist"][0][0]
+    anti = _pol_run(n, py=1.0, ax=0, ay=-1, az=0)["hist"][0][0]
+    orth = _pol_run(n, py=1.0, ax=1, ay=0, az=0)["hist"][0][0]
+    zero = _pol_run(n, py=0.0, ax=0, ay=1, az=0)["hist"][0][0]
+    oblique = _pol_run(n, py=1.0, ax=0.6, ay=0.8, az=0)["hist"][0][0]
+    assert par.sum() > 0
+    assert np.all(anti == 0)
+    assert np.allclose(orth, 0.5 * par, rtol=1e-6, atol=0)
+    assert np.allclose(zero, 0.5 * par, rtol=1e-6, atol=0)
+    assert np.allclose(oblique, 0.9 * par, rtol=1e-6, atol=0)
+
+
+def test_linear_collimator_closed_form():
+    """collimator_lin.cl:3-55 with collimator_lin.py's slopes tan(div'):
+    a history dies there exactly when |vx/vz| > tan(30') or |vy/vz| >
+    tan(60'), and a survivor's weight is 1/N * 0.95(1 - phi_H/s_H) *
+    0.95(1 - phi_V/s_V) (analyser parallel to the polariser: factor 1)."""
+    n = 50000
+    n16, _, _ = _pol_run(n, py=1.0, ax=0, ay=1, az=0)["state"]
+    v = n16[:, 3:6].astype(np.float64)
+    sh, sv = np.tan(30 * np.pi / 10800), np.tan(60 * np.pi / 10800)
+    ph, pv = np.abs(v[:, 0] / v[:, 2]), np.abs(v[:, 1] / v[:, 2])
+    at_coll = n16[:, 12] == 3  # prevcomp: source 1, polariser 2, collimator 3
+    killed = at_coll & (n16[:, 15] == 1)
+    assert killed.sum() > 1000
+    assert np.all((ph[killed] > sh * (1 - 1e-6)) | (pv[killed] > sv * (1 - 1e-6)))
+    ok = n16[:, 12] == 5  # reached the monitor
+    assert ok.sum() > 1000
+    assert np.all((ph[ok] <= sh * (1 + 1e-6)) & (pv[ok] <= sv * (1 + 1e-6)))
+    expect = (1.0 / n) * 0.95 * (1 - ph[ok] / sh) * 0.95 * (1 - pv[ok] / sv)
+    # FP32 phi/slope: an absolute error of a few ulps of the unit factor
+    assert np.allclose(n16[ok, 9], expect, rtol=0, atol=2e-6 / n)
