@@ -76,6 +76,7 @@ extern "C" {
 #define RAMP_S_POLARISER 19  /* SPolariser     scat/polariser.cl:3-32 */
 #define RAMP_S_ANALYSER 20   /* SAnalyser      scat/analyser.cl:1-31 (with the consts.h include it lacks) */
 #define RAMP_S_COLLIMATOR 21 /* SLinearCollimator scat/collimator_lin.cl:3-55 (with the ';' it lacks at :30) */
+#define RAMP_S_PLUGIN 22     /* user scatter compiled at run time (ramp_plugin_compile; mcramp.py:13-17, 80-92) */
 
 /* ---- plan descriptors --------------------------------------------------- */
 #define RAMP_NP 48  /* float parameters per component part */
@@ -214,6 +215,25 @@ int ramp_stats_read(const ramp_plan *plan, ramp_run_stats *out);
  * which copies the whole neutron buffer back and filters it in Python;
  * the file header is written by the caller. */
 int ramp_mcpl_read(ramp_plan *plan, uint32_t comp, float *records, uint64_t cap, uint64_t *count);
+
+/* ---- user-written scatter plugins -----------------------------------------
+ * The reference resolves unknown component names to user modules whose
+ * OpenCL kernels it builds at run time (build_kernel, mcramp.py:13-17;
+ * importlib fallback :80-92; template doc/writing/scatter.rst:51-69).
+ * Here a plugin is a HIP device function with the signature of
+ * ramp_amd/csrc/ramp_plugin.h, compiled for the context's GPU with hipRTC.
+ * `source` holds the function (and anything it needs), `entry` its name.
+ * A component whose scatter part has type RAMP_S_PLUGIN runs the plugin
+ * bound to it after its geometry and terminate steps; every such
+ * component needs a bound plugin before ramp_run.  Plans with plugins run
+ * in full-state mode; the plugin step is a segment of its own.  The
+ * compiler's log of a failed build is the error message. */
+typedef struct ramp_plugin ramp_plugin;
+int ramp_plugin_compile(ramp_ctx *ctx, const char *source, const char *entry, ramp_plugin **out);
+int ramp_plugin_destroy(ramp_plugin *plugin);
+/* Bind `plugin` (of the plan's context) to component `comp` (its index in
+ * ramp_plan_desc.comps). */
+int ramp_plan_bind_plugin(ramp_plan *plan, uint32_t comp, ramp_plugin *plugin);
 
 /* ---- SRescal event lists --------------------------------------------------
  * Events of SRescal component `comp` (its index among the plan's SRescal

@@ -52,6 +52,11 @@ void oracle_set_mcpl_dump(float *records, uint32_t *tags, uint64_t cap);
  * appended at counts[slot] (not cleared), in chunk and history order. */
 void oracle_set_rescal_out(float *events, uint64_t cap, uint64_t *counts);
 
+/* The host build of the plugin bound to component `comp` (its index in
+ * desc->comps): a function (float *neutron16, float *isect8, uint32_t
+ * *iidx, uint32_t comp_idx, const float *params, uint32_t tid). */
+void oracle_set_plugin(uint32_t comp, void *fn);
+
 /* Test hook: the shared rotate-about-axis restatement (geom.h:1-19). */
 void oracle_rotate_axis(float angle, const float *axis, const float *v, float *out);
 

@@ -38,6 +38,8 @@ def load():
                                        C.POINTER(C.c_uint32), C.c_uint32, C.c_uint32,
                                        C.POINTER(C.c_float), C.POINTER(C.c_double), C.c_uint32,
                                        C.c_int]
+        lib.oracle_set_plugin.restype = None
+        lib.oracle_set_plugin.argtypes = [C.c_uint32, C.c_void_p]
         lib.oracle_set_rescal_out.restype = None
         lib.oracle_set_rescal_out.argtypes = [C.POINTER(C.c_float), C.c_uint64, C.POINTER(C.c_uint64)]
         lib.oracle_set_mcpl_dump.restype = None
@@ -128,3 +130,27 @@ def adjoint(inst, n_hist, seeds, det, dL_dh, n_grad, chunk=0, threads=0):
     if rc != 0:
         raise RuntimeError(f"oracle error {rc}: {lib.oracle_last_error().decode()}")
     return grad
+
+
+def build_host_plugin(source, entry, out_dir):
+    """Compile a scatter plugin's source for the host (g++, the same
+    arithmetic flags as the oracle) and return (library, function address)
+    for oracle_set_plugin.  The source sees ramp_plugin.h as on the GPU."""
+    import ctypes
+    csrc = os.path.join(os.path.dirname(HERE), "ramp_amd", "csrc")
+    src = os.path.join(out_dir, f"plugin_{entry}.cpp")
+    so = os.path.join(out_dir, f"plugin_{entry}.so")
+    with open(src, "w") as f:
+        f.write('#include <stdint.h>\n#include "rrng.h"\n#include "ramp_plugin.h"\n')
+        f.write(source)
+        f.write(f'\nextern "C" void ramp_host_entry(float *n, float *is, uint32_t *ii, uint32_t ci, '
+                f'const float *p, uint32_t tid) {{ {entry}((RampNeutron *)n, (RampIntersection *)is, '
+                f'ii, ci, p, tid); }}\n')
+    subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-I", csrc, src, "-o", so],
+                   check=True)
+    lib = ctypes.CDLL(so)
+    return lib, ctypes.cast(lib.ramp_host_entry, ctypes.c_void_p).value
+
+
+def set_plugin(comp, fn_addr):
+    load().oracle_set_plugin(int(comp), fn_addr)

@@ -1629,6 +1629,19 @@ static void o_mcpl_out(const octx *cx, on_t *n, const ramp_part_desc *s)
     g_dump_tag[slot] = cx->ck + 1;
 }
 
+/* ======================= user plugins ===================================== */
+
+/* a plugin compiled for the host from the same source as the GPU one:
+ * (neutron16, isect8, iidx, comp_idx, params, tid) */
+typedef void (*oracle_plugin_fn)(float *, float *, uint32_t *, uint32_t, const float *, uint32_t);
+#define ORACLE_MAX_PLUGINS 256
+static oracle_plugin_fn g_plugin[ORACLE_MAX_PLUGINS];
+
+void oracle_set_plugin(uint32_t comp, void *fn)
+{
+    if (comp < ORACLE_MAX_PLUGINS) g_plugin[comp] = (oracle_plugin_fn)fn;
+}
+
 /* ======================= host loop ====================================== */
 
 /* ExecutionBlock.execute: mcramp.py:118-220 */
@@ -1643,7 +1656,13 @@ static void o_block(octx *cx, on_t *n, const ramp_block_desc *b)
             o_terminate(n, c->restore);
             if (c->scat.type == RAMP_S_MCPL_IN) o_mcpl_in(cx, n, &c->scat);
             else if (c->scat.type == RAMP_S_MCPL_OUT) o_mcpl_out(cx, n, &c->scat);
-            else o_scatter(cx, n, c);
+            else if (c->scat.type == RAMP_S_PLUGIN) {
+                /* the plugin checks iidx and the dead flag itself, as the
+                 * reference's kernels do (doc/writing/scatter.rst:9-47) */
+                const uint32_t ci = (uint32_t)(c - cx->d->comps);
+                if (ci < ORACLE_MAX_PLUGINS && g_plugin[ci])
+                    g_plugin[ci](n->s, n->is, &n->iidx, c->comp_idx, c->scat.p, n->tid);
+            } else o_scatter(cx, n, c);
             o_untransform(n, c);
         }
     } else {

@@ -146,6 +146,9 @@ _SIGS = {
                                  C.POINTER(C.c_uint64)]),
     "ramp_rescal_read": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_float), C.c_uint64,
                                    C.POINTER(C.c_uint64)]),
+    "ramp_plugin_compile": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p)]),
+    "ramp_plugin_destroy": (C.c_int, [C.c_void_p]),
+    "ramp_plan_bind_plugin": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
     "ramp_math_eval": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float),
                                  C.POINTER(C.c_float), C.c_uint64, C.c_int]),
     "ramp_rng_eval": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_float),

@@ -1,0 +1,73 @@
+/* dev_queue.h - the structure-of-arrays neutron queue between segment
+ * launches: one neutron's load and store, and the map from a dense index to
+ * its slot among the producing segment's per-wave regions.  Shared by the
+ * segment kernels and by run-time compiled plugin kernels. */
+#ifndef RAMP_DEV_QUEUE_H
+#define RAMP_DEV_QUEUE_H
+
+/* status word of the queue: iidx in bits 0..23, bit 24: intersection is all
+ * zeros (GBox miss with restore) instead of the sentinel.  The full-state
+ * mode stores the whole intersection instead (components that leave it as
+ * their geometry set it, such as the MCPL steps, need that).  Bit 25: the
+ * neutron died in a plugin step, which rewrites the queue in place; the
+ * next segment finalizes it. */
+#define ST_ZERO_ISECT (1u << 24)
+#define ST_DEAD (1u << 25)
+
+template <bool FULL>
+DEV void queue_load(NS &n, const DQueue &q, uint64_t i)
+{
+    n.x = q.x[i]; n.y = q.y[i]; n.z = q.z[i];
+    n.vx = q.vx[i]; n.vy = q.vy[i]; n.vz = q.vz[i];
+    n.w = q.w[i]; n.t = q.t[i]; n.ctr = q.ctr[i];
+    n.key = q.key[i];
+    uint32_t st = q.status[i];
+    n.iidx = st & 0x00ffffffu;
+    n.dead = FULL && (st & ST_DEAD) != 0u;
+    n.pend = false;
+    n.aux = 0u;
+    if (FULL) {
+        for (int k = 0; k < 8; k++) n.is[k] = q.is[k][i];
+    } else if (st & ST_ZERO_ISECT) {
+        for (int k = 0; k < 8; k++) n.is[k] = 0.0f;
+    } else {
+        isect_reset(n);
+    }
+    if (FULL) {
+        n.px = q.px[i]; n.py = q.py[i]; n.pz = q.pz[i];
+        n.sc = q.sc[i]; n.sd = q.sd[i]; n.se = q.se[i];
+    } else {
+        n.px = n.py = n.pz = n.sc = n.sd = n.se = 0.0f;
+    }
+}
+
+template <bool FULL>
+DEV void queue_store(const NS &n, const DQueue &q, uint64_t i)
+{
+    q.x[i] = n.x; q.y[i] = n.y; q.z[i] = n.z;
+    q.vx[i] = n.vx; q.vy[i] = n.vy; q.vz[i] = n.vz;
+    q.w[i] = n.w; q.t[i] = n.t; q.ctr[i] = n.ctr;
+    q.key[i] = n.key;
+    bool zero = (n.is[3] == 0.0f && n.is[7] == 0.0f);
+    q.status[i] = (n.iidx & 0x00ffffffu) | (zero ? ST_ZERO_ISECT : 0u) | ((FULL && n.dead) ? ST_DEAD : 0u);
+    if (FULL) {
+        q.px[i] = n.px; q.py[i] = n.py; q.pz[i] = n.pz;
+        q.sc[i] = n.sc; q.sd[i] = n.sd; q.se[i] = n.se;
+        for (int k = 0; k < 8; k++) q.is[k][i] = n.is[k];
+    }
+}
+
+/* map a dense input index to its slot in the previous segment's wave
+ * regions: the region is the last one whose prefix is <= v */
+DEV uint64_t region_slot(const uint64_t *prefix, uint64_t nreg, uint64_t stride, uint64_t v)
+{
+    uint64_t lo = 0, hi = nreg;  /* prefix[lo] <= v < prefix[hi] */
+    while (hi - lo > 1) {
+        uint64_t mid = (lo + hi) >> 1;
+        if (prefix[mid] <= v) lo = mid;
+        else hi = mid;
+    }
+    return lo * stride + (v - prefix[lo]);
+}
+
+#endif

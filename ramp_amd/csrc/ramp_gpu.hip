@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cctype>
 #include <cstdarg>
 #include <cmath>
 #include <cstdio>
@@ -24,6 +25,8 @@
 #include <cstring>
 #include <string>
 #include <vector>
+
+#include <hip/hiprtc.h>
 
 #include "../../include/ramp.h"
 #include "dev_components.h"
@@ -102,54 +105,7 @@ struct SegArgs {
     int n_acc, n_dets;
 };
 
-/* status word of the queue: iidx in bits 0..23, bit 24: intersection is all
- * zeros (GBox miss with restore) instead of the sentinel.  The full-state
- * mode stores the whole intersection instead (components that leave it as
- * their geometry set it, such as the MCPL steps, need that). */
-#define ST_ZERO_ISECT (1u << 24)
-
-template <bool FULL>
-DEV void queue_load(NS &n, const DQueue &q, uint64_t i)
-{
-    n.x = q.x[i]; n.y = q.y[i]; n.z = q.z[i];
-    n.vx = q.vx[i]; n.vy = q.vy[i]; n.vz = q.vz[i];
-    n.w = q.w[i]; n.t = q.t[i]; n.ctr = q.ctr[i];
-    n.key = q.key[i];
-    uint32_t st = q.status[i];
-    n.iidx = st & 0x00ffffffu;
-    n.dead = false;
-    n.pend = false;
-    n.aux = 0u;
-    if (FULL) {
-        for (int k = 0; k < 8; k++) n.is[k] = q.is[k][i];
-    } else if (st & ST_ZERO_ISECT) {
-        for (int k = 0; k < 8; k++) n.is[k] = 0.0f;
-    } else {
-        isect_reset(n);
-    }
-    if (FULL) {
-        n.px = q.px[i]; n.py = q.py[i]; n.pz = q.pz[i];
-        n.sc = q.sc[i]; n.sd = q.sd[i]; n.se = q.se[i];
-    } else {
-        n.px = n.py = n.pz = n.sc = n.sd = n.se = 0.0f;
-    }
-}
-
-template <bool FULL>
-DEV void queue_store(const NS &n, const DQueue &q, uint64_t i)
-{
-    q.x[i] = n.x; q.y[i] = n.y; q.z[i] = n.z;
-    q.vx[i] = n.vx; q.vy[i] = n.vy; q.vz[i] = n.vz;
-    q.w[i] = n.w; q.t[i] = n.t; q.ctr[i] = n.ctr;
-    q.key[i] = n.key;
-    bool zero = (n.is[3] == 0.0f && n.is[7] == 0.0f);
-    q.status[i] = (n.iidx & 0x00ffffffu) | (zero ? ST_ZERO_ISECT : 0u);
-    if (FULL) {
-        q.px[i] = n.px; q.py[i] = n.py; q.pz[i] = n.pz;
-        q.sc[i] = n.sc; q.sd[i] = n.sd; q.se[i] = n.se;
-        for (int k = 0; k < 8; k++) q.is[k][i] = n.is[k];
-    }
-}
+#include "dev_queue.h"
 
 /* Deferral of the monochromator's reflection branch.  s_mono is cheap
  * for the many neutrons it does not reflect and long (mosaic search,
@@ -348,19 +304,6 @@ DEV uint32_t wave_sum(uint32_t v)
 {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
     return v;
-}
-
-/* map a dense input index to its slot in the previous segment's wave
- * regions: the region is the last one whose prefix is <= v */
-DEV uint64_t region_slot(const uint64_t *prefix, uint64_t nreg, uint64_t stride, uint64_t v)
-{
-    uint64_t lo = 0, hi = nreg;  /* prefix[lo] <= v < prefix[hi] */
-    while (hi - lo > 1) {
-        uint64_t mid = (lo + hi) >> 1;
-        if (prefix[mid] <= v) lo = mid;
-        else hi = mid;
-    }
-    return lo * stride + (v - prefix[lo]);
 }
 
 /* end of a group of 64 in a segment: survivors to the wave's output
@@ -603,6 +546,13 @@ struct ramp_ctx {
     int n_cu;
     hipStream_t stream;
     hipEvent_t ev[6];
+    std::string arch;   /* gcnArchName, for run-time compiled plugins */
+};
+
+struct ramp_plugin {
+    ramp_ctx *ctx = nullptr;
+    hipModule_t mod = nullptr;
+    hipFunction_t fn = nullptr;
 };
 
 struct DevBuf {
@@ -665,6 +615,8 @@ struct ramp_plan {
     bool need_full = false;   /* MCPL, SRescal or SAnalyser present: full-state execution */
     uint32_t n_rescal = 0;    /* SRescal components (event slots) */
     int n_acc = 0;            /* single-bin detectors accumulated in LDS */
+    std::vector<int> plugin_step;           /* per step: the component number of a plugin step, else -1 */
+    std::vector<ramp_plugin *> plugin_of;   /* per component: the bound plugin */
     std::vector<uint32_t> rescal_comp; /* their component numbers */
     DevBuf ev, ev_tag, ev_list, ev_list_n; /* per-chunk events, tags; accumulated lists [slot][5][list_cap], counts */
     uint64_t ev_cap = 0, ev_list_cap = 0;
@@ -693,6 +645,7 @@ extern "C" int ramp_ctx_create(int device, ramp_ctx **out)
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, device));
     c->n_cu = prop.multiProcessorCount;
+    c->arch = prop.gcnArchName;
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (int i = 0; i < 6; i++) HIPCHK(hipEventCreate(&c->ev[i]));
     *out = c;
@@ -984,8 +937,9 @@ static int validate_plan(const ramp_plan_desc *d)
         if (!bl.linear && bl.multi > 100000) return fail(RAMP_E_RANGE, "block %u: multi too large", b);
         if (!bl.linear)
             for (uint32_t k = bl.first; k < bl.first + bl.count; k++)
-                if (d->comps[k].scat.type == RAMP_S_MCPL_IN || d->comps[k].scat.type == RAMP_S_MCPL_OUT)
-                    return fail(RAMP_E_INVALID, "component %u: SMCPLIn/SMCPLOut must be in a linear block", k);
+                if (d->comps[k].scat.type == RAMP_S_MCPL_IN || d->comps[k].scat.type == RAMP_S_MCPL_OUT ||
+                    d->comps[k].scat.type == RAMP_S_PLUGIN)
+                    return fail(RAMP_E_INVALID, "component %u: SMCPLIn/SMCPLOut/plugins must be in a linear block", k);
         /* the event filter reads the detected flag at the end of the block
          * (rescal.py:51-55 is called when its block ends) */
         if (b + 1 < d->n_blocks)
@@ -997,7 +951,7 @@ static int validate_plan(const ramp_plan_desc *d)
         const ramp_component_desc &c = d->comps[k];
         if (c.geom.type < RAMP_G_ARM || c.geom.type > RAMP_G_MESH)
             return fail(RAMP_E_INVALID, "component %u: unknown geometry type %u", k, c.geom.type);
-        if (c.scat.type < RAMP_S_ARM || c.scat.type > RAMP_S_COLLIMATOR)
+        if (c.scat.type < RAMP_S_ARM || c.scat.type > RAMP_S_PLUGIN)
             return fail(RAMP_E_INVALID, "component %u: unknown scatter type %u", k, c.scat.type);
         if (c.comp_idx == 0 || c.comp_idx > 0x00ffffffu)
             return fail(RAMP_E_INVALID, "component %u: index %u out of range", k, c.comp_idx);
@@ -1130,9 +1084,17 @@ extern "C" int ramp_plan_create(ramp_ctx *ctx, const ramp_plan_desc *d, ramp_pla
     for (size_t k = 0; k < p->hsteps.size(); k++)
         if (p->hsteps[k].kind == 0 && p->hcomps[p->hsteps[k].first].s.type == RAMP_S_MCPL_IN)
             p->revive_k = (int)k;
-    bool analyser = false;
-    for (const DComp &c : p->hcomps) analyser = analyser || c.s.type == RAMP_S_ANALYSER;
-    p->need_full = p->has_mcpl || p->n_rescal > 0 || analyser;
+    bool analyser = false, plugin = false;
+    for (const DComp &c : p->hcomps) {
+        analyser = analyser || c.s.type == RAMP_S_ANALYSER;
+        plugin = plugin || c.s.type == RAMP_S_PLUGIN;
+    }
+    p->need_full = p->has_mcpl || p->n_rescal > 0 || analyser || plugin;
+    p->plugin_of.assign(p->hcomps.size(), nullptr);
+    p->plugin_step.assign(p->hsteps.size(), -1);
+    for (size_t k = 0; k < p->hsteps.size(); k++)
+        if (p->hsteps[k].kind == 0 && p->hcomps[p->hsteps[k].first].s.type == RAMP_S_PLUGIN)
+            p->plugin_step[k] = (int)p->hsteps[k].first;
     p->ev_count.assign(p->n_rescal, 0);
     p->det_desc.assign(d->dets, d->dets + d->n_dets);
     p->hdets.resize(d->n_dets);
@@ -1372,11 +1334,103 @@ static int choose_bounds(ramp_plan *p, const SegArgs &base, uint64_t chunk, unsi
         }
     }
     p->bounds.push_back(ns);
+    /* a plugin step is a segment of its own (run by its plugin kernel) */
+    bool first_plugin = false;
+    for (int k = 0; k < ns; k++)
+        if (p->plugin_step[k] >= 0) {
+            p->bounds.push_back(k);
+            p->bounds.push_back(k + 1);
+            first_plugin = first_plugin || k == 0;
+        }
+    std::sort(p->bounds.begin(), p->bounds.end());
+    p->bounds.erase(std::unique(p->bounds.begin(), p->bounds.end()), p->bounds.end());
+    if (first_plugin) p->bounds.insert(p->bounds.begin(), 0);  /* the source alone first */
+    const size_t nb = p->bounds.size();
+    if (nb >= 2 && p->bounds[nb - 1] == p->bounds[nb - 2] + 1 && p->plugin_step[p->bounds[nb - 2]] >= 0)
+        p->bounds.push_back(ns);  /* a last, empty segment finalizes the neutrons */
     p->bounds_set = true;
     return 0;
 }
 
 #include "mcpl_run.h"
+
+/* ---- run-time compiled scatter plugins (hipRTC) ----------------------- */
+#include "plugin_headers.inc"
+
+struct PluginArgs {  /* the kernel argument of ramp_plugin_kernel (plugin_kernel.h) */
+    DQueue q;
+    const uint64_t *prefix;
+    const uint64_t *meta;
+    const DComp *comp;
+};
+
+extern "C" int ramp_plugin_compile(ramp_ctx *ctx, const char *source, const char *entry, ramp_plugin **out)
+{
+    if (!ctx || !source || !entry || !out) return fail(RAMP_E_INVALID, "null argument");
+    if (!*entry) return fail(RAMP_E_INVALID, "empty plugin entry name");
+    for (const char *c = entry; *c; c++)
+        if (!(std::isalnum((unsigned char)*c) || *c == '_'))
+            return fail(RAMP_E_INVALID, "plugin entry '%s' is not an identifier", entry);
+    std::string prog = std::string("#include \"plugin_kernel.h\"\n#line 1 \"plugin\"\n") + source +
+                       "\nRAMP_PLUGIN_KERNEL(" + entry + ")\n";
+    hiprtcProgram pr;
+    if (hiprtcCreateProgram(&pr, prog.c_str(), "ramp_plugin.hip", kPluginHeaderCount, kPluginHeaderTexts,
+                            kPluginHeaderNames) != HIPRTC_SUCCESS)
+        return fail(RAMP_E_DEVICE, "hiprtcCreateProgram failed");
+    std::string arch = "--offload-arch=" + ctx->arch;
+    const char *opts[] = {arch.c_str(), "-O3", "-std=c++17", "-ffp-contract=off",
+                          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+    hiprtcResult r = hiprtcCompileProgram(pr, 8, opts);
+    if (r != HIPRTC_SUCCESS) {
+        size_t n = 0;
+        hiprtcGetProgramLogSize(pr, &n);
+        std::string log(n, '\0');
+        if (n) hiprtcGetProgramLog(pr, &log[0]);
+        hiprtcDestroyProgram(&pr);
+        g_err = "plugin compilation failed:\n" + log;
+        return RAMP_E_INVALID;
+    }
+    size_t n = 0;
+    hiprtcGetCodeSize(pr, &n);
+    std::vector<char> code(n);
+    hiprtcGetCode(pr, code.data());
+    hiprtcDestroyProgram(&pr);
+    HIPCHK(hipSetDevice(ctx->device));
+    ramp_plugin *pl = new ramp_plugin();
+    pl->ctx = ctx;
+    hipError_t e = hipModuleLoadData(&pl->mod, code.data());
+    if (e == hipSuccess) e = hipModuleGetFunction(&pl->fn, pl->mod, "ramp_plugin_kernel");
+    if (e != hipSuccess) {
+        if (pl->mod) (void)hipModuleUnload(pl->mod);
+        delete pl;
+        return fail(RAMP_E_DEVICE, "loading the plugin: %s", hipGetErrorString(e));
+    }
+    *out = pl;
+    return 0;
+}
+
+extern "C" int ramp_plugin_destroy(ramp_plugin *pl)
+{
+    if (!pl) return 0;
+    if (pl->mod) {
+        (void)hipSetDevice(pl->ctx->device);
+        (void)hipStreamSynchronize(pl->ctx->stream);
+        (void)hipModuleUnload(pl->mod);
+    }
+    delete pl;
+    return 0;
+}
+
+extern "C" int ramp_plan_bind_plugin(ramp_plan *p, uint32_t comp, ramp_plugin *pl)
+{
+    if (!p || !pl) return fail(RAMP_E_INVALID, "null argument");
+    if (comp >= p->hcomps.size()) return fail(RAMP_E_INVALID, "component %u of %zu", comp, p->hcomps.size());
+    if (p->hcomps[comp].s.type != RAMP_S_PLUGIN)
+        return fail(RAMP_E_INVALID, "component %u is not a plugin component", comp);
+    if (pl->ctx != p->ctx) return fail(RAMP_E_INVALID, "plugin of another context");
+    p->plugin_of[comp] = pl;
+    return 0;
+}
 
 extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uint32_t *seeds,
                         uint32_t n_seeds, const ramp_run_opts *opts)
@@ -1397,6 +1451,9 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
         return fail(RAMP_E_INVALID, "need %llu seeds, got %u", (unsigned long long)n_chunks, n_seeds);
     for (uint64_t c = 0; c < n_chunks; c++)
         if (seeds[c] > 0x7fffffffu) return fail(RAMP_E_INVALID, "seed %u out of range", seeds[c]);
+    for (size_t k = 0; k < p->hcomps.size(); k++)
+        if (p->hcomps[k].s.type == RAMP_S_PLUGIN && !p->plugin_of[k])
+            return fail(RAMP_E_STATE, "component %zu: no plugin bound (ramp_plan_bind_plugin)", k);
     ramp_ctx *ctx = p->ctx;
     HIPCHK(hipSetDevice(ctx->device));
     hipStream_t s = ctx->stream;
@@ -1501,13 +1558,27 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
         a.n = torun;
         a.chunk_tag = ++p->dump_chunk_tag ? p->dump_chunk_tag : ++p->dump_chunk_tag;
         if (p->n_rescal) launch_fill_f32(s, p->ev.p, (uint64_t)p->n_rescal * RS_NEV * p->ev_cap);
+        int cur = 0; /* the queue holding the live neutrons */
         for (int sg = 0; sg < nseg; sg++) {
             a.s0 = p->bounds[sg];
             a.s1 = p->bounds[sg + 1];
+            if (sg > 0 && a.s1 == a.s0 + 1 && p->plugin_step[a.s0] >= 0) {
+                /* a plugin step rewrites the queue in place */
+                ramp_plugin *pl = p->plugin_of[p->plugin_step[a.s0]];
+                PluginArgs pa;
+                pa.q = queue_view(p, cur);
+                pa.prefix = (const uint64_t *)p->prefix.p;
+                pa.meta = meta + 2;
+                pa.comp = (const DComp *)p->comps.p + p->plugin_step[a.s0];
+                void *args[] = {&pa};
+                HIPCHK(hipModuleLaunchKernel(pl->fn, (unsigned)ctx->n_cu * 8, 1, 1, 256, 1, 1, 0, s, args, nullptr));
+                segs++;
+                continue;
+            }
             a.last = (sg == nseg - 1) ? 1 : 0;
             a.pilot = 0;
-            if (sg >= 1) a.in = queue_view(p, (sg - 1) % 2);
-            a.out = queue_view(p, sg % 2);
+            if (sg >= 1) a.in = queue_view(p, cur);
+            a.out = queue_view(p, sg == 0 ? 0 : cur ^ 1);
             unsigned grid;
             if (sg == 0) grid = (unsigned)std::min<uint64_t>((torun + wg - 1) / wg, grid_first);
             else grid = grid_rest;
@@ -1516,6 +1587,7 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
             if (sg == 0 && ck == 0) HIPCHK(hipEventRecord(ctx->ev[3], s));
             HIPCHK(hipGetLastError());
             segs++;
+            cur = sg == 0 ? 0 : cur ^ 1;
             if (sg + 1 < nseg) {
                 uint64_t nreg = (uint64_t)grid * (wg / 64);
                 hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, (const uint32_t *)p->rcounts.p,

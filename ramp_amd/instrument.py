@@ -133,6 +133,8 @@ class Context:
 
     def close(self):
         if getattr(self, "handle", None) is not None:
+            from .plugins import release
+            release(self.handle)
             capi.load_library().ramp_ctx_destroy(self.handle)
             self.handle = None
 
@@ -310,6 +312,11 @@ class Instrument:
             h = C.c_void_p()
             capi.check(lib.ramp_plan_create(self._ctx_handle(), C.byref(desc), C.byref(h)))
             self._plan = h
+            for k, c in enumerate(self._comp_list):
+                if c.scat.type_code == capi.RAMP_S_PLUGIN:
+                    from .plugins import build_kernel
+                    pl = build_kernel(c.scat.plugin_source(), c.scat.entry, self._ctx_handle())
+                    capi.check(lib.ramp_plan_bind_plugin(self._plan, k, pl))
         return lib
 
     # ---- execution ----------------------------------------------------------
@@ -434,6 +441,8 @@ class Instrument:
             lib.ramp_plan_destroy(self._plan)
             self._plan = None
         if self._gctx is not None:
+            from .plugins import release
+            release(self._gctx)
             lib.ramp_ctx_destroy(self._gctx)
             self._gctx = None
 
