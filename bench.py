@@ -61,6 +61,7 @@ WORKLOADS = {
     "mcpl_out": dict(file="mcpl_out.json", histories=2e7, name="mcpl_out_guide_dump_2e7", bins=1000),
     "mcpl_in": dict(file="mcpl_in.json", histories=2e7, name="mcpl_in_2e7_particles", bins=180 * 560),
     "rescal": dict(file="rescal.json", histories=5e7, name="rescal_events_5e7", bins=1),
+    "plugin": dict(file="plugin.json", histories=5e7, name="plugin_attenuator_5e7", bins=500),
 }
 
 
@@ -140,7 +141,7 @@ def main():
         return bench_tas_scan(a, wl, sync)
     if a.workload == "powder_adjoint":
         return bench_powder_adjoint(a, wl, sync)
-    if a.workload in ("mcpl_out", "mcpl_in", "rescal"):
+    if a.workload in ("mcpl_out", "mcpl_in", "rescal", "plugin"):
         return bench_mcpl(a, wl, sync)
     tmpdir = tempfile.mkdtemp(prefix="ramp_bench_")
     kw = instrument_args(a.workload, tmpdir)
@@ -441,6 +442,10 @@ def bench_mcpl(a, wl, sync):
         kw = {"mcpl_file": repr(""), "standard": 0}
     elif a.workload == "rescal":
         kw = {"tx": 0.3, "tz": 1.0, "arm_rot": math.atan2(0.3, 1.0)}
+    elif a.workload == "plugin":
+        # the plugin module sits beside its instrument file, as a user's would
+        sys.path.insert(0, os.path.join(ROOT, "ramp_amd", "instruments"))
+        kw = {"where": 1, "f3": 0.5, "q3": 0.5}
     else:
         rng = np.random.default_rng(7)
         rec = np.zeros((N, 11), np.float32)
@@ -473,6 +478,9 @@ def bench_mcpl(a, wl, sync):
             recs += inst._mcpl_out[0].scat.records.shape[0]
         if a.workload == "rescal":
             app_ms += st["ms_rescal_append"]
+        if a.workload == "plugin":
+            app_ms += st["ms_plugin"]
+            ev0 += st["plugin_items"]
     sync()
     dt = time.perf_counter() - t0
     K = max(a.steps, 1)
@@ -516,6 +524,19 @@ def bench_mcpl(a, wl, sync):
                            "k_ev_append, k_ev_advance)", "achieved": ach, "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None, "launch_ms": a_ms,
                            "algorithmic_bytes_per_launch": alg}
+    elif a.workload == "plugin":
+        items = ev0 / K
+        p_ms = app_ms / K
+        # the plugin step reads and rewrites each queued neutron's full-state
+        # record: 23 floats, key and status = 100 B each way
+        alg = 200.0 * items
+        ach = alg / (p_ms * 1e-3) / 1e9 if p_ms > 0 else 0.0
+        out["config"].update({"plugin_items_per_step": items, "plugin_ms": p_ms,
+                              "plugin": "ramp_amd/instruments/SAttenuatorPlugin.py (hipRTC)"})
+        out["roofline"] = {"bound": "hbm", "kernel": "ramp_plugin_kernel (SAttenuatorPlugin, in place on "
+                           "the full-state queue)", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": ach / HBM_PEAK_GBS, "traffic": None, "launch_ms": p_ms,
+                           "algorithmic_bytes_per_launch": alg}
     else:
         # the source segment reads each particle's 44-byte record
         seg_ms = inst.stats()["ms_source_segment"]
@@ -541,6 +562,15 @@ def cpu_baseline_mcpl(path, kw, target_s, label, workload):
     from oracle import pyoracle
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     inst = ramp_amd.Instrument(path, **kw)
+    if workload == "plugin":
+        # the plugin's host build (the same source, g++) runs in the oracle
+        import tempfile
+        inst.plan_desc()
+        for k, c in enumerate(inst._comp_list):
+            if c.scat.type_code == ramp_amd.capi.RAMP_S_PLUGIN:
+                _lib, fn = pyoracle.build_host_plugin(c.scat.plugin_source(), c.scat.entry,
+                                                      tempfile.mkdtemp(prefix="ramp_plugin_"))
+                pyoracle.set_plugin(k, fn)
     n = 200_000
     t = time.perf_counter()
     pyoracle.run(inst, n, [1], threads=threads)

@@ -163,6 +163,8 @@ typedef struct {
     double ms_mcpl_compact;  /* last ramp_mcpl_read: device time of its three compaction kernels */
     double ms_mcpl_write;    /* last ramp_mcpl_read: device time of the record-writing pass alone */
     double ms_rescal_append; /* device time of the first chunk's SRescal event append (count, scan, append) */
+    double ms_plugin;        /* device time of the first chunk's first plugin step */
+    uint64_t plugin_items;   /* neutrons that step processed (the queue it rewrote) */
 } ramp_run_stats;
 
 typedef struct ramp_ctx ramp_ctx;

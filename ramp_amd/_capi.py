@@ -114,6 +114,8 @@ class RunStats(C.Structure):
         ("ms_mcpl_compact", C.c_double),
         ("ms_mcpl_write", C.c_double),
         ("ms_rescal_append", C.c_double),
+        ("ms_plugin", C.c_double),
+        ("plugin_items", C.c_uint64),
     ]
 
 

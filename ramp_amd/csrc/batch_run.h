@@ -57,8 +57,7 @@ void k_batch(BatchArgs b)
     hc.events = hc.oob = hc.overflow = 0;
     hc.dets = nullptr;
     hc.mode = 0;
-    hc.acc_f = nullptr;  /* settings change per tile: global atomics */
-    hc.acc_q = nullptr;
+    hc.acc = nullptr;  /* settings change per tile: global atomics */
     const uint32_t lane = __lane_id();
     DeferQ dq;
     dq.b = batch_lds;

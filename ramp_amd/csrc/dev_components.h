@@ -443,9 +443,9 @@ struct HistCtx {
     int mode;       /* RAMP_HIST_* */
     int enabled;    /* 0 in pilot runs */
     uint32_t events, oob, overflow;
-    /* LDS accumulators of the single-bin detectors (null: global atomics) */
-    float *acc_f;                 /* [DET_ACC_MAX][2] atomic mode */
-    unsigned long long *acc_q;    /* [DET_ACC_MAX][2] deterministic mode */
+    /* the workgroup's LDS copy of the small detectors (null: global
+     * atomics only): (w, w^2) per bin, float or fixed point by mode */
+    void *acc;
 };
 
 DEV void hist_add(HistCtx &hc, int det, uint64_t idx, float w)
@@ -464,16 +464,18 @@ DEV void hist_add(HistCtx &hc, int det, uint64_t idx, float w)
             hc.overflow++;
             return;
         }
-        if (d.acc >= 0 && hc.acc_q) {
-            atomicAdd(&hc.acc_q[2 * d.acc], (unsigned long long)(long long)a);
-            atomicAdd(&hc.acc_q[2 * d.acc + 1], (unsigned long long)(long long)b);
+        if (d.acc >= 0 && hc.acc) {
+            unsigned long long *q = (unsigned long long *)hc.acc + 2 * ((uint64_t)d.acc + idx);
+            atomicAdd(q, (unsigned long long)(long long)a);
+            atomicAdd(q + 1, (unsigned long long)(long long)b);
         } else {
             atomicAdd(&d.qw[idx], (unsigned long long)(long long)a);
             atomicAdd(&d.qw2[idx], (unsigned long long)(long long)b);
         }
-    } else if (d.acc >= 0 && hc.acc_f) {
-        atomicAdd(&hc.acc_f[2 * d.acc], w);
-        atomicAdd(&hc.acc_f[2 * d.acc + 1], w2);
+    } else if (d.acc >= 0 && hc.acc) {
+        float *f = (float *)hc.acc + 2 * ((uint64_t)d.acc + idx);
+        atomicAdd(f, w);
+        atomicAdd(f + 1, w2);
     } else {
         unsafeAtomicAdd(&d.w[idx], w);
         unsafeAtomicAdd(&d.w2[idx], w2);
@@ -481,31 +483,33 @@ DEV void hist_add(HistCtx &hc, int det, uint64_t idx, float w)
     hc.events++;
 }
 
-/* start and end of a workgroup's LDS accumulation (all threads call both;
- * the end adds each accumulator to its detector's only bin) */
-DEV void hist_acc_begin(HistCtx &hc, float *f, unsigned long long *q)
+/* start and end of a workgroup's LDS accumulation (all threads call both):
+ * the copy is zeroed, and at the end every nonzero bin is added to HBM */
+DEV void hist_acc_begin(HistCtx &hc, void *lds, uint32_t bytes)
 {
-    if (threadIdx.x < 2 * DET_ACC_MAX) {
-        f[threadIdx.x] = 0.0f;
-        q[threadIdx.x] = 0ull;
-    }
+    for (uint32_t i = threadIdx.x; i < bytes / 4; i += blockDim.x) ((uint32_t *)lds)[i] = 0u;
     __syncthreads();
-    hc.acc_f = f;
-    hc.acc_q = q;
+    hc.acc = lds;
 }
 
-DEV void hist_acc_end(const HistCtx &hc, int n_det)
+DEV void hist_acc_end(const HistCtx &hc, int n_det, uint32_t bins)
 {
     __syncthreads();
-    const uint32_t t = threadIdx.x;
-    if (t >= 2 * DET_ACC_MAX) return;
-    for (int k = 0; k < n_det; k++) {
-        const DDet &d = hc.dets[k];
-        if (d.acc < 0 || (uint32_t)d.acc != t / 2) continue;
-        if (hc.mode == RAMP_HIST_DETERMINISTIC) {
-            if (hc.acc_q[t]) atomicAdd(t & 1 ? d.qw2 : d.qw, hc.acc_q[t]);
-        } else if (hc.acc_f[t] != 0.0f) {
-            unsafeAtomicAdd(t & 1 ? d.w2 : d.w, hc.acc_f[t]);
+    for (uint32_t i = threadIdx.x; i < bins; i += blockDim.x) {
+        for (int k = 0; k < n_det; k++) {
+            const DDet &d = hc.dets[k];
+            if (d.acc < 0 || i < (uint32_t)d.acc || i >= (uint32_t)d.acc + d.n) continue;
+            const uint64_t b = i - (uint32_t)d.acc;
+            if (hc.mode == RAMP_HIST_DETERMINISTIC) {
+                const unsigned long long *q = (const unsigned long long *)hc.acc + 2 * (uint64_t)i;
+                if (q[0]) atomicAdd(&d.qw[b], q[0]);
+                if (q[1]) atomicAdd(&d.qw2[b], q[1]);
+            } else {
+                const float *f = (const float *)hc.acc + 2 * (uint64_t)i;
+                if (f[0] != 0.0f) unsafeAtomicAdd(&d.w[b], f[0]);
+                if (f[1] != 0.0f) unsafeAtomicAdd(&d.w2[b], f[1]);
+            }
+            break;
         }
     }
 }

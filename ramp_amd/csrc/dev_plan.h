@@ -53,14 +53,15 @@ struct DDet {
     uint64_t n;
     int32_t sh_w;                /* fixed-point exponents: q = rint(w * 2^sh) */
     int32_t sh_w2;
-    int32_t acc;                 /* single-bin detector: its LDS accumulator slot (0..3), else -1 */
+    int32_t acc;                 /* small detector: its first bin in the workgroup's LDS copy, else -1 */
     int32_t pad;
 };
 
-/* single-bin detectors (an SCounter: every event hits the same address)
- * accumulate per workgroup in LDS and flush one global atomic per
- * workgroup; at most this many per plan */
-#define DET_ACC_MAX 4
+/* detectors small enough (all together within this many bytes of LDS per
+ * workgroup: 8 B per bin for float atomics, 16 B for fixed point)
+ * accumulate per workgroup in LDS and are added to HBM once per workgroup,
+ * instead of one same-address atomic per event */
+#define DET_LDS_BYTES 32768
 
 /* SoA neutron queue */
 struct DQueue {

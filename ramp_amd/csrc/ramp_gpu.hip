@@ -101,8 +101,10 @@ struct SegArgs {
     uint32_t *ev_tag;
     uint64_t ev_cap;
     uint32_t n_rescal;
-    /* single-bin detectors accumulated in LDS (DDet.acc >= 0), detector count */
-    int n_acc, n_dets;
+    /* small detectors accumulated in LDS (DDet.acc >= 0): their bins and
+     * the bytes after the deferral queues; the detector count */
+    uint32_t acc_bins, acc_bytes;
+    int n_dets;
 };
 
 #include "dev_queue.h"
@@ -338,16 +340,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SEG_MIN_WAV
 void k_segment(SegArgs a)
 {
     extern __shared__ float seg_lds[];
-    __shared__ float acc_f[2 * DET_ACC_MAX];
-    __shared__ unsigned long long acc_q[2 * DET_ACC_MAX];
     HistCtx hc;
     hc.dets = a.dets;
     hc.mode = a.hist_mode;
     hc.enabled = !a.pilot;
     hc.events = hc.oob = hc.overflow = 0;
-    hc.acc_f = nullptr;
-    hc.acc_q = nullptr;
-    if (a.n_acc) hist_acc_begin(hc, acc_f, acc_q);
+    hc.acc = nullptr;
+    if (a.acc_bins) {
+        /* the detectors' LDS copy follows the deferral queues */
+        const size_t defer_bytes = (!FULL && (F & FEAT_MONO) != 0) ? (blockDim.x >> 6) * DQ_BYTES_PER_WAVE : 0;
+        hist_acc_begin(hc, (char *)seg_lds + defer_bytes, a.acc_bytes);
+    }
     const uint32_t lane = __lane_id();
     const uint64_t G = (uint64_t)gridDim.x * (blockDim.x >> 6);
     const uint64_t g = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -413,7 +416,7 @@ void k_segment(SegArgs a)
         }
     }
     if (!a.last && lane == 0) a.out_counts[g] = wcount;
-    if (a.n_acc) hist_acc_end(hc, a.n_dets);
+    if (a.acc_bins) hist_acc_end(hc, a.n_dets, a.acc_bins);
     uint32_t ev = wave_sum(hc.events), oob = wave_sum(hc.oob), ovf = wave_sum(hc.overflow);
     if (lane == 0) {
         if (ev) atomicAdd(&a.cnt[CNT_EVENTS], (unsigned long long)ev);
@@ -545,7 +548,7 @@ struct ramp_ctx {
     int device;
     int n_cu;
     hipStream_t stream;
-    hipEvent_t ev[6];
+    hipEvent_t ev[8];
     std::string arch;   /* gcnArchName, for run-time compiled plugins */
 };
 
@@ -614,7 +617,7 @@ struct ramp_plan {
     bool has_mcpl = false;    /* SMCPLIn/SMCPLOut present */
     bool need_full = false;   /* MCPL, SRescal or SAnalyser present: full-state execution */
     uint32_t n_rescal = 0;    /* SRescal components (event slots) */
-    int n_acc = 0;            /* single-bin detectors accumulated in LDS */
+    uint32_t acc_bins = 0, acc_bytes = 0; /* small detectors' LDS copy (set per run by the mode) */
     std::vector<int> plugin_step;           /* per step: the component number of a plugin step, else -1 */
     std::vector<ramp_plugin *> plugin_of;   /* per component: the bound plugin */
     std::vector<uint32_t> rescal_comp; /* their component numbers */
@@ -647,7 +650,7 @@ extern "C" int ramp_ctx_create(int device, ramp_ctx **out)
     c->n_cu = prop.multiProcessorCount;
     c->arch = prop.gcnArchName;
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    for (int i = 0; i < 6; i++) HIPCHK(hipEventCreate(&c->ev[i]));
+    for (int i = 0; i < 8; i++) HIPCHK(hipEventCreate(&c->ev[i]));
     *out = c;
     return 0;
 }
@@ -657,7 +660,7 @@ extern "C" int ramp_ctx_destroy(ramp_ctx *ctx)
     if (!ctx) return 0;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    for (int i = 0; i < 6; i++) (void)hipEventDestroy(ctx->ev[i]);
+    for (int i = 0; i < 8; i++) (void)hipEventDestroy(ctx->ev[i]);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return 0;
@@ -1106,7 +1109,6 @@ extern "C" int ramp_plan_create(ramp_ctx *ctx, const ramp_plan_desc *d, ramp_pla
         std::memset(&p->hdets[k], 0, sizeof(DDet));
         p->hdets[k].n = d->dets[k].n_bins;
         p->hdets[k].acc = -1;
-        if (d->dets[k].n_bins == 1 && p->n_acc < DET_ACC_MAX) p->hdets[k].acc = p->n_acc++;
     }
     PCHK(dev_alloc(p->comps, std::max<size_t>(1, p->hcomps.size()) * sizeof(DComp)));
     PCHK(dev_alloc(p->steps, std::max<size_t>(1, p->hsteps.size()) * sizeof(DStep)));
@@ -1204,6 +1206,17 @@ static int prepare_dets(ramp_plan *p, int mode, double weight_bound, uint64_t n_
 {
     int rc = ensure_hist(p, mode);
     if (rc) return rc;
+    /* small detectors, in plan order while they fit, get an LDS copy */
+    const uint32_t per = mode == RAMP_HIST_DETERMINISTIC ? 16u : 8u;
+    p->acc_bins = 0;
+    for (auto &d : p->hdets) {
+        d.acc = -1;
+        if ((uint64_t)(p->acc_bins + d.n) * per <= DET_LDS_BYTES) {
+            d.acc = (int32_t)p->acc_bins;
+            p->acc_bins += (uint32_t)d.n;
+        }
+    }
+    p->acc_bytes = p->acc_bins * per;
     if (mode == RAMP_HIST_DETERMINISTIC && !p->det_scale_set) {
         double wb = weight_bound > 0 ? weight_bound : 1.0 / (double)chunk;
         double tot = wb * (double)n_hist * 16.0;
@@ -1283,15 +1296,16 @@ static size_t seg_lds_bytes(bool full, int v, unsigned wg)
 static void launch_segment(const SegArgs &a, bool full, bool first, int v, unsigned grid,
                            unsigned wg, hipStream_t s)
 {
-    hipLaunchKernelGGL(pick_kernel(full, first, v), dim3(grid), dim3(wg), seg_lds_bytes(full, v, wg), s, a);
+    hipLaunchKernelGGL(pick_kernel(full, first, v), dim3(grid), dim3(wg),
+                       seg_lds_bytes(full, v, wg) + a.acc_bytes, s, a);
 }
 
 /* resident blocks per CU of a kernel instance (for a persistent grid) */
-static unsigned blocks_per_cu(bool full, bool first, int v, unsigned wg)
+static unsigned blocks_per_cu(bool full, bool first, int v, unsigned wg, uint32_t acc_bytes)
 {
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pick_kernel(full, first, v), (int)wg,
-                                                     seg_lds_bytes(full, v, wg)) != hipSuccess || nb < 1)
+                                                     seg_lds_bytes(full, v, wg) + acc_bytes) != hipSuccess || nb < 1)
         nb = 1;
     return (unsigned)nb;
 }
@@ -1521,7 +1535,8 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
     a.ev_tag = (uint32_t *)p->ev_tag.p;
     a.ev_cap = p->ev_cap;
     a.n_rescal = p->n_rescal;
-    a.n_acc = p->n_acc;
+    a.acc_bins = p->acc_bins;
+    a.acc_bytes = p->acc_bytes;
     a.n_dets = (int)p->hdets.size();
     a.slot_ctr = (float *)p->slot_ctr.p;
     a.slot_iidx = (uint32_t *)p->slot_iidx.p;
@@ -1534,8 +1549,8 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
     }
     int nseg = (int)p->bounds.size() - 1;
     /* persistent grids: as many blocks as are co-resident */
-    unsigned grid_first = (unsigned)ctx->n_cu * blocks_per_cu(full, true, p->variant, wg);
-    unsigned grid_rest = (unsigned)ctx->n_cu * blocks_per_cu(full, false, p->variant, wg);
+    unsigned grid_first = (unsigned)ctx->n_cu * blocks_per_cu(full, true, p->variant, wg, p->acc_bytes);
+    unsigned grid_rest = (unsigned)ctx->n_cu * blocks_per_cu(full, false, p->variant, wg, p->acc_bytes);
     uint64_t max_waves = (uint64_t)std::max(grid_first, grid_rest) * (wg / 64);
     if (nseg > 1) {
         rc = ensure_queues(p, chunk + max_waves, full);
@@ -1551,6 +1566,8 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
     launch_fill_u64(s, p->cnt.p, CNT_N);
     HIPCHK(hipEventRecord(ctx->ev[0], s));
     uint64_t segs = 0;
+    bool plugin_timed = false;
+    uint64_t plugin_items = 0, plugin_nreg = 0;
     for (uint64_t ck = 0; ck < n_chunks; ck++) {
         uint64_t torun = std::min(chunk, n_hist - ck * chunk);
         a.num_sim = (int)torun;
@@ -1571,7 +1588,15 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
                 pa.meta = meta + 2;
                 pa.comp = (const DComp *)p->comps.p + p->plugin_step[a.s0];
                 void *args[] = {&pa};
+                const bool timed = ck == 0 && !plugin_timed;
+                if (timed) HIPCHK(hipEventRecord(ctx->ev[6], s));
                 HIPCHK(hipModuleLaunchKernel(pl->fn, (unsigned)ctx->n_cu * 8, 1, 1, 256, 1, 1, 0, s, args, nullptr));
+                if (timed) {
+                    HIPCHK(hipEventRecord(ctx->ev[7], s));
+                    HIPCHK(hipMemcpyAsync(&plugin_items, (const uint64_t *)p->prefix.p + plugin_nreg,
+                                          sizeof plugin_items, hipMemcpyDeviceToHost, s));
+                    plugin_timed = true;
+                }
                 segs++;
                 continue;
             }
@@ -1590,6 +1615,7 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
             cur = sg == 0 ? 0 : cur ^ 1;
             if (sg + 1 < nseg) {
                 uint64_t nreg = (uint64_t)grid * (wg / 64);
+                plugin_nreg = nreg;
                 hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, (const uint32_t *)p->rcounts.p,
                                    nreg, (const uint64_t *)meta, (uint64_t *)p->prefix.p, meta + 2,
                                    (unsigned long long *)p->cnt.p, sg == 0 ? 1 : 0);
@@ -1621,9 +1647,12 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
     p->stats.state_in = cnt[CNT_QIN];
     p->stats.ms_total = ms;
     p->stats.ms_source_segment = ms0;
-    float ms_ev = 0;
+    float ms_ev = 0, ms_pl = 0;
     if (p->n_rescal) HIPCHK(hipEventElapsedTime(&ms_ev, ctx->ev[4], ctx->ev[5]));
+    if (plugin_timed) HIPCHK(hipEventElapsedTime(&ms_pl, ctx->ev[6], ctx->ev[7]));
     p->stats.ms_rescal_append = ms_ev;
+    p->stats.ms_plugin = ms_pl;
+    p->stats.plugin_items = plugin_items;
     if (p->n_rescal)
         HIPCHK(hipMemcpy(p->ev_count.data(), p->ev_list_n.p, p->n_rescal * sizeof(uint64_t),
                          hipMemcpyDeviceToHost));

@@ -1,7 +1,7 @@
 """User-written scatter plugins on the CPU: the instrument file resolves
 the plugin class through its module, as mcramp.py:80-92 does, and the
 oracle runs the plugin's host build (the same source, g++) with the
-reference's per-neutron contract.  The plugin (tests/plugins/
+reference's per-neutron contract.  The plugin (ramp_amd/instruments/
 SAttenuatorPlugin.py) multiplies the weight by f + (1-f)u and kills with
 probability q: on the neutrons that reach it the kill fraction is q and the
 mean factor (1+f)/2 (closed forms)."""
@@ -15,8 +15,8 @@ import pytest
 from ramp_amd import Instrument, capi
 from tests.parity_util import ROOT
 
-PLUGINS = os.path.join(ROOT, "tests", "plugins")
-PATH = os.path.join(ROOT, "tests", "instruments", "plugin.json")
+PLUGINS = os.path.join(ROOT, "ramp_amd", "instruments")
+PATH = os.path.join(ROOT, "ramp_amd", "instruments", "plugin.json")
 _HOST = {}
 
 
