@@ -37,6 +37,7 @@ def instrument_cases():
         "mesh_multi": (os.path.join(ROOT, "tests", "instruments", "mesh_multi.json"),
                        {"inner": repr(m["inner"]), "outer": repr(m["outer"])}),
         "minimal": (os.path.join(ROOT, "ramp_amd", "instruments", "minimal.json"), {}),
+        "let_like": (os.path.join(ROOT, "ramp_amd", "instruments", "let_like.json"), {}),
         "guide_chopper": (os.path.join(ROOT, "tests", "instruments", "guide_chopper.json"),
                           {"v_foc": 841.3}),
         "powder": (os.path.join(ROOT, "tests", "instruments", "powder.json"), {"Ei": 25.0}),

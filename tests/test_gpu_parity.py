@@ -13,7 +13,7 @@ from tests.parity_util import compare_states, instrument_cases, run_gpu, run_ora
 CASES = instrument_cases()
 # allowed rows differing beyond 1e-5 (out of N) per case
 BUDGET = {"minimal": 0, "guide_chopper": 0, "powder": 0, "tas": 8, "imaging": 0, "mesh_multi": 0,
-          "powder_diff": 0, "isis_isotropic": 0, "polarisation": 0}
+          "powder_diff": 0, "isis_isotropic": 0, "polarisation": 0, "let_like": 0}
 N = 65536
 
 
@@ -128,3 +128,21 @@ def test_bvh_equals_gpu_brute_force_at_scale(case, n, tmp_path):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
     for (w1, q1), (w2, q2) in zip(hb, hf):
         assert np.array_equal(w1, w2) and np.array_equal(q1, q2)
+
+
+@pytest.mark.gpu
+def test_let_like_parity_1e6():
+    """The LET-like beamline (20 tapered guide segments, 5 choppers) at 1e6
+    histories: every neutron's guide bounces - positions, times, weights -
+    bit-identical to the oracle's (the guide's division-free wall choice
+    must make the reference's decisions)."""
+    path, kw = CASES["let_like"]
+    n, seeds = 1_000_000, [77]
+    print(f"case let_like N={n}", flush=True)
+    g, gh, st = run_gpu(path, kw, n, seeds)
+    o = run_oracle(path, kw, n, seeds)
+    d = compare_states(g, o["state"])
+    print(d, st, flush=True)
+    assert d["ctr"] == 0 and d["dead"] == 0 and d["iidx"] == 0
+    assert d["bitdiff_rows"] == 0
+    _hist_close(gh, o["hist"], 1e-6)
