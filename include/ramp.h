@@ -100,7 +100,8 @@ typedef struct {
     uint32_t comp_idx;   /* reference component index (mcramp.py:275,112) */
     uint32_t restore;    /* component-level restore_neutron (terminator.cl:17) */
     int32_t det;         /* histogram slot written by the scatter part, or -1 */
-    uint32_t pad;
+    uint32_t vis;        /* 1: drawn and traced (the component's 'visualise', default
+                            true; mcramp.py:67, 141, 159) */
     float pos[3];        /* component position (after 'relative' resolution) */
     float rot[3];        /* ZYX Euler angles in radians */
 } ramp_component_desc;
@@ -136,6 +137,9 @@ typedef struct {
 #define RAMP_RUN_MESH_BRUTE_FORCE 1  /* test every triangle of every GMesh (the
                                         reference's loop, mesh.cl:52-112) instead
                                         of traversing the BVH; for checking the BVH */
+#define RAMP_RUN_TRACE 2             /* record the trace points of every traced
+                                        component (ramp_trace_read); the run is
+                                        in full-state mode */
 
 typedef struct {
     uint32_t hist_mode;      /* RAMP_HIST_* */
@@ -203,6 +207,23 @@ int ramp_hist_reset(ramp_plan *plan);
 int ramp_state_read(ramp_plan *plan, float *neutrons16, float *isect8, uint32_t *iidx, uint64_t n);
 
 int ramp_stats_read(const ramp_plan *plan, ramp_run_stats *out);
+
+/* ---- trace -----------------------------------------------------------------
+ * Trace points of the last run made with RAMP_RUN_TRACE.  The traced
+ * components are those with vis = 1 in linear blocks, numbered in plan
+ * order from 0 (`slot`).  For every history h of the run (h = chunk *
+ * chunk_size + chunk-local index, n = the run's n_hist) `out` receives, as
+ * four arrays of n floats: the entry point (x1, y1, z1) of the intersection
+ * the component's geometry produced, in the component's frame, as it stood
+ * after terminate; and 1.0 where the neutron was alive after the
+ * component's scatter and untransform, else 0.0 (then the point is
+ * meaningless; histories that never reached the component read 0.0 too).
+ * Replaces the per-component copies of the intersection and neutron
+ * buffers that Instrument.execute(trace=True) makes (mcramp.py:141-161)
+ * and from which _get_trace_lines (:222-251) draws each surviving
+ * neutron's line; the host forms the lines.  Components in non-linear
+ * blocks are not traced (the reference keeps no trace positions there). */
+int ramp_trace_read(ramp_plan *plan, uint32_t slot, float *out, uint64_t n);
 
 /* ---- MCPL particle output -------------------------------------------------
  * Records of SMCPLOut component `comp` (its index among the plan's

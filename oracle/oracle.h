@@ -57,6 +57,14 @@ void oracle_set_rescal_out(float *events, uint64_t cap, uint64_t *counts);
  * *iidx, uint32_t comp_idx, const float *params, uint32_t tid). */
 void oracle_set_plugin(uint32_t comp, void *fn);
 
+/* Where trace points go (Instrument.execute(trace=True), mcramp.py:118-165,
+ * 222-251): points[slot][4][cap] = x1, y1, z1 of the intersection after
+ * terminate (component frame) and 1.0 if alive after the component, at the
+ * history's run-wide index, for the drawn (vis) components of linear blocks
+ * numbered in plan order.  Every history that reaches the component writes
+ * its entry, dead or alive.  NULL: no trace. */
+void oracle_set_trace(float *points, uint64_t cap);
+
 /* Test hook: the shared rotate-about-axis restatement (geom.h:1-19). */
 void oracle_rotate_axis(float angle, const float *axis, const float *v, float *out);
 

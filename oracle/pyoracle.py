@@ -42,6 +42,8 @@ def load():
         lib.oracle_set_plugin.argtypes = [C.c_uint32, C.c_void_p]
         lib.oracle_set_rescal_out.restype = None
         lib.oracle_set_rescal_out.argtypes = [C.POINTER(C.c_float), C.c_uint64, C.POINTER(C.c_uint64)]
+        lib.oracle_set_trace.restype = None
+        lib.oracle_set_trace.argtypes = [C.POINTER(C.c_float), C.c_uint64]
         lib.oracle_set_mcpl_dump.restype = None
         lib.oracle_set_mcpl_dump.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_uint32), C.c_uint64]
         lib.oracle_last_error.restype = C.c_char_p
@@ -51,15 +53,22 @@ def load():
     return _lib
 
 
-def run(inst, n_hist, seeds, chunk=0, keep_state=False, threads=0):
+def run(inst, n_hist, seeds, chunk=0, keep_state=False, threads=0, trace=False):
     """Run `inst` (a ramp_amd.Instrument) on the oracle.
 
     Returns dict: 'hist' list of (w, w2) float64 per detector slot, 'oob',
     with keep_state the final AoS state (n16, isect8, iidx) of the last
     chunk, and 'mcpl': per SMCPLOut component the records (n, 11) float32
-    it wrote in the last chunk, in history order."""
+    it wrote in the last chunk, in history order.  With trace, 'trace':
+    {component name: (xyz (n_hist, 3) float32, alive (n_hist,) bool)} for
+    the drawn components of linear blocks."""
     lib = load()
     desc = inst.plan_desc()
+    traced = [c.name for b in inst.blocks if b.linear for c in b.components.values() if c.vis]
+    tr = None
+    if trace and traced and n_hist:
+        tr = np.zeros((len(traced), 4, int(n_hist)), np.float32)
+        lib.oracle_set_trace(tr.ctypes.data_as(C.POINTER(C.c_float)), int(n_hist))
     n_out = sum(1 for c in inst._comp_list if type(c.scat).__name__ == "SMCPLOut")
     ch0 = int(chunk) if chunk else int(n_hist)
     dump = tags = None
@@ -98,6 +107,8 @@ def run(inst, n_hist, seeds, chunk=0, keep_state=False, threads=0):
                         i8.ctypes.data_as(C.POINTER(C.c_float)) if keep_state else None,
                         ii.ctypes.data_as(C.POINTER(C.c_uint32)) if keep_state else None,
                         int(threads), C.byref(oob))
+    if tr is not None:
+        lib.oracle_set_trace(None, 0)
     if n_out:
         lib.oracle_set_mcpl_dump(None, None, 0)
     if n_rs:
@@ -112,6 +123,9 @@ def run(inst, n_hist, seeds, chunk=0, keep_state=False, threads=0):
         out["mcpl"] = [dump[s][tags[s] == n_chunks] for s in range(n_out)]
     if n_rs:
         out["rescal"] = [rs_ev[r][:, :int(rs_cnt[r])] for r in range(n_rs)]
+    if tr is not None:
+        out["trace"] = {name: (np.ascontiguousarray(tr[k, :3].T), tr[k, 3] != 0.0)
+                        for k, name in enumerate(traced)}
     return out
 
 

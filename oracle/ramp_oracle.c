@@ -1642,6 +1642,45 @@ void oracle_set_plugin(uint32_t comp, void *fn)
     if (comp < ORACLE_MAX_PLUGINS) g_plugin[comp] = (oracle_plugin_fn)fn;
 }
 
+/* ======================= trace ========================================== */
+
+/* Instrument.execute(trace=True), mcramp.py:118-165 and 222-251: for each
+ * drawn component (vis) of a linear block, the intersection copied after
+ * terminate (its entry point x1, y1, z1 is the trace point, in the
+ * component frame) and whether the neutron is alive (s15 == 0) after the
+ * component.  Recorded as [slot][4][cap] at the history's run-wide index;
+ * slots number the drawn linear-block components in plan order. */
+static float *g_trace;
+static uint64_t g_trace_cap;
+static uint64_t g_trace_chunk;   /* chunk size of the running oracle_run */
+#define ORACLE_MAX_COMPS 4096
+static int32_t g_trace_slot[ORACLE_MAX_COMPS];
+
+void oracle_set_trace(float *points, uint64_t cap)
+{
+    g_trace = points;
+    g_trace_cap = cap;
+}
+
+static void trace_slots(const ramp_plan_desc *d)
+{
+    int32_t next = 0;
+    for (uint32_t k = 0; k < d->n_comps && k < ORACLE_MAX_COMPS; k++) g_trace_slot[k] = -1;
+    for (uint32_t b = 0; b < d->n_blocks; b++) {
+        if (!d->blocks[b].linear) continue;
+        for (uint32_t k = d->blocks[b].first; k < d->blocks[b].first + d->blocks[b].count; k++)
+            if (k < ORACLE_MAX_COMPS && d->comps[k].vis) g_trace_slot[k] = next++;
+    }
+}
+
+static float *trace_at(const octx *cx, const on_t *n, uint32_t comp)
+{
+    if (!g_trace || comp >= ORACLE_MAX_COMPS || g_trace_slot[comp] < 0) return NULL;
+    const uint64_t h = (uint64_t)cx->ck * g_trace_chunk + n->tid;
+    if (h >= g_trace_cap) return NULL;
+    return g_trace + (size_t)g_trace_slot[comp] * 4 * g_trace_cap + h;
+}
+
 /* ======================= host loop ====================================== */
 
 /* ExecutionBlock.execute: mcramp.py:118-220 */
@@ -1651,9 +1690,15 @@ static void o_block(octx *cx, on_t *n, const ramp_block_desc *b)
     if (b->linear) {
         for (uint32_t k = 0; k < b->count; k++) {
             const ramp_component_desc *c = cs + k;
+            float *tr = trace_at(cx, n, (uint32_t)(c - cx->d->comps));
             o_transform(n, c);
             o_geom(cx, n, c);
             o_terminate(n, c->restore);
+            if (tr) {
+                tr[0] = n->is[0];
+                tr[g_trace_cap] = n->is[1];
+                tr[2 * g_trace_cap] = n->is[2];
+            }
             if (c->scat.type == RAMP_S_MCPL_IN) o_mcpl_in(cx, n, &c->scat);
             else if (c->scat.type == RAMP_S_MCPL_OUT) o_mcpl_out(cx, n, &c->scat);
             else if (c->scat.type == RAMP_S_PLUGIN) {
@@ -1664,6 +1709,7 @@ static void o_block(octx *cx, on_t *n, const ramp_block_desc *b)
                     g_plugin[ci](n->s, n->is, &n->iidx, c->comp_idx, c->scat.p, n->tid);
             } else o_scatter(cx, n, c);
             o_untransform(n, c);
+            if (tr) tr[3 * g_trace_cap] = n->N_DIE == 0.0f ? 1.0f : 0.0f;
         }
     } else {
         for (uint32_t ev = 0; ev < b->multi; ev++) {
@@ -1736,6 +1782,8 @@ int oracle_run(const ramp_plan_desc *desc, uint64_t n_hist, uint64_t chunk, cons
             return RAMP_E_NOMEM;
         }
     }
+    trace_slots(desc);
+    g_trace_chunk = chunk;
     octx cx;
     cx.d = desc;
     cx.hw = hist_w;

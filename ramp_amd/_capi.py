@@ -51,7 +51,7 @@ class ComponentDesc(C.Structure):
         ("comp_idx", C.c_uint32),
         ("restore", C.c_uint32),
         ("det", C.c_int32),
-        ("pad", C.c_uint32),
+        ("vis", C.c_uint32),
         ("pos", C.c_float * 3),
         ("rot", C.c_float * 3),
     ]
@@ -144,6 +144,7 @@ _SIGS = {
     "ramp_state_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float),
                                   C.POINTER(C.c_uint32), C.c_uint64]),
     "ramp_stats_read": (C.c_int, [C.c_void_p, C.POINTER(RunStats)]),
+    "ramp_trace_read": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_float), C.c_uint64]),
     "ramp_mcpl_read": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_float), C.c_uint64,
                                  C.POINTER(C.c_uint64)]),
     "ramp_rescal_read": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_float), C.c_uint64,

@@ -341,6 +341,18 @@ class GBox(_Geom):
         self.idx = np.uint32(idx)
         self.p[S.GP_W], self.p[S.GP_H], self.p[S.GP_D] = self.width, self.height, self.depth
 
+    # corner walk that draws the twelve edges as one polyline (box.py:58-82)
+    _WALK = (0, 1, 2, 3, 0, 4, 5, 1, 2, 6, 5, 6, 7, 4, 0, 3, 7)
+
+    def lines(self):
+        w, h, d = self.width / 2.0, self.height / 2.0, self.depth / 2.0
+        # corners 0-3 on the z = -d face, 4-7 on z = +d, each face walked
+        # (-w, h), (w, h), (w, -h), (-w, -h)
+        face = [(-w, h), (w, h), (w, -h), (-w, -h)]
+        corners = [(x, y, -d) for x, y in face] + [(x, y, d) for x, y in face]
+        pts = [corners[k] for k in self._WALK]
+        return [[p[0] for p in pts], [p[1] for p in pts], [p[2] for p in pts]]
+
 
 class GSphere(_Geom):
     """geom/sphere.py:9-83"""
@@ -351,6 +363,25 @@ class GSphere(_Geom):
         self.radius = np.float32(radius)
         self.idx = np.uint32(idx)
         self.p[S.GP_R] = self.radius
+
+    def lines(self):
+        """Three great circles and a quarter arc as one polyline
+        (sphere.py:49-83): the x = 0 circle, back to its start, the z = 0
+        circle, back to the same start, a quarter of the z = 0 circle again,
+        then the y = 0 circle."""
+        r = self.radius
+        full = np.linspace(0, 2 * np.pi)
+        quarter = np.linspace(0, np.pi / 2)
+        zero = np.zeros_like(full)
+        parts = [
+            (zero, r * np.cos(full), r * np.sin(full)),
+            (np.zeros(1), np.array([r * np.cos(0.0)]), np.array([r * np.sin(0.0)])),
+            (r * np.sin(full), r * np.cos(full), zero),
+            (np.zeros(1), np.array([r * np.cos(0.0)]), np.array([r * np.sin(0.0)])),
+            (r * np.sin(quarter), r * np.cos(quarter), np.zeros_like(quarter)),
+            (r * np.cos(full), zero, r * np.sin(full)),
+        ]
+        return [list(np.concatenate([p[k] for p in parts])) for k in range(3)]
 
 
 class GSphereInt(GSphere):
@@ -368,6 +399,19 @@ class GCylinderExt(_Geom):
         self.idx = idx
         self.p[S.GP_R], self.p[S.GP_HEIGHT] = self.radius, self.height
 
+    def lines(self):
+        """The two half-cylinders x >= 0 and x <= 0, each as top rim, bottom
+        rim walked back, and the closing point (cylinder_ext.py:57-97)."""
+        r, h = self.radius, self.height / 2.0
+        xs, ys, zs = [], [], []
+        for sweep in (np.linspace(0, np.pi), np.linspace(0, -np.pi)):
+            for a_seq, y in ((sweep, h), (sweep[::-1], -h)):
+                xs += list(r * np.sin(a_seq))
+                zs += list(r * np.cos(a_seq))
+                ys += [y] * len(a_seq)
+            xs.append(xs[0]); ys.append(ys[0]); zs.append(zs[0])
+        return [xs, ys, zs]
+
 
 class GBanana(_Geom):
     """geom/banana.py:10-87"""
@@ -380,6 +424,20 @@ class GBanana(_Geom):
         self.idx = idx
         self.p[S.GP_R], self.p[S.GP_HEIGHT] = self.radius, self.height
         self.p[S.GP_MINTH], self.p[S.GP_MAXTH] = self.mintheta, self.maxtheta
+
+    def lines(self):
+        """The arc between mintheta and maxtheta (degrees) in 50 steps: top
+        rim walked back from the last step, bottom rim forward, closed
+        (banana.py:65-87)."""
+        n = 50
+        lo = np.radians(self.mintheta)
+        step = (np.radians(self.maxtheta) - lo) / float(n)
+        ang = [lo + i * step for i in range(n)]
+        top, bottom = ang[::-1], ang
+        xs = [self.radius * np.sin(a) for a in top + bottom]
+        zs = [self.radius * np.cos(a) for a in top + bottom]
+        ys = [self.height / 2.0] * n + [-self.height / 2.0] * n
+        return [xs + xs[:1], ys + ys[:1], zs + zs[:1]]
 
 
 def read_stl(filename):
@@ -513,6 +571,17 @@ class SGuide(_Scat):
                             S.GU_ALPHA, S.GU_M, S.GU_W), vals):
             self.p[slot] = np.float32(v)
         self.ni[S.GUI_MAXB] = self.max_bounces
+
+    def lines(self):
+        """Outline of the tapered guide, entrance (w1 x h1) at z = 0, exit
+        (w2 x h2) at z = l, as one polyline (guide.py:109-123)."""
+        a, b = self.w1 / 2, self.h1 / 2      # entrance half-sizes
+        c, d = self.w2 / 2, self.h2 / 2      # exit half-sizes
+        l = self.l
+        pts = [(-a, b, 0), (-c, d, l), (c, d, l), (a, b, 0), (-a, b, 0), (-a, -b, 0),
+               (a, -b, 0), (a, b, 0), (c, d, l), (c, -d, l), (a, -b, 0), (-a, -b, 0),
+               (-c, -d, l), (c, -d, l), (-c, -d, l), (-c, d, l)]
+        return [[p[0] for p in pts], [p[1] for p in pts], [p[2] for p in pts]]
 
 
 def _simps_avg(y, x, axis=-1):

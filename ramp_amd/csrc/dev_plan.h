@@ -34,6 +34,8 @@ struct DComp {
     uint32_t pad;            /* 1: zero Euler angles (identity up to signed zeros) */
     float pos[3];
     float R[9];
+    int32_t trace;           /* trace slot (RAMP_RUN_TRACE), or -1 */
+    uint32_t pad2;
 };
 
 /* one step of a segment: a linear-block component, or a whole non-linear

@@ -70,4 +70,23 @@ DEV uint64_t region_slot(const uint64_t *prefix, uint64_t nreg, uint64_t stride,
     return lo * stride + (v - prefix[lo]);
 }
 
+/* Trace points (full-state runs with RAMP_RUN_TRACE): for traced component
+ * `slot`, field-major [slot][4][cap] at the history's run-wide index h,
+ * the intersection's entry point (x1, y1, z1) in the component frame after
+ * terminate, and 1.0 if the neutron is alive after the component.  The
+ * buffer is zero-filled before the run, so histories that never reach the
+ * component read "not alive". */
+DEV void trace_entry(const NS &n, float *tr, uint64_t cap, uint64_t h, int slot)
+{
+    float *o = tr + (size_t)slot * 4 * cap + h;
+    o[0] = n.is[0];
+    o[cap] = n.is[1];
+    o[2 * cap] = n.is[2];
+}
+
+DEV void trace_alive(const NS &n, float *tr, uint64_t cap, uint64_t h, int slot)
+{
+    tr[(size_t)slot * 4 * cap + 3 * cap + h] = n.dead ? 0.0f : 1.0f;
+}
+
 #endif

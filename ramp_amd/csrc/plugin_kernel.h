@@ -16,16 +16,21 @@ struct PluginArgs {
     const uint64_t *prefix;  /* [nreg + 1] region prefix of the queue */
     const uint64_t *meta;    /* [0] region stride, [1] number of regions */
     const DComp *comp;
+    float *trace;            /* trace points (RAMP_RUN_TRACE), or null */
+    uint64_t trace_cap, trace_base;
 };
 
 typedef void (*ramp_scatter_fn)(RampNeutron *, RampIntersection *, uint32_t *, uint32_t, const float *,
                                 uint32_t);
 
-DEV void plugin_one(NS &n, const DComp &c, ramp_scatter_fn f)
+template <bool TRACE>
+DEV void plugin_one(NS &n, const DComp &c, ramp_scatter_fn f, const PluginArgs &a)
 {
     tr_in<true>(n, c);
     geom_dispatch<FEAT_ALL>(n, c);
     terminate(n, c.restore);
+    const bool traced = TRACE && c.trace >= 0;
+    if (traced) trace_entry(n, a.trace, a.trace_cap, a.trace_base + n.key, c.trace);
     RampNeutron r;
     r.x = n.x; r.y = n.y; r.z = n.z; r.vx = n.vx; r.vy = n.vy; r.vz = n.vz;
     r.px = n.px; r.py = n.py; r.pz = n.pz; r.p = n.w; r.tof = n.t; r.sb = n.ctr;
@@ -42,21 +47,34 @@ DEV void plugin_one(NS &n, const DComp &c, ramp_scatter_fn f)
     n.is[4] = is.x2; n.is[5] = is.y2; n.is[6] = is.z2; n.is[7] = is.t2;
     n.iidx = ii;
     tr_out<true>(n, c);
+    if (traced) trace_alive(n, a.trace, a.trace_cap, a.trace_base + n.key, c.trace);
+}
+
+/* the plugin's kernels: ramp_plugin_kernel, and ramp_plugin_kernel_trace
+ * for runs that record trace points */
+template <bool TRACE>
+DEV void plugin_queue(const PluginArgs &a, ramp_scatter_fn f)
+{
+    const uint64_t nreg = a.meta[1], stride = a.meta[0], total = a.prefix[nreg];
+    const DComp &c = *a.comp;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t slot = region_slot(a.prefix, nreg, stride, i);
+        NS n;
+        queue_load<true>(n, a.q, slot);
+        if (!n.dead) plugin_one<TRACE>(n, c, f, a);
+        queue_store<true>(n, a.q, slot);
+    }
 }
 
 #define RAMP_PLUGIN_KERNEL(entry)                                                          \
     extern "C" __global__ __launch_bounds__(256) void ramp_plugin_kernel(PluginArgs a)     \
     {                                                                                      \
-        const uint64_t nreg = a.meta[1], stride = a.meta[0], total = a.prefix[nreg];       \
-        const DComp &c = *a.comp;                                                          \
-        for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;      \
-             i += (uint64_t)gridDim.x * blockDim.x) {                                      \
-            const uint64_t slot = region_slot(a.prefix, nreg, stride, i);                  \
-            NS n;                                                                          \
-            queue_load<true>(n, a.q, slot);                                                \
-            if (!n.dead) plugin_one(n, c, entry);                                          \
-            queue_store<true>(n, a.q, slot);                                               \
-        }                                                                                  \
+        plugin_queue<false>(a, entry);                                                     \
+    }                                                                                      \
+    extern "C" __global__ __launch_bounds__(256) void ramp_plugin_kernel_trace(PluginArgs a) \
+    {                                                                                      \
+        plugin_queue<true>(a, entry);                                                      \
     }
 
 #endif

@@ -105,6 +105,10 @@ struct SegArgs {
      * the bytes after the deferral queues; the detector count */
     uint32_t acc_bins, acc_bytes;
     int n_dets;
+    /* trace points [slot][4][trace_cap] (full-state, RAMP_RUN_TRACE), or
+     * null; trace_base is the run-wide index of the chunk's first history */
+    float *trace;
+    uint64_t trace_cap, trace_base;
 };
 
 #include "dev_queue.h"
@@ -184,12 +188,16 @@ DEV void run_range(NS &n, const SegArgs &a, HistCtx &hc, int k0, int k1, DeferQ 
             tr_in<FULL>(n, c);
             geom_dispatch<F>(n, c);
             terminate(n, c.restore);
+            bool traced = false;
             if constexpr (FULL) {
+                traced = a.trace && !a.pilot && c.trace >= 0 && n.key != 0xffffffffu;
+                if (traced) trace_entry(n, a.trace, a.trace_cap, a.trace_base + n.key, c.trace);
                 /* plans holding MCPL components run in full-state mode only */
                 if (c.s.type == RAMP_S_MCPL_IN || c.s.type == RAMP_S_MCPL_OUT) {
                     if (c.s.type == RAMP_S_MCPL_IN) mcpl_in_step(n, c, (uint32_t)a.seed);
                     else if (!a.pilot) mcpl_out_step(n, c, a.dump, a.dump_tag, a.dump_cap, a.chunk_tag);
                     tr_out<FULL>(n, c);
+                    if (traced) trace_alive(n, a.trace, a.trace_cap, a.trace_base + n.key, c.trace);
                     continue;
                 }
             }
@@ -213,6 +221,9 @@ DEV void run_range(NS &n, const SegArgs &a, HistCtx &hc, int k0, int k1, DeferQ 
             }
             scat_dispatch<FULL, F>(n, c, hc);
             tr_out<FULL>(n, c);
+            if constexpr (FULL) {
+                if (traced) trace_alive(n, a.trace, a.trace_cap, a.trace_base + n.key, c.trace);
+            }
         } else {
             for (uint32_t ev = 0; ev < st.multi; ev++) {
                 for (uint32_t j = 0; j < st.count; j++) {
@@ -555,7 +566,8 @@ struct ramp_ctx {
 struct ramp_plugin {
     ramp_ctx *ctx = nullptr;
     hipModule_t mod = nullptr;
-    hipFunction_t fn = nullptr;
+    hipFunction_t fn = nullptr;        /* ramp_plugin_kernel */
+    hipFunction_t fn_trace = nullptr;  /* ramp_plugin_kernel_trace (runs with RAMP_RUN_TRACE) */
 };
 
 struct DevBuf {
@@ -593,6 +605,10 @@ static const uint32_t kVariants[] = {0u, FEAT_SAMPLE, FEAT_MONO, FEAT_MESH, FEAT
                                      FEAT_ALL};
 static const int kNumVariants = 6;
 
+/* the largest trace buffer a run may allocate (16 B per traced component
+ * and history) */
+static const double TRACE_MAX_BYTES = 32.0 * 1073741824.0;
+
 struct ramp_plan {
     ramp_ctx *ctx = nullptr;
     std::vector<DComp> hcomps;
@@ -618,6 +634,9 @@ struct ramp_plan {
     bool need_full = false;   /* MCPL, SRescal or SAnalyser present: full-state execution */
     uint32_t n_rescal = 0;    /* SRescal components (event slots) */
     uint32_t acc_bins = 0, acc_bytes = 0; /* small detectors' LDS copy (set per run by the mode) */
+    uint32_t n_trace = 0;     /* traced components (vis = 1 in linear blocks) */
+    DevBuf trace;             /* their trace points [slot][4][trace_n] */
+    uint64_t trace_n = 0;     /* histories of the last traced run (0: the last run was not traced) */
     std::vector<int> plugin_step;           /* per step: the component number of a plugin step, else -1 */
     std::vector<ramp_plugin *> plugin_of;   /* per component: the bound plugin */
     std::vector<uint32_t> rescal_comp; /* their component numbers */
@@ -1079,6 +1098,12 @@ extern "C" int ramp_plan_create(ramp_ctx *ctx, const ramp_plan_desc *d, ramp_pla
             p->hsteps.push_back(s);
         }
     }
+    /* trace slots: components drawn (vis) in linear blocks, in plan order */
+    for (DComp &c : p->hcomps) c.trace = -1;
+    for (uint32_t b = 0; b < d->n_blocks; b++)
+        if (d->blocks[b].linear)
+            for (uint32_t k = d->blocks[b].first; k < d->blocks[b].first + d->blocks[b].count; k++)
+                if (d->comps[k].vis) p->hcomps[k].trace = (int32_t)p->n_trace++;
     if (p->hsteps.size() > CNT_N - CNT_NSTEP0)
         PCHK(fail(RAMP_E_RANGE, "too many steps (%zu)", p->hsteps.size()));
     for (size_t k = 0; k < p->hsteps.size() && p->defer_k < 0; k++)
@@ -1376,6 +1401,8 @@ struct PluginArgs {  /* the kernel argument of ramp_plugin_kernel (plugin_kernel
     const uint64_t *prefix;
     const uint64_t *meta;
     const DComp *comp;
+    float *trace;
+    uint64_t trace_cap, trace_base;
 };
 
 extern "C" int ramp_plugin_compile(ramp_ctx *ctx, const char *source, const char *entry, ramp_plugin **out)
@@ -1414,6 +1441,7 @@ extern "C" int ramp_plugin_compile(ramp_ctx *ctx, const char *source, const char
     pl->ctx = ctx;
     hipError_t e = hipModuleLoadData(&pl->mod, code.data());
     if (e == hipSuccess) e = hipModuleGetFunction(&pl->fn, pl->mod, "ramp_plugin_kernel");
+    if (e == hipSuccess) e = hipModuleGetFunction(&pl->fn_trace, pl->mod, "ramp_plugin_kernel_trace");
     if (e != hipSuccess) {
         if (pl->mod) (void)hipModuleUnload(pl->mod);
         delete pl;
@@ -1455,9 +1483,14 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
     if (opts) o = *opts;
     if (o.hist_mode != RAMP_HIST_ATOMIC && o.hist_mode != RAMP_HIST_DETERMINISTIC)
         return fail(RAMP_E_INVALID, "unknown histogram mode %u", o.hist_mode);
-    if (o.flags & ~(uint32_t)RAMP_RUN_MESH_BRUTE_FORCE)
+    if (o.flags & ~(uint32_t)(RAMP_RUN_MESH_BRUTE_FORCE | RAMP_RUN_TRACE))
         return fail(RAMP_E_INVALID, "unknown run flags 0x%x", o.flags);
     if (n_hist == 0) return 0;
+    const bool trace = (o.flags & RAMP_RUN_TRACE) != 0 && p->n_trace > 0;
+    if (trace && (double)p->n_trace * 16.0 * (double)n_hist > TRACE_MAX_BYTES)
+        return fail(RAMP_E_RANGE, "a trace of %u components x %llu histories needs %.3g GB (limit %.3g GB)",
+                    p->n_trace, (unsigned long long)n_hist, (double)p->n_trace * 16.0 * (double)n_hist * 1e-9,
+                    TRACE_MAX_BYTES * 1e-9);
     if (chunk == 0 || chunk > n_hist) chunk = n_hist;
     if (chunk > 0x7fffffffull) return fail(RAMP_E_RANGE, "chunk %llu exceeds 2^31-1", (unsigned long long)chunk);
     uint64_t n_chunks = (n_hist + chunk - 1) / chunk;
@@ -1476,7 +1509,7 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
     const bool keep = o.keep_state != 0;
     /* the MCPL steps need the polarisation and the whole intersection,
      * SRescal's filter the detected flag */
-    const bool full = keep || p->need_full;
+    const bool full = keep || p->need_full || trace;
     int rc = prepare_dets(p, (int)o.hist_mode, o.weight_bound, n_hist, chunk);
     if (rc) return rc;
     bool carry = n_chunks > 1;
@@ -1505,6 +1538,12 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
     if (p->n_rescal) {
         rc = rescal_prepare(p, chunk, n_hist, s);
         if (rc) return rc;
+    }
+    p->trace_n = 0;
+    if (trace) {
+        rc = dev_alloc(p->trace, (uint64_t)p->n_trace * 4 * n_hist * sizeof(float));
+        if (rc) return rc;
+        launch_fill_f32(s, p->trace.p, (uint64_t)p->n_trace * 4 * n_hist);
     }
     SegArgs a;
     std::memset(&a, 0, sizeof a);
@@ -1543,6 +1582,8 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
     a.st16 = (float *)p->st16.p;
     a.st8 = (float *)p->st8.p;
     a.stii = (uint32_t *)p->stii.p;
+    a.trace = trace ? (float *)p->trace.p : nullptr;
+    a.trace_cap = n_hist;
     if (!p->bounds_set || (o.max_segments && (int)p->bounds.size() - 1 > (int)o.max_segments)) {
         rc = choose_bounds(p, a, chunk, wg, (int)o.max_segments);
         if (rc) return rc;
@@ -1573,6 +1614,7 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
         a.num_sim = (int)torun;
         a.seed = (int)seeds[ck];
         a.n = torun;
+        a.trace_base = ck * chunk;
         a.chunk_tag = ++p->dump_chunk_tag ? p->dump_chunk_tag : ++p->dump_chunk_tag;
         if (p->n_rescal) launch_fill_f32(s, p->ev.p, (uint64_t)p->n_rescal * RS_NEV * p->ev_cap);
         int cur = 0; /* the queue holding the live neutrons */
@@ -1587,10 +1629,14 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
                 pa.prefix = (const uint64_t *)p->prefix.p;
                 pa.meta = meta + 2;
                 pa.comp = (const DComp *)p->comps.p + p->plugin_step[a.s0];
+                pa.trace = a.trace;
+                pa.trace_cap = a.trace_cap;
+                pa.trace_base = a.trace_base;
                 void *args[] = {&pa};
                 const bool timed = ck == 0 && !plugin_timed;
                 if (timed) HIPCHK(hipEventRecord(ctx->ev[6], s));
-                HIPCHK(hipModuleLaunchKernel(pl->fn, (unsigned)ctx->n_cu * 8, 1, 1, 256, 1, 1, 0, s, args, nullptr));
+                HIPCHK(hipModuleLaunchKernel(a.trace ? pl->fn_trace : pl->fn, (unsigned)ctx->n_cu * 8, 1, 1, 256,
+                                             1, 1, 0, s, args, nullptr));
                 if (timed) {
                     HIPCHK(hipEventRecord(ctx->ev[7], s));
                     HIPCHK(hipMemcpyAsync(&plugin_items, (const uint64_t *)p->prefix.p + plugin_nreg,
@@ -1657,7 +1703,23 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
         HIPCHK(hipMemcpy(p->ev_count.data(), p->ev_list_n.p, p->n_rescal * sizeof(uint64_t),
                          hipMemcpyDeviceToHost));
     p->st_n = keep ? std::min(chunk, n_hist - (n_chunks - 1) * chunk) : 0;
+    p->trace_n = trace ? n_hist : 0;
     p->dump_n = std::min(chunk, n_hist - (n_chunks - 1) * chunk);
+    return 0;
+}
+
+extern "C" int ramp_trace_read(ramp_plan *p, uint32_t slot, float *out, uint64_t n)
+{
+    if (!p || !out) return fail(RAMP_E_INVALID, "null argument");
+    if (p->trace_n == 0) return fail(RAMP_E_STATE, "no trace: run with RAMP_RUN_TRACE first");
+    if (slot >= p->n_trace) return fail(RAMP_E_INVALID, "trace slot %u of %u", slot, p->n_trace);
+    if (n != p->trace_n)
+        return fail(RAMP_E_INVALID, "the traced run had %llu histories, asked %llu",
+                    (unsigned long long)p->trace_n, (unsigned long long)n);
+    HIPCHK(hipSetDevice(p->ctx->device));
+    HIPCHK(hipStreamSynchronize(p->ctx->stream));
+    HIPCHK(hipMemcpy(out, (const float *)p->trace.p + (size_t)slot * 4 * n, 4 * n * sizeof(float),
+                     hipMemcpyDeviceToHost));
     return 0;
 }
 

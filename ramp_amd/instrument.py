@@ -46,6 +46,20 @@ def frame_rotate(vec, rot):
             np.dot([r20, r21, r22], vec)]
 
 
+def _rotate_points(pts, rot):
+    """frame_rotate (mcramp.py:457-476) applied to an (n, 3) float32 array
+    of points: the matrix formed from the float32 angles as frame_rotate
+    forms it, each component summed in float32 in order x, y, z."""
+    x_a, y_a, z_a = np.float32(rot[0]), np.float32(rot[1]), np.float32(rot[2])
+    rows = frame_rotate(np.eye(3, dtype=np.float32), [x_a, y_a, z_a])
+    R = np.asarray(rows, dtype=np.float32)
+    x, y, z = pts[:, 0], pts[:, 1], pts[:, 2]
+    out = np.empty_like(pts)
+    for i in range(3):
+        out[:, i] = (R[i, 0] * x + R[i, 1] * y) + R[i, 2] * z
+    return out
+
+
 def _resolve(name):
     """Class by name: the package namespace first, then a module of that
     name (mcramp.py:80-92)."""
@@ -76,6 +90,7 @@ class ExecutionBlock:
         self.components = components
         self.linear = linear
         self.max_events = max_events
+        self.trace_lines = []
 
     @staticmethod
     def fromJSON(block, parent, linear, events, bidx):
@@ -170,6 +185,7 @@ class Instrument:
         self._desc_keep = None
         self._seed_rng = np.random.default_rng(seed) if seed is not None else None
         self.last_stats = None
+        self._trace_pts = OrderedDict()
 
     # ---- JSON --------------------------------------------------------------
     def _substitute_params(self, json_str, **kwargs):
@@ -241,6 +257,7 @@ class Instrument:
             c.scat.fill(d.scat, pool)
             d.comp_idx = int(c.idx)
             d.restore = 1 if c.restore else 0
+            d.vis = 1 if c.vis else 0
             if getattr(c.scat, "n_bins", 0) > 0:
                 d.det = len(det_list)
                 self._det_of[k] = d.det
@@ -328,7 +345,19 @@ class Instrument:
         per-neutron state of the last chunk), `mesh_brute_force` (test every
         triangle of every GMesh as the reference does instead of traversing
         the BVH; for checking the BVH), `wg`/`npt` (launch shape of the
-        source segment; 0 = library default)."""
+        source segment; 0 = library default).
+
+        trace=True appends, for every neutron alive after each drawn
+        component of a linear block, the line from its previous point to
+        the component's intersection entry point in the lab frame to that
+        block's trace_lines (mcramp.py:118-165, 222-251); the points are
+        recorded on the GPU (RAMP_RUN_TRACE)."""
+        if trace:
+            for b in self.blocks:
+                if not b.linear and any(c.vis for c in b.components.values()):
+                    raise ValueError("trace: a non-linear block holds drawn components; the "
+                                     "reference keeps trace positions for linear blocks only "
+                                     "(mcramp.py:118-122, 209-211); set 'visualise': false on them")
         lib = self._ensure_plan()
         N = int(N)
         chunk = self.chunking(N, buf_max_mod)
@@ -341,7 +370,8 @@ class Instrument:
                           else capi.RAMP_HIST_ATOMIC)
         opts.keep_state = 1 if (keep_state or debug) else 0
         opts.weight_bound = self.weight_bound(chunk) if N else 0.0
-        opts.flags = capi.RAMP_RUN_MESH_BRUTE_FORCE if mesh_brute_force else 0
+        opts.flags = ((capi.RAMP_RUN_MESH_BRUTE_FORCE if mesh_brute_force else 0)
+                      | (capi.RAMP_RUN_TRACE if trace else 0))
         opts.wg = int(wg)
         opts.npt = int(npt)
         capi.check(lib.ramp_run(self._plan, N, chunk,
@@ -356,10 +386,61 @@ class Instrument:
             # last chunk's records are what remains
             for slot, c in enumerate(self._mcpl_out):
                 c.scat._collect(lib, self._plan, slot)
+        if trace and N:
+            self._trace_lines(lib, N, chunk)
         if debug:
             n16, _, _ = self.state(min(chunk, N))
             live = n16[n16[:, 15] == 0.0]
             print(live)
+
+    def trace_points(self):
+        """Trace points of the last traced run (extension): {component
+        name: (xyz, alive)} with xyz the intersection entry point in the
+        component frame, float32 of shape (N, 3), and alive a bool array of
+        shape (N,), for every drawn component of a linear block."""
+        return dict(self._trace_pts)
+
+    def _trace_lines(self, lib, N, chunk):
+        """Read the GPU trace points and form the reference's trace lines
+        (_get_trace_lines, mcramp.py:222-251): per chunk and per linear
+        block, every neutron starts at the origin (prev_locs is zeroed when
+        the block starts, :121-122); after each drawn component each live
+        neutron gets a line from its previous point to the lab-frame entry
+        point frame_rotate(x1, y1, z1) + pos, in index order, and that
+        point becomes its previous point."""
+        pts = OrderedDict()
+        slot = 0
+        for b in self.blocks:
+            if not b.linear:
+                continue
+            for c in b.components.values():
+                if not c.vis:
+                    continue
+                buf = np.empty(4 * N, dtype=np.float32)
+                capi.check(lib.ramp_trace_read(self._plan, slot, buf.ctypes.data_as(C.POINTER(C.c_float)),
+                                               N))
+                buf = buf.reshape(4, N)
+                pts[c.name] = (np.ascontiguousarray(buf[:3].T), buf[3] != 0.0)
+                slot += 1
+        self._trace_pts = pts
+        for lo in range(0, N, chunk):
+            hi = min(N, lo + chunk)
+            for b in self.blocks:
+                if not b.linear:
+                    continue
+                prev = np.zeros((hi - lo, 3), dtype=np.float32)
+                for c in b.components.values():
+                    if not c.vis:
+                        continue
+                    xyz, alive = pts[c.name]
+                    idx = np.nonzero(alive[lo:hi])[0]
+                    loc = xyz[lo:hi][idx]
+                    lab = _rotate_points(loc, c.rot) + c.pos
+                    p0 = prev[idx].tolist()
+                    p1 = lab.tolist()
+                    b.trace_lines.extend([[a[0], q[0]], [a[1], q[1]], [a[2], q[2]]]
+                                         for a, q in zip(p0, p1))
+                    prev[idx] = lab
 
     def _pull_hists(self):
         if self._plan is None:
@@ -432,8 +513,15 @@ class Instrument:
         for c in self.kernel_refs:
             c.scat.save(self.queue)
 
-    def visualise(self, *a, **k):
-        raise NotImplementedError("geometry visualisation is not part of the MI355X path")
+    def visualise(self, controls=True, xlim=None, ylim=None, zlim=None, focus=None, **kwargs):
+        """Figure of the instrument's drawn components and the trace lines of
+        traced runs (mcramp.py:317-325); returns the Visualisation
+        (extension) after showing it."""
+        from .visualisation import Visualisation
+        vis = Visualisation(self, controls=controls, xlim=xlim, ylim=ylim, zlim=zlim, focus=focus,
+                            **kwargs)
+        vis.show()
+        return vis
 
     def close(self):
         lib = capi.load_library() if (self._plan is not None or self._gctx is not None) else None

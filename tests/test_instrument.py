@@ -119,8 +119,6 @@ def test_unknown_and_out_of_scope_components(tmp_path):
     bad = IDF.replace('"name" : "SArm" } },\n        "skipped"', '"name" : "SNoSuchThing" } },\n        "skipped"')
     with pytest.raises(ModuleNotFoundError):
         make(tmp_path, bad, Ei=10.0, ang=0.25)
-    with pytest.raises(NotImplementedError):
-        make(tmp_path, IDF, Ei=10.0, ang=0.25).visualise()
     with pytest.raises(ValueError):
         ramp_amd.SIsotropic()
 

@@ -38,14 +38,14 @@ def host_plugin():
     return M, _HOST["fn"]
 
 
-def oracle_run(inst, n, seeds, chunk=0):
+def oracle_run(inst, n, seeds, chunk=0, trace=False):
     from oracle import pyoracle
     _, fn = host_plugin()
     inst.plan_desc()
     for k, c in enumerate(inst._comp_list):
         if c.scat.type_code == capi.RAMP_S_PLUGIN:
             pyoracle.set_plugin(k, fn)
-    return pyoracle.run(inst, n, seeds, chunk=chunk, keep_state=True)
+    return pyoracle.run(inst, n, seeds, chunk=chunk, keep_state=True, trace=trace)
 
 
 def test_plugin_resolves_by_module_name():
