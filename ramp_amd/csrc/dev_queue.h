@@ -57,17 +57,42 @@ DEV void queue_store(const NS &n, const DQueue &q, uint64_t i)
     }
 }
 
-/* map a dense input index to its slot in the previous segment's wave
- * regions: the region is the last one whose prefix is <= v */
-DEV uint64_t region_slot(const uint64_t *prefix, uint64_t nreg, uint64_t stride, uint64_t v)
+/* Map dense input indices to slots in the previous segment's wave
+ * regions (region r holds inputs [prefix[r], prefix[r+1]) at slots
+ * r * stride + ...).  A region may be empty; the region of v is the last
+ * one whose prefix is <= v. */
+
+/* the region of v, given prefix[lo] <= v < prefix[hi] */
+DEV uint64_t region_find(const uint64_t *prefix, uint64_t lo, uint64_t hi, uint64_t v)
 {
-    uint64_t lo = 0, hi = nreg;  /* prefix[lo] <= v < prefix[hi] */
     while (hi - lo > 1) {
         uint64_t mid = (lo + hi) >> 1;
         if (prefix[mid] <= v) lo = mid;
         else hi = mid;
     }
-    return lo * stride + (v - prefix[lo]);
+    return lo;
+}
+
+/* the region of v from a region r with prefix[r] <= v: steps of 1, 2,
+ * 4 ... until past v, then bisection (prefix[nreg] is the total, > v) */
+DEV uint64_t region_advance(const uint64_t *prefix, uint64_t nreg, uint64_t r, uint64_t v)
+{
+    uint64_t step = 1;
+    while (r + step < nreg && prefix[r + step] <= v) step <<= 1;
+    return region_find(prefix, r + (step >> 1), r + step < nreg ? r + step : nreg, v);
+}
+
+/* A wave walks its inputs in order, 64 at a time: `cur` (wave-uniform)
+ * is the region of the group's first input `first`, advanced from the
+ * previous group's; each lane then looks up its own input v among the few
+ * regions up to the group's last input `last`. */
+DEV uint64_t region_slot_group(const uint64_t *prefix, uint64_t nreg, uint64_t stride, uint64_t &cur,
+                               uint64_t first, uint64_t last, uint64_t v)
+{
+    cur = region_advance(prefix, nreg, cur, first);
+    const uint64_t r1 = region_advance(prefix, nreg, cur, last);
+    const uint64_t r = region_find(prefix, cur, r1 + 1, v);
+    return r * stride + (v - prefix[r]);
 }
 
 /* Trace points (full-state runs with RAMP_RUN_TRACE): for traced component

@@ -57,13 +57,22 @@ DEV void plugin_queue(const PluginArgs &a, ramp_scatter_fn f)
 {
     const uint64_t nreg = a.meta[1], stride = a.meta[0], total = a.prefix[nreg];
     const DComp &c = *a.comp;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-         i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t slot = region_slot(a.prefix, nreg, stride, i);
-        NS n;
-        queue_load<true>(n, a.q, slot);
-        if (!n.dead) plugin_one<TRACE>(n, c, f, a);
-        queue_store<true>(n, a.q, slot);
+    /* wave g of G takes the inputs [g * per, (g + 1) * per) in order */
+    const uint64_t G = (uint64_t)gridDim.x * (blockDim.x >> 6);
+    const uint64_t g = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const uint64_t per = (total + G - 1) / G;
+    const uint64_t beg = g * per, end = beg + per < total ? beg + per : total;
+    uint64_t rcur = 0;
+    for (uint64_t base = beg; base < end; base += 64) {
+        const uint64_t i = base + __lane_id();
+        const uint64_t last = base + 63 < end ? base + 63 : end - 1;
+        if (i < end) {
+            const uint64_t slot = region_slot_group(a.prefix, nreg, stride, rcur, base, last, i);
+            NS n;
+            queue_load<true>(n, a.q, slot);
+            if (!n.dead) plugin_one<TRACE>(n, c, f, a);
+            queue_store<true>(n, a.q, slot);
+        }
     }
 }
 

@@ -386,9 +386,11 @@ void k_segment(SegArgs a)
         defer = !a.pilot && a.defer_k >= a.s0 && a.defer_k < a.s1;
         if (defer) defer_init(dq, seg_lds);
     }
+    uint64_t rcur = 0; /* region of the group's first input (later segments) */
     for (uint64_t base = beg; base < end; base += 64) {
         uint64_t i = base + lane;
         bool valid = i < end;
+        const uint64_t last = base + 63 < end ? base + 63 : end - 1;
         NS n;
         if (valid) {
             if (FIRST) {
@@ -397,7 +399,7 @@ void k_segment(SegArgs a)
                 n.key = (uint32_t)i;
                 src_generate<FULL, F>(n, *a.src, a.num_sim, a.seed, sb_old);
             } else {
-                queue_load<FULL>(n, a.in, region_slot(a.in_prefix, nreg, stride, i));
+                queue_load<FULL>(n, a.in, region_slot_group(a.in_prefix, nreg, stride, rcur, base, last, i));
             }
         } else {
             n.dead = true;
