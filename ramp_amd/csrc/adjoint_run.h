@@ -76,19 +76,19 @@ __global__ __launch_bounds__(256) void k_adjoint(SegArgs a, AdjArgs x)
             float sb_old = a.carry ? a.slot_ctr[i] : 0.0f;
             n.iidx = a.carry ? a.slot_iidx[i] : 0u;
             n.key = (uint32_t)i;
-            src_generate<false, F>(n, *a.src, a.num_sim, a.seed, sb_old);
+            src_generate<false, F>(n, plan_at(a.src, 0), a.num_sim, a.seed, sb_old);
         } else {
             n.dead = true;
             n.pend = false;
             n.key = 0;
             n.iidx = 0;
         }
-        adj_history_reset(hc, valid ? source_dlogw_dT(n, *a.src) : 0.0f);
+        adj_history_reset(hc, valid ? source_dlogw_dT(n, plan_at(a.src, 0)) : 0.0f);
         for (int k = 0; k < ns; k++) {
             if (!__any(!n.dead)) break;
-            const DStep st = a.steps[k];
+            const DStep st = plan_at(a.steps, k);
             if (st.kind == 0) {
-                const DComp &c = a.comps[st.first];
+                const DComp &c = plan_at(a.comps, st.first);
                 tr_in<false>(n, c);
                 geom_dispatch<F>(n, c);
                 terminate(n, c.restore);
@@ -97,13 +97,13 @@ __global__ __launch_bounds__(256) void k_adjoint(SegArgs a, AdjArgs x)
             } else {
                 for (uint32_t ev = 0; ev < st.multi; ev++) {
                     for (uint32_t j = 0; j < st.count; j++) {
-                        const DComp &c = a.comps[st.first + j];
+                        const DComp &c = plan_at(a.comps, st.first + j);
                         tr_in<false>(n, c);
                         geom_dispatch<F>(n, c);
                         tr_out<false>(n, c);
                     }
                     for (uint32_t j = 0; j < st.count; j++) {
-                        const DComp &c = a.comps[st.first + j];
+                        const DComp &c = plan_at(a.comps, st.first + j);
                         tr_in<false>(n, c);
                         scat_dispatch<false, F>(n, c, hc);
                         tr_out<false>(n, c);

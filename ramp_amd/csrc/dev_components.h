@@ -26,6 +26,17 @@
 
 #define DEV __device__ static inline __attribute__((always_inline))
 
+/* The plan (components, steps, source, detectors) is written by the host
+ * before a launch and only read by the kernels.  Reading it through the
+ * constant address space tells the compiler so, and it then loads the
+ * wave-uniform plan fields with scalar loads into SGPRs instead of
+ * per-lane vector loads into VGPRs. */
+template <class T>
+DEV const T &plan_at(const T *p, size_t i)
+{
+    return *(const T *)((const __attribute__((address_space(4))) T *)p + i);
+}
+
 /* feature sets compiled into a segment-kernel instance: component types
  * outside the set are compiled out, which keeps register use (occupancy)
  * at what the instrument actually needs */
@@ -451,7 +462,7 @@ struct HistCtx {
 DEV void hist_add(HistCtx &hc, int det, uint64_t idx, float w)
 {
     if (!hc.enabled) return;
-    const DDet &d = hc.dets[det];
+    const DDet &d = plan_at(hc.dets, det);
     if (idx >= d.n) {
         hc.oob++;
         return;
@@ -497,7 +508,7 @@ DEV void hist_acc_end(const HistCtx &hc, int n_det, uint32_t bins)
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < bins; i += blockDim.x) {
         for (int k = 0; k < n_det; k++) {
-            const DDet &d = hc.dets[k];
+            const DDet &d = plan_at(hc.dets, k);
             if (d.acc < 0 || i < (uint32_t)d.acc || i >= (uint32_t)d.acc + d.n) continue;
             const uint64_t b = i - (uint32_t)d.acc;
             if (hc.mode == RAMP_HIST_DETERMINISTIC) {

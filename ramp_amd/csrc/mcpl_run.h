@@ -12,16 +12,19 @@
  * cross PCIe.  Included by ramp_gpu.hip ahead of ramp_run.
  */
 
-/* waves of a compaction pass: one per SEL_PER_WAVE candidates, so that
- * several rounds of waves keep HBM busy; the scan takes up to 2^20 counts.
- * A wave takes SEL_GROUP = 256 candidates per step, four per lane. */
-#define SEL_PER_WAVE 2048u
-#define SEL_WAVES_MAX (1u << 20)
+/* waves of a compaction pass: the waves of its writing kernel that fit on
+ * the GPU at once (a persistent grid; more would leave a partly filled last
+ * round of blocks), but no more than one per SEL_GROUP candidates.  A wave
+ * takes SEL_GROUP = 256 candidates per step, four per lane. */
 #define SEL_GROUP 256u
 
-static uint32_t sel_waves(uint64_t n)
+static uint32_t sel_waves(const ramp_ctx *ctx, const void *kernel, uint64_t n)
 {
-    return (uint32_t)std::min<uint64_t>(SEL_WAVES_MAX, std::max<uint64_t>(1, (n + SEL_PER_WAVE - 1) / SEL_PER_WAVE));
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, 256, 0) != hipSuccess || nb < 1) nb = 1;
+    const uint64_t resident = (uint64_t)ctx->n_cu * (uint64_t)nb * 4;
+    const uint64_t need = (n + SEL_GROUP - 1) / SEL_GROUP;
+    return (uint32_t)std::max<uint64_t>(1, std::min(resident, need));
 }
 
 /* waves over the n tags of one slot; each takes a contiguous range */
@@ -169,7 +172,7 @@ __global__ void k_ev_advance(unsigned long long *list_n, const uint64_t *total)
 /* the append of one chunk's events, for every SRescal slot (stream order) */
 static int rescal_append(ramp_plan *p, uint64_t n, uint32_t tag, hipStream_t s)
 {
-    const uint32_t G = sel_waves(n);
+    const uint32_t G = sel_waves(p->ctx, (const void *)k_ev_append, n);
     int rc = dev_alloc(p->sel_counts, G * sizeof(uint32_t));
     if (!rc) rc = dev_alloc(p->sel_prefix, (G + 1) * sizeof(uint64_t));
     if (rc) return rc;
@@ -264,7 +267,7 @@ extern "C" int ramp_mcpl_read(ramp_plan *p, uint32_t comp, float *records, uint6
     HIPCHK(hipSetDevice(p->ctx->device));
     hipStream_t s = p->ctx->stream;
     const uint64_t n = p->dump_n;
-    const uint32_t G = sel_waves(n);
+    const uint32_t G = sel_waves(p->ctx, (const void *)k_sel_write, n);
     int rc = dev_alloc(p->sel_counts, G * sizeof(uint32_t));
     if (!rc) rc = dev_alloc(p->sel_prefix, (G + 1) * sizeof(uint64_t));
     if (rc) return rc;

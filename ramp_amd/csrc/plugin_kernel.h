@@ -56,7 +56,7 @@ template <bool TRACE>
 DEV void plugin_queue(const PluginArgs &a, ramp_scatter_fn f)
 {
     const uint64_t nreg = a.meta[1], stride = a.meta[0], total = a.prefix[nreg];
-    const DComp &c = *a.comp;
+    const DComp &c = plan_at(a.comp, 0);
     /* wave g of G takes the inputs [g * per, (g + 1) * per) in order */
     const uint64_t G = (uint64_t)gridDim.x * (blockDim.x >> 6);
     const uint64_t g = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);

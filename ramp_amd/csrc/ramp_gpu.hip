@@ -176,15 +176,25 @@ DEV void defer_load(NS &n, const float *b, uint32_t i)
 
 /* Steps [k0, k1); with defer, the step defer_k parks reflecting neutrons
  * in dq (they leave the range dead with pend set). */
-template <bool FULL, uint32_t F>
+/* the plan through scalar loads (plan_at), or through vector loads where
+ * the SGPRs that takes would push a kernel into scratch (the batched
+ * monochromator kernel) */
+template <bool SP, class T>
+DEV const T &plan_get(const T *p, size_t i)
+{
+    if constexpr (SP) return plan_at(p, i);
+    else return p[i];
+}
+
+template <bool FULL, uint32_t F, bool SP = true>
 DEV void run_range(NS &n, const SegArgs &a, HistCtx &hc, int k0, int k1, DeferQ &dq, bool defer)
 {
     for (int k = k0; k < k1; k++) {
         /* SMCPLIn revives dead slots; its plans run in full-state mode only */
         if (!a.pilot && (!FULL || k > a.revive_k) && !__any(!n.dead)) break;
-        const DStep st = a.steps[k];
+        const DStep st = plan_get<SP>(a.steps, k);
         if (st.kind == 0) {
-            const DComp &c = a.comps[st.first];
+            const DComp &c = plan_get<SP>(a.comps, st.first);
             tr_in<FULL>(n, c);
             geom_dispatch<F>(n, c);
             terminate(n, c.restore);
@@ -227,13 +237,13 @@ DEV void run_range(NS &n, const SegArgs &a, HistCtx &hc, int k0, int k1, DeferQ 
         } else {
             for (uint32_t ev = 0; ev < st.multi; ev++) {
                 for (uint32_t j = 0; j < st.count; j++) {
-                    const DComp &c = a.comps[st.first + j];
+                    const DComp &c = plan_get<SP>(a.comps, st.first + j);
                     tr_in<FULL>(n, c);
                     geom_dispatch<F>(n, c);
                     tr_out<FULL>(n, c);
                 }
                 for (uint32_t j = 0; j < st.count; j++) {
-                    const DComp &c = a.comps[st.first + j];
+                    const DComp &c = plan_get<SP>(a.comps, st.first + j);
                     tr_in<FULL>(n, c);
                     scat_dispatch<FULL, F>(n, c, hc);
                     tr_out<FULL>(n, c);
@@ -279,17 +289,17 @@ DEV uint32_t defer_pop(DeferQ &dq, NS &r)
 }
 
 /* reflection half of the deferred step, then the rest of the range */
-template <uint32_t F>
+template <uint32_t F, bool SP = true>
 DEV void defer_finish(NS &r, const SegArgs &a, HistCtx &hc, int k1, DeferQ &dq)
 {
     if constexpr ((F & FEAT_MONO) != 0) {
-        const DComp &c = a.comps[a.steps[a.defer_k].first];
+        const DComp &c = plan_get<SP>(a.comps, plan_get<SP>(a.steps, a.defer_k).first);
         if (r.pend && !r.dead) {
             s_mono_b(r, c, r.aux);
             tr_out<false>(r, c);
         }
         r.pend = false;
-        run_range<false, F>(r, a, hc, a.defer_k + 1, k1, dq, false);
+        run_range<false, F, SP>(r, a, hc, a.defer_k + 1, k1, dq, false);
     }
 }
 
@@ -342,9 +352,10 @@ DEV void seg_tail(const NS &n, bool valid, const SegArgs &a, uint64_t beg, uint3
  * region counts are prefix-summed by k_scan before the next segment.  A
  * segment holding the deferred monochromator step drains the wave's
  * deferral queue whenever it holds 64, and at the end. */
-/* the monochromator instances sit a few registers above the limit of
- * three waves per SIMD; ask the compiler to fit them there */
-#define SEG_MIN_WAVES(FULL, F) ((!(FULL) && ((F) & FEAT_MONO) != 0) ? 3 : 1)
+/* the fast-mode monochromator instances and the plain full-state ones sit
+ * a few registers above the limit of three waves per SIMD; ask the
+ * compiler to fit them there (the other full-state instances would spill) */
+#define SEG_MIN_WAVES(FULL, F) ((((FULL) && (F) == 0u) || (!(FULL) && ((F) & FEAT_MONO) != 0)) ? 3 : 1)
 
 template <bool FULL, bool FIRST, uint32_t F>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SEG_MIN_WAVES(FULL, F))))
@@ -397,7 +408,7 @@ void k_segment(SegArgs a)
                 float sb_old = a.carry ? a.slot_ctr[i] : 0.0f;
                 n.iidx = a.carry ? a.slot_iidx[i] : 0u;
                 n.key = (uint32_t)i;
-                src_generate<FULL, F>(n, *a.src, a.num_sim, a.seed, sb_old);
+                src_generate<FULL, F>(n, plan_at(a.src, 0), a.num_sim, a.seed, sb_old);
             } else {
                 queue_load<FULL>(n, a.in, region_slot_group(a.in_prefix, nreg, stride, rcur, base, last, i));
             }
@@ -570,6 +581,7 @@ struct ramp_plugin {
     hipModule_t mod = nullptr;
     hipFunction_t fn = nullptr;        /* ramp_plugin_kernel */
     hipFunction_t fn_trace = nullptr;  /* ramp_plugin_kernel_trace (runs with RAMP_RUN_TRACE) */
+    unsigned grid = 0, grid_trace = 0; /* persistent grids: the co-resident blocks of 256 */
 };
 
 struct DevBuf {
@@ -1444,6 +1456,11 @@ extern "C" int ramp_plugin_compile(ramp_ctx *ctx, const char *source, const char
     hipError_t e = hipModuleLoadData(&pl->mod, code.data());
     if (e == hipSuccess) e = hipModuleGetFunction(&pl->fn, pl->mod, "ramp_plugin_kernel");
     if (e == hipSuccess) e = hipModuleGetFunction(&pl->fn_trace, pl->mod, "ramp_plugin_kernel_trace");
+    int nb = 0, nbt = 0;
+    if (e == hipSuccess) e = hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pl->fn, 256, 0);
+    if (e == hipSuccess) e = hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nbt, pl->fn_trace, 256, 0);
+    pl->grid = (unsigned)ctx->n_cu * (unsigned)std::max(nb, 1);
+    pl->grid_trace = (unsigned)ctx->n_cu * (unsigned)std::max(nbt, 1);
     if (e != hipSuccess) {
         if (pl->mod) (void)hipModuleUnload(pl->mod);
         delete pl;
@@ -1637,8 +1654,8 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
                 void *args[] = {&pa};
                 const bool timed = ck == 0 && !plugin_timed;
                 if (timed) HIPCHK(hipEventRecord(ctx->ev[6], s));
-                HIPCHK(hipModuleLaunchKernel(a.trace ? pl->fn_trace : pl->fn, (unsigned)ctx->n_cu * 8, 1, 1, 256,
-                                             1, 1, 0, s, args, nullptr));
+                HIPCHK(hipModuleLaunchKernel(a.trace ? pl->fn_trace : pl->fn, a.trace ? pl->grid_trace : pl->grid,
+                                             1, 1, 256, 1, 1, 0, s, args, nullptr));
                 if (timed) {
                     HIPCHK(hipEventRecord(ctx->ev[7], s));
                     HIPCHK(hipMemcpyAsync(&plugin_items, (const uint64_t *)p->prefix.p + plugin_nreg,
