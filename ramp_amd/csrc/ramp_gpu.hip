@@ -198,19 +198,20 @@ DEV void run_range(NS &n, const SegArgs &a, HistCtx &hc, int k0, int k1, DeferQ 
             tr_in<FULL>(n, c);
             geom_dispatch<F>(n, c);
             terminate(n, c.restore);
-            bool traced = false;
+            bool traced = false, mcpl = false;
             if constexpr (FULL) {
                 traced = a.trace && !a.pilot && c.trace >= 0 && n.key != 0xffffffffu;
                 if (traced) trace_entry(n, a.trace, a.trace_cap, a.trace_base + n.key, c.trace);
                 /* plans holding MCPL components run in full-state mode only */
-                if (c.s.type == RAMP_S_MCPL_IN || c.s.type == RAMP_S_MCPL_OUT) {
+                mcpl = c.s.type == RAMP_S_MCPL_IN || c.s.type == RAMP_S_MCPL_OUT;
+                if (mcpl) {
                     if (c.s.type == RAMP_S_MCPL_IN) mcpl_in_step(n, c, (uint32_t)a.seed);
                     else if (!a.pilot) mcpl_out_step(n, c, a.dump, a.dump_tag, a.dump_cap, a.chunk_tag);
                     tr_out<FULL>(n, c);
-                    if (traced) trace_alive(n, a.trace, a.trace_cap, a.trace_base + n.key, c.trace);
-                    continue;
                 }
             }
+            /* (the pilot never defers: the deferred step's live count is
+             * taken at its end like every other step's) */
             if constexpr (!FULL && (F & FEAT_MONO) != 0) {
                 if (defer && k == a.defer_k) {
                     bool park = false;
@@ -229,8 +230,10 @@ DEV void run_range(NS &n, const SegArgs &a, HistCtx &hc, int k0, int k1, DeferQ 
                     continue;
                 }
             }
-            scat_dispatch<FULL, F>(n, c, hc);
-            tr_out<FULL>(n, c);
+            if (!mcpl) {
+                scat_dispatch<FULL, F>(n, c, hc);
+                tr_out<FULL>(n, c);
+            }
             if constexpr (FULL) {
                 if (traced) trace_alive(n, a.trace, a.trace_cap, a.trace_base + n.key, c.trace);
             }
