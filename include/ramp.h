@@ -281,7 +281,8 @@ int ramp_rescal_read(ramp_plan *plan, uint32_t comp, float *events, uint64_t cap
  * `det` (n_bins floats, the layout of ramp_hist_read); grad receives, in
  * FP64, [dL/dT1, dL/dsigma_inc, dL/dsigma_abs, dL/dU, dL/dd_0 ..
  * dL/dd_{n-1}] (n_grad = 4 + number of lines; T1 only for a
- * single-temperature MCont, else 0).  opts: wg only.  The run statistics
+ * single-temperature MCont, else 0).  opts: wg only (flags and keep_state
+ * must be 0).  The run statistics
  * afterwards describe the adjoint pass. */
 int ramp_adjoint(ramp_plan *plan, uint64_t n_hist, uint64_t chunk, const uint32_t *seeds,
                  uint32_t n_seeds, uint32_t det, const float *dL_dh, uint64_t n_bins, double *grad,

@@ -653,7 +653,13 @@ DEV void s_chopper(NS &n, const DComp &c)
     setpos(n, ipos2(n));
     n.t += n.is[7];
     float Tg = s.d[0];
-    float pha_error = s.p[CP_JITTER] * 2.0f * (drw(n) - 0.5f);
+    /* chopper.cl:35 draws even without jitter.  With jitter = 0 the phase
+     * error is a signed zero, and no sign of zero can change the result
+     * (it is added to a nonzero value or vanishes under the fabs below), so
+     * only the draw's counter step is kept: the stream stays draw-for-draw. */
+    float pha_error = 0.0f;
+    if (s.p[CP_JITTER] != 0.0f) pha_error = s.p[CP_JITTER] * 2.0f * (drw(n) - 0.5f);
+    else n.ctr = n.ctr + 1.0f;
     float thi_chop = s.p[CP_FREQ] * (n.t - rm_fabs(s.p[CP_PHASE])) + pha_error;
     float thi_neut = rm_atan2(n.x, n.y + s.p[CP_RADIUS]);
     float thi_diff = rm_fabs(thi_chop - thi_neut);
