@@ -23,11 +23,11 @@ struct PluginArgs {
 typedef void (*ramp_scatter_fn)(RampNeutron *, RampIntersection *, uint32_t *, uint32_t, const float *,
                                 uint32_t);
 
-template <bool TRACE>
+template <bool TRACE, uint32_t GF>
 DEV void plugin_one(NS &n, const DComp &c, ramp_scatter_fn f, const PluginArgs &a)
 {
     tr_in<true>(n, c);
-    geom_dispatch<FEAT_ALL>(n, c);
+    geom_dispatch<GF>(n, c);
     terminate(n, c.restore);
     const bool traced = TRACE && c.trace >= 0;
     if (traced) trace_entry(n, a.trace, a.trace_cap, a.trace_base + n.key, c.trace);
@@ -50,9 +50,11 @@ DEV void plugin_one(NS &n, const DComp &c, ramp_scatter_fn f, const PluginArgs &
     if (traced) trace_alive(n, a.trace, a.trace_cap, a.trace_base + n.key, c.trace);
 }
 
-/* the plugin's kernels: ramp_plugin_kernel, and ramp_plugin_kernel_trace
- * for runs that record trace points */
-template <bool TRACE>
+/* the plugin's kernels: ramp_plugin_kernel, ramp_plugin_kernel_trace for
+ * runs that record trace points, and the _mesh forms of both for a plugin
+ * component whose geometry is a GMesh (the mesh traversal costs registers
+ * that every other geometry would pay for) */
+template <bool TRACE, uint32_t GF>
 DEV void plugin_queue(const PluginArgs &a, ramp_scatter_fn f)
 {
     const uint64_t nreg = a.meta[1], stride = a.meta[0], total = a.prefix[nreg];
@@ -70,20 +72,28 @@ DEV void plugin_queue(const PluginArgs &a, ramp_scatter_fn f)
             const uint64_t slot = region_slot_group(a.prefix, nreg, stride, rcur, base, last, i);
             NS n;
             queue_load<true>(n, a.q, slot);
-            if (!n.dead) plugin_one<TRACE>(n, c, f, a);
+            if (!n.dead) plugin_one<TRACE, GF>(n, c, f, a);
             queue_store<true>(n, a.q, slot);
         }
     }
 }
 
-#define RAMP_PLUGIN_KERNEL(entry)                                                          \
-    extern "C" __global__ __launch_bounds__(256) void ramp_plugin_kernel(PluginArgs a)     \
-    {                                                                                      \
-        plugin_queue<false>(a, entry);                                                     \
-    }                                                                                      \
-    extern "C" __global__ __launch_bounds__(256) void ramp_plugin_kernel_trace(PluginArgs a) \
-    {                                                                                      \
-        plugin_queue<true>(a, entry);                                                      \
+#define RAMP_PLUGIN_KERNEL(entry)                                                               \
+    extern "C" __global__ __launch_bounds__(256) void ramp_plugin_kernel(PluginArgs a)          \
+    {                                                                                           \
+        plugin_queue<false, 0u>(a, entry);                                                      \
+    }                                                                                           \
+    extern "C" __global__ __launch_bounds__(256) void ramp_plugin_kernel_trace(PluginArgs a)    \
+    {                                                                                           \
+        plugin_queue<true, 0u>(a, entry);                                                       \
+    }                                                                                           \
+    extern "C" __global__ __launch_bounds__(256) void ramp_plugin_kernel_mesh(PluginArgs a)     \
+    {                                                                                           \
+        plugin_queue<false, FEAT_MESH>(a, entry);                                               \
+    }                                                                                           \
+    extern "C" __global__ __launch_bounds__(256) void ramp_plugin_kernel_mesh_trace(PluginArgs a) \
+    {                                                                                           \
+        plugin_queue<true, FEAT_MESH>(a, entry);                                                \
     }
 
 #endif

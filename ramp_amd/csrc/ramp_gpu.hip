@@ -582,9 +582,10 @@ struct ramp_ctx {
 struct ramp_plugin {
     ramp_ctx *ctx = nullptr;
     hipModule_t mod = nullptr;
-    hipFunction_t fn = nullptr;        /* ramp_plugin_kernel */
-    hipFunction_t fn_trace = nullptr;  /* ramp_plugin_kernel_trace (runs with RAMP_RUN_TRACE) */
-    unsigned grid = 0, grid_trace = 0; /* persistent grids: the co-resident blocks of 256 */
+    /* ramp_plugin_kernel, ramp_plugin_kernel_trace (runs with RAMP_RUN_TRACE)
+     * and their _mesh forms (for a GMesh geometry), indexed [mesh][trace] */
+    hipFunction_t fn[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    unsigned grid[2][2] = {{0, 0}, {0, 0}}; /* persistent grids: the co-resident blocks of 256 */
 };
 
 struct DevBuf {
@@ -1457,13 +1458,15 @@ extern "C" int ramp_plugin_compile(ramp_ctx *ctx, const char *source, const char
     ramp_plugin *pl = new ramp_plugin();
     pl->ctx = ctx;
     hipError_t e = hipModuleLoadData(&pl->mod, code.data());
-    if (e == hipSuccess) e = hipModuleGetFunction(&pl->fn, pl->mod, "ramp_plugin_kernel");
-    if (e == hipSuccess) e = hipModuleGetFunction(&pl->fn_trace, pl->mod, "ramp_plugin_kernel_trace");
-    int nb = 0, nbt = 0;
-    if (e == hipSuccess) e = hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pl->fn, 256, 0);
-    if (e == hipSuccess) e = hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nbt, pl->fn_trace, 256, 0);
-    pl->grid = (unsigned)ctx->n_cu * (unsigned)std::max(nb, 1);
-    pl->grid_trace = (unsigned)ctx->n_cu * (unsigned)std::max(nbt, 1);
+    static const char *const names[2][2] = {{"ramp_plugin_kernel", "ramp_plugin_kernel_trace"},
+                                            {"ramp_plugin_kernel_mesh", "ramp_plugin_kernel_mesh_trace"}};
+    for (int m = 0; m < 2; m++)
+        for (int t = 0; t < 2; t++) {
+            int nb = 0;
+            if (e == hipSuccess) e = hipModuleGetFunction(&pl->fn[m][t], pl->mod, names[m][t]);
+            if (e == hipSuccess) e = hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pl->fn[m][t], 256, 0);
+            pl->grid[m][t] = (unsigned)ctx->n_cu * (unsigned)std::max(nb, 1);
+        }
     if (e != hipSuccess) {
         if (pl->mod) (void)hipModuleUnload(pl->mod);
         delete pl;
@@ -1657,8 +1660,10 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
                 void *args[] = {&pa};
                 const bool timed = ck == 0 && !plugin_timed;
                 if (timed) HIPCHK(hipEventRecord(ctx->ev[6], s));
-                HIPCHK(hipModuleLaunchKernel(a.trace ? pl->fn_trace : pl->fn, a.trace ? pl->grid_trace : pl->grid,
-                                             1, 1, 256, 1, 1, 0, s, args, nullptr));
+                const int mesh = p->hcomps[p->plugin_step[a.s0]].g.type == RAMP_G_MESH ? 1 : 0;
+                const int tr = a.trace ? 1 : 0;
+                HIPCHK(hipModuleLaunchKernel(pl->fn[mesh][tr], pl->grid[mesh][tr], 1, 1, 256, 1, 1, 0, s, args,
+                                             nullptr));
                 if (timed) {
                     HIPCHK(hipEventRecord(ctx->ev[7], s));
                     HIPCHK(hipMemcpyAsync(&plugin_items, (const uint64_t *)p->prefix.p + plugin_nreg,

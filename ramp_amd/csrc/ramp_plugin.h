@@ -37,7 +37,7 @@ typedef struct {
     float x1, y1, z1, t1, x2, y2, z2, t2;
 } RampIntersection;
 
-#ifdef __HIPCC_RTC__
+#if defined(__HIPCC_RTC__) || defined(__HIPCC__)
 #define RAMP_FN __device__ static inline
 #define RAMP_SCATTER extern "C" __device__
 #elif defined(__cplusplus)
