@@ -44,7 +44,7 @@ VALU_PEAK_TOPS = 2 * VALU_SINGLE_ISSUE_TOPS
 #   fetch = FETCH_SIZE / histories, write = WRITE_SIZE / histories (bytes,
 #           after the plain-copy calibration of the counters: x2.0, x1.0)
 PMC_PER_HISTORY = {
-    "let_like": {"valu": 77.59, "lanes": 0.655, "fetch": 0.0076, "write": 1.2598},
+    "let_like": {"valu": 75.78, "lanes": 0.647, "fetch": 0.0076, "write": 1.2598},
     "imaging": {"valu": 454.00, "lanes": 0.175, "fetch": 59.98, "write": 63.93},
     "minimal": None,
 }
