@@ -126,10 +126,26 @@ __global__ __launch_bounds__(256) void k_sel_write(const uint32_t *tags, const f
     const uint32_t lane = __lane_id();
     for (uint64_t base = b; base < e; base += SEL_WGROUP) {
         const uint32_t cnt = sel_group<2>(tags, base, e, tag, src[wv]);
+        /* a lane gathers its (at most two) records' fields with all loads in
+         * flight at once, then writes them to the tile */
+        float v[2][MCR_N];
 #pragma unroll
-        for (int k = 0; k < MCR_N; k++)
-            for (uint32_t r = lane; r < cnt; r += 64)
-                tile[wv][r * MCR_N + k] = recs[(size_t)k * cap + base + src[wv][r]];
+        for (int q = 0; q < 2; q++) {
+            const uint32_t r = lane + 64u * q;
+            if (r < cnt) {
+                const float *rp = recs + base + src[wv][r];
+#pragma unroll
+                for (int k = 0; k < MCR_N; k++) v[q][k] = rp[(size_t)k * cap];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const uint32_t r = lane + 64u * q;
+            if (r < cnt) {
+#pragma unroll
+                for (int k = 0; k < MCR_N; k++) tile[wv][r * MCR_N + k] = v[q][k];
+            }
+        }
         wave_lds_sync();
         float *w = out + (size_t)MCR_N * o;
         for (uint32_t j = lane; j < cnt * MCR_N; j += 64) w[j] = tile[wv][j];
@@ -156,9 +172,26 @@ __global__ __launch_bounds__(256) void k_ev_append(const uint32_t *tags, const f
     const uint32_t lane = __lane_id();
     for (uint64_t base = b; base < e; base += SEL_GROUP) {
         const uint32_t cnt = sel_group<4>(tags, base, e, tag, src[wv]);
-        for (int f = 0; f < RS_NEV; f++)
-            for (uint32_t r = lane; r < cnt; r += 64)
-                list[(size_t)f * list_cap + o + r] = ev[(size_t)f * cap + base + src[wv][r]];
+        /* a lane gathers its (at most four) events with all loads in flight
+         * at once, then stores them */
+        float v[4][RS_NEV];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint32_t r = lane + 64u * q;
+            if (r < cnt) {
+                const float *ep = ev + base + src[wv][r];
+#pragma unroll
+                for (int f = 0; f < RS_NEV; f++) v[q][f] = ep[(size_t)f * cap];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint32_t r = lane + 64u * q;
+            if (r < cnt) {
+#pragma unroll
+                for (int f = 0; f < RS_NEV; f++) list[(size_t)f * list_cap + o + r] = v[q][f];
+            }
+        }
         wave_lds_sync();
         o += cnt;
     }
