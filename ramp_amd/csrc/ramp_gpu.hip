@@ -627,6 +627,9 @@ static const int kNumVariants = 6;
  * and history) */
 static const double TRACE_MAX_BYTES = 32.0 * 1073741824.0;
 
+/* the largest pinned staging buffer a plan keeps for histogram reads */
+static const size_t PIN_MAX = (size_t)64 << 20;
+
 struct ramp_plan {
     ramp_ctx *ctx = nullptr;
     std::vector<DComp> hcomps;
@@ -655,6 +658,8 @@ struct ramp_plan {
     uint32_t n_trace = 0;     /* traced components (vis = 1 in linear blocks) */
     DevBuf trace;             /* their trace points [slot][4][trace_n] */
     uint64_t trace_n = 0;     /* histories of the last traced run (0: the last run was not traced) */
+    void *pin = nullptr;      /* pinned host staging for histogram reads (hipHostMalloc) */
+    size_t pin_bytes = 0;
     std::vector<int> plugin_step;           /* per step: the component number of a plugin step, else -1 */
     std::vector<ramp_plugin *> plugin_of;   /* per component: the bound plugin */
     std::vector<uint32_t> rescal_comp; /* their component numbers */
@@ -1180,8 +1185,9 @@ extern "C" int ramp_plan_destroy(ramp_plan *p)
                         &p->slot_ctr, &p->slot_iidx, &p->st16, &p->st8, &p->stii,
                         &p->rcounts, &p->prefix, &p->comps_brute, &p->dump, &p->dump_tag,
                         &p->sel_counts, &p->sel_prefix, &p->sel_out, &p->ev, &p->ev_tag,
-                        &p->ev_list, &p->ev_list_n};
+                        &p->ev_list, &p->ev_list_n, &p->trace};
     for (DevBuf *b : single) dev_free(*b);
+    if (p->pin) (void)hipHostFree(p->pin);
     for (auto *v : {&p->hw, &p->hw2, &p->hq, &p->hq2, &p->mesh_nodes})
         for (auto &b : *v) dev_free(b);
     for (int i = 0; i < 2; i++) {
@@ -1758,24 +1764,46 @@ extern "C" int ramp_hist_read(ramp_plan *p, uint32_t det, double *w, double *w2,
     if (n != d.n) return fail(RAMP_E_INVALID, "detector %u has %llu bins, asked %llu", det,
                               (unsigned long long)d.n, (unsigned long long)n);
     HIPCHK(hipSetDevice(p->ctx->device));
-    HIPCHK(hipStreamSynchronize(p->ctx->stream));
+    hipStream_t s = p->ctx->stream;
+    /* both moments of both modes go through one pinned staging buffer of at
+     * most PIN_MAX bytes, in pieces (DMA instead of the runtime's pageable
+     * bounce copies) */
+    const size_t want = std::min<size_t>(PIN_MAX, std::max<size_t>(2 * n * sizeof(long long), 4096));
+    if (p->pin_bytes < want) {
+        if (p->pin) (void)hipHostFree(p->pin);
+        p->pin = nullptr;
+        p->pin_bytes = 0;
+        HIPCHK(hipHostMalloc(&p->pin, want, hipHostMallocDefault));
+        p->pin_bytes = want;
+    }
     std::fill(w, w + n, 0.0);
     std::fill(w2, w2 + n, 0.0);
     if (p->has_atomic && d.w) {
-        std::vector<float> a(n), b(n);
-        HIPCHK(hipMemcpy(a.data(), d.w, n * sizeof(float), hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(b.data(), d.w2, n * sizeof(float), hipMemcpyDeviceToHost));
-        for (uint64_t i = 0; i < n; i++) { w[i] += a[i]; w2[i] += b[i]; }
-    }
-    if (p->has_det && d.qw) {
-        std::vector<long long> a(n), b(n);
-        HIPCHK(hipMemcpy(a.data(), d.qw, n * sizeof(long long), hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(b.data(), d.qw2, n * sizeof(long long), hipMemcpyDeviceToHost));
-        for (uint64_t i = 0; i < n; i++) {
-            w[i] += std::ldexp((double)a[i], -d.sh_w);
-            w2[i] += std::ldexp((double)b[i], -d.sh_w2);
+        const uint64_t step = p->pin_bytes / (2 * sizeof(float));
+        for (uint64_t o = 0; o < n; o += step) {
+            const uint64_t m = std::min(step, n - o);
+            float *a = (float *)p->pin, *b = a + m;
+            HIPCHK(hipMemcpyAsync(a, d.w + o, m * sizeof(float), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipMemcpyAsync(b, d.w2 + o, m * sizeof(float), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            for (uint64_t i = 0; i < m; i++) { w[o + i] += a[i]; w2[o + i] += b[i]; }
         }
     }
+    if (p->has_det && d.qw) {
+        const uint64_t step = p->pin_bytes / (2 * sizeof(long long));
+        for (uint64_t o = 0; o < n; o += step) {
+            const uint64_t m = std::min(step, n - o);
+            long long *a = (long long *)p->pin, *b = a + m;
+            HIPCHK(hipMemcpyAsync(a, d.qw + o, m * sizeof(long long), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipMemcpyAsync(b, d.qw2 + o, m * sizeof(long long), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            for (uint64_t i = 0; i < m; i++) {
+                w[o + i] += std::ldexp((double)a[i], -d.sh_w);
+                w2[o + i] += std::ldexp((double)b[i], -d.sh_w2);
+            }
+        }
+    }
+    HIPCHK(hipStreamSynchronize(s));
     return 0;
 }
 
