@@ -148,8 +148,16 @@ typedef struct {
     uint32_t npt;            /* neutrons per thread in the source segment (0: default) */
     uint32_t max_segments;   /* compaction boundaries allowed (0: automatic) */
     uint32_t flags;          /* RAMP_RUN_* bits */
-    uint32_t pad[2];
+    uint32_t pad;
     double weight_bound;     /* deterministic mode: bound on one event's weight (0: derive) */
+    /* One shard of a chunk (shard_total > 0): the run's n_hist histories are
+     * histories [key_base, key_base + n_hist) of a chunk of shard_total, with
+     * those indices as Threefry keys and weights normalised to shard_total,
+     * so shards run in turn (or on several GPUs) add up to the histograms of
+     * the whole chunk.  One chunk only (chunk = 0 or n_hist), fast-mode plans
+     * only, no keep_state or trace.  0, 0: an ordinary run. */
+    uint64_t key_base;
+    uint64_t shard_total;
 } ramp_run_opts;
 
 typedef struct {

@@ -93,8 +93,10 @@ class RunOpts(C.Structure):
         ("npt", C.c_uint32),
         ("max_segments", C.c_uint32),
         ("flags", C.c_uint32),
-        ("pad", C.c_uint32 * 2),
+        ("pad", C.c_uint32),
         ("weight_bound", C.c_double),
+        ("key_base", C.c_uint64),
+        ("shard_total", C.c_uint64),
     ]
 
 

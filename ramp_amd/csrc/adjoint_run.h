@@ -159,8 +159,8 @@ extern "C" int ramp_adjoint(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const
     ramp_run_opts o;
     std::memset(&o, 0, sizeof o);
     if (opts) o = *opts;
-    if (o.flags || o.keep_state)
-        return fail(RAMP_E_INVALID, "run flags 0x%x and keep_state are not available in the adjoint", o.flags);
+    if (o.flags || o.keep_state || o.key_base || o.shard_total)
+        return fail(RAMP_E_INVALID, "run flags 0x%x, keep_state and shards are not available in the adjoint", o.flags);
     if (det >= p->hdets.size()) return fail(RAMP_E_INVALID, "detector %u of %zu", det, p->hdets.size());
     if (n_bins != p->hdets[det].n)
         return fail(RAMP_E_INVALID, "detector %u has %llu bins, dL/dh has %llu", det,

@@ -217,6 +217,7 @@ extern "C" int ramp_batch_run(ramp_batch *b, const uint64_t *counts, const uint3
         return fail(RAMP_E_INVALID, "unknown histogram mode %u", o.hist_mode);
     if (o.keep_state) return fail(RAMP_E_INVALID, "a batch keeps no per-neutron state");
     if (o.flags) return fail(RAMP_E_INVALID, "run flags 0x%x are not available in a batch", o.flags);
+    if (o.key_base || o.shard_total) return fail(RAMP_E_INVALID, "a batch does not run shards");
     unsigned wg = o.wg ? o.wg : 256;
     if (wg != 64 && wg != 128 && wg != 256 && wg != 512) return fail(RAMP_E_INVALID, "workgroup size %u", wg);
     uint32_t npt = o.npt ? o.npt : 4;

@@ -109,6 +109,8 @@ struct SegArgs {
      * null; trace_base is the run-wide index of the chunk's first history */
     float *trace;
     uint64_t trace_cap, trace_base;
+    /* a shard of a chunk: the first history's key (its chunk-local index) */
+    uint64_t key_base;
 };
 
 #include "dev_queue.h"
@@ -410,7 +412,7 @@ void k_segment(SegArgs a)
             if (FIRST) {
                 float sb_old = a.carry ? a.slot_ctr[i] : 0.0f;
                 n.iidx = a.carry ? a.slot_iidx[i] : 0u;
-                n.key = (uint32_t)i;
+                n.key = (uint32_t)(a.key_base + i);
                 src_generate<FULL, F>(n, plan_at(a.src, 0), a.num_sim, a.seed, sb_old);
             } else {
                 queue_load<FULL>(n, a.in, region_slot_group(a.in_prefix, nreg, stride, rcur, base, last, i));
@@ -1517,6 +1519,17 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
     if (o.flags & ~(uint32_t)(RAMP_RUN_MESH_BRUTE_FORCE | RAMP_RUN_TRACE))
         return fail(RAMP_E_INVALID, "unknown run flags 0x%x", o.flags);
     if (n_hist == 0) return 0;
+    const bool shard = o.shard_total != 0;
+    if (!shard && o.key_base) return fail(RAMP_E_INVALID, "key_base without shard_total");
+    if (shard) {
+        if (o.shard_total > 0x7fffffffull || o.key_base + n_hist > o.shard_total)
+            return fail(RAMP_E_RANGE, "shard [%llu, %llu) of a chunk of %llu",
+                        (unsigned long long)o.key_base, (unsigned long long)(o.key_base + n_hist),
+                        (unsigned long long)o.shard_total);
+        if (chunk != 0 && chunk < n_hist) return fail(RAMP_E_INVALID, "a shard runs as one chunk");
+        if (p->need_full || o.keep_state || (o.flags & RAMP_RUN_TRACE))
+            return fail(RAMP_E_INVALID, "shards run fast-mode plans without keep_state or trace");
+    }
     const bool trace = (o.flags & RAMP_RUN_TRACE) != 0 && p->n_trace > 0;
     if (trace && (double)p->n_trace * 16.0 * (double)n_hist > TRACE_MAX_BYTES)
         return fail(RAMP_E_RANGE, "a trace of %u components x %llu histories needs %.3g GB (limit %.3g GB)",
@@ -1541,7 +1554,10 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
     /* the MCPL steps need the polarisation and the whole intersection,
      * SRescal's filter the detected flag */
     const bool full = keep || p->need_full || trace;
-    int rc = prepare_dets(p, (int)o.hist_mode, o.weight_bound, n_hist, chunk);
+    /* a shard takes the fixed-point scale of its whole chunk, so that the
+     * shards' sums add up exactly to the unsharded run's */
+    int rc = prepare_dets(p, (int)o.hist_mode, o.weight_bound, shard ? o.shard_total : n_hist,
+                          shard ? o.shard_total : chunk);
     if (rc) return rc;
     bool carry = n_chunks > 1;
     if (carry) {
@@ -1642,7 +1658,8 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
     uint64_t plugin_items = 0, plugin_nreg = 0;
     for (uint64_t ck = 0; ck < n_chunks; ck++) {
         uint64_t torun = std::min(chunk, n_hist - ck * chunk);
-        a.num_sim = (int)torun;
+        a.num_sim = shard ? (int)o.shard_total : (int)torun;
+        a.key_base = shard ? o.key_base : 0;
         a.seed = (int)seeds[ck];
         a.n = torun;
         a.trace_base = ck * chunk;

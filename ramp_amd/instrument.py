@@ -338,7 +338,7 @@ class Instrument:
 
     # ---- execution ----------------------------------------------------------
     def execute(self, N, buf_max_mod=None, debug=False, trace=False, keep_state=False,
-                seeds=None, mesh_brute_force=False, wg=0, npt=0):
+                seeds=None, mesh_brute_force=False, wg=0, npt=0, shard=None):
         """Run N histories (mcramp.py:403-455).
 
         Extensions: `seeds` (one per chunk), `keep_state` (keep the final
@@ -346,6 +346,11 @@ class Instrument:
         triangle of every GMesh as the reference does instead of traversing
         the BVH; for checking the BVH), `wg`/`npt` (launch shape of the
         source segment; 0 = library default).
+
+        shard=(first, total) runs histories [first, first + N) of a single
+        chunk of `total` (keys and 1/total weights as in the whole chunk), so
+        that shards run in turn add up to the whole chunk's histograms
+        (bit for bit in deterministic mode).
 
         trace=True appends, for every neutron alive after each drawn
         component of a linear block, the line from its previous point to
@@ -360,7 +365,7 @@ class Instrument:
                                      "(mcramp.py:118-122, 209-211); set 'visualise': false on them")
         lib = self._ensure_plan()
         N = int(N)
-        chunk = self.chunking(N, buf_max_mod)
+        chunk = N if shard is not None else self.chunking(N, buf_max_mod)
         n_chunks = (N + chunk - 1) // chunk if N else 0
         if seeds is None:
             seeds = self.chunk_seeds(n_chunks)
@@ -369,7 +374,10 @@ class Instrument:
         opts.hist_mode = (capi.RAMP_HIST_DETERMINISTIC if self.hist_mode == "deterministic"
                           else capi.RAMP_HIST_ATOMIC)
         opts.keep_state = 1 if (keep_state or debug) else 0
-        opts.weight_bound = self.weight_bound(chunk) if N else 0.0
+        opts.weight_bound = self.weight_bound(int(shard[1]) if shard is not None else chunk) if N else 0.0
+        if shard is not None:
+            opts.key_base = int(shard[0])
+            opts.shard_total = int(shard[1])
         opts.flags = ((capi.RAMP_RUN_MESH_BRUTE_FORCE if mesh_brute_force else 0)
                       | (capi.RAMP_RUN_TRACE if trace else 0))
         opts.wg = int(wg)
