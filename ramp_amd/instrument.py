@@ -501,7 +501,7 @@ class Instrument:
         """Zero all histograms (extension; the reference accumulates)."""
         if self._plan is not None:
             capi.check(capi.load_library().ramp_hist_reset(self._plan))
-            self._pull_hists()
+            self._hist_stale = True   # read (as zeros) when next asked for
 
     # ---- results ------------------------------------------------------------
     def data(self):
