@@ -1814,9 +1814,12 @@ extern "C" int ramp_hist_read(ramp_plan *p, uint32_t det, double *w, double *w2,
             HIPCHK(hipMemcpyAsync(a, d.qw + o, m * sizeof(long long), hipMemcpyDeviceToHost, s));
             HIPCHK(hipMemcpyAsync(b, d.qw2 + o, m * sizeof(long long), hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
+            /* q * 2^-sh: a product with a power of two in the normal range,
+             * exact, as ldexp would be */
+            const double sw = std::ldexp(1.0, -d.sh_w), sw2 = std::ldexp(1.0, -d.sh_w2);
             for (uint64_t i = 0; i < m; i++) {
-                w[o + i] += std::ldexp((double)a[i], -d.sh_w);
-                w2[o + i] += std::ldexp((double)b[i], -d.sh_w2);
+                w[o + i] += (double)a[i] * sw;
+                w2[o + i] += (double)b[i] * sw2;
             }
         }
     }
