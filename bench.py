@@ -75,9 +75,29 @@ def parse():
     ap.add_argument("--hist-mode", default="atomic", choices=["atomic", "deterministic"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--prefix", type=int, default=0,
+                    help="keep only the first K entries (source included) of the instrument's "
+                         "first block: the cost of an instrument prefix, e.g. LET-like cut after "
+                         "its k-th component (the CPU baseline is skipped)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="first run a device-to-device copy of this many MiB (HBM counter scale)")
     return ap.parse_args()
+
+
+def instrument_prefix(path, kw, k, tmpdir):
+    """The instrument file cut to the first k entries of its first block
+    (the source counts as one), written to tmpdir; returns (path, names)."""
+    import ramp_amd
+    from collections import OrderedDict
+    text = ramp_amd.Instrument._substitute_params(None, open(path).read(), **kw)
+    desc = json.loads(text, object_pairs_hook=OrderedDict)
+    bname, block = next(iter(desc.items()))
+    names = [n for n in block if n not in ("linear", "multi")][:k]
+    cut = OrderedDict((n, v) for n, v in block.items() if n in ("linear", "multi") or n in names)
+    out = os.path.join(tmpdir, f"prefix{k}.json")
+    with open(out, "w") as f:
+        json.dump({bname: cut}, f)
+    return out, names
 
 
 def instrument_args(workload, tmpdir):
@@ -146,6 +166,10 @@ def main():
     tmpdir = tempfile.mkdtemp(prefix="ramp_bench_")
     kw = instrument_args(a.workload, tmpdir)
     path = os.path.join(ROOT, "ramp_amd", "instruments", wl["file"])
+    prefix_names = None
+    if a.prefix:
+        path, prefix_names = instrument_prefix(path, kw, a.prefix, tmpdir)
+        a.no_cpu = True
     N = int(a.histories if a.histories else wl["histories"])
     inst = ramp_amd.Instrument(path, hist_mode=a.hist_mode, chunk=N, **kw)
     seed = 20261203
@@ -177,7 +201,7 @@ def main():
                     "frac": hbm_gbs / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": bytes_seg0},
             "traffic": None, "launch_ms": seg_avg_ms,
             "achieved": None, "peak": VALU_PEAK_TOPS, "unit": "TOP/s", "frac": None}
-    pmc = PMC_PER_HISTORY.get(a.workload)
+    pmc = None if a.prefix else PMC_PER_HISTORY.get(a.workload)
     if pmc and seg_avg_ms > 0:
         # lane-op slots issued per second by the launch (measured instruction
         # count per history x 64 x histories / measured launch time)
@@ -202,7 +226,8 @@ def main():
         "config": {"workload": wl["name"], "histories_per_step": N, "chunk": N,
                    "hist_mode": a.hist_mode, "instrument": "ramp_amd/instruments/" + wl["file"],
                    "detector_bins": wl["bins"], "events_per_step": ev / K,
-                   "queue_in_per_step": qin / K, "queue_out_per_step": qout / K},
+                   "queue_in_per_step": qin / K, "queue_out_per_step": qout / K,
+                   **({"prefix_components": prefix_names} if prefix_names else {})},
         "roofline": roof,
     }
     if not a.no_cpu:

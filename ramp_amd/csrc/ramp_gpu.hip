@@ -1410,6 +1410,7 @@ static int choose_bounds(ramp_plan *p, const SegArgs &base, uint64_t chunk, unsi
     std::sort(p->bounds.begin(), p->bounds.end());
     p->bounds.erase(std::unique(p->bounds.begin(), p->bounds.end()), p->bounds.end());
     if (first_plugin) p->bounds.insert(p->bounds.begin(), 0);  /* the source alone first */
+    if (p->bounds.size() == 1) p->bounds.push_back(ns);  /* no steps: the source segment alone */
     const size_t nb = p->bounds.size();
     if (nb >= 2 && p->bounds[nb - 1] == p->bounds[nb - 2] + 1 && p->plugin_step[p->bounds[nb - 2]] >= 0)
         p->bounds.push_back(ns);  /* a last, empty segment finalizes the neutrons */
