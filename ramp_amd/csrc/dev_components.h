@@ -37,6 +37,21 @@ DEV const T &plan_at(const T *p, size_t i)
     return *(const T *)((const __attribute__((address_space(4))) T *)p + i);
 }
 
+/* a pointer known to address global memory (HBM) or the workgroup's LDS:
+ * the compiler then emits global or LDS instructions instead of flat ones
+ * (which also wait on the other path) */
+template <class T>
+DEV T *in_global(T *p)
+{
+    return (T *)(__attribute__((address_space(1))) T *)p;
+}
+
+template <class T>
+DEV T *in_lds(T *p)
+{
+    return (T *)(__attribute__((address_space(3))) T *)p;
+}
+
 /* feature sets compiled into a segment-kernel instance: component types
  * outside the set are compiled out, which keeps register use (occupancy)
  * at what the instrument actually needs */
@@ -476,20 +491,20 @@ DEV void hist_add(HistCtx &hc, int det, uint64_t idx, float w)
             return;
         }
         if (d.acc >= 0 && hc.acc) {
-            unsigned long long *q = (unsigned long long *)hc.acc + 2 * ((uint64_t)d.acc + idx);
+            unsigned long long *q = in_lds((unsigned long long *)hc.acc + 2 * ((uint64_t)d.acc + idx));
             atomicAdd(q, (unsigned long long)(long long)a);
             atomicAdd(q + 1, (unsigned long long)(long long)b);
         } else {
-            atomicAdd(&d.qw[idx], (unsigned long long)(long long)a);
-            atomicAdd(&d.qw2[idx], (unsigned long long)(long long)b);
+            atomicAdd(in_global(&d.qw[idx]), (unsigned long long)(long long)a);
+            atomicAdd(in_global(&d.qw2[idx]), (unsigned long long)(long long)b);
         }
     } else if (d.acc >= 0 && hc.acc) {
-        float *f = (float *)hc.acc + 2 * ((uint64_t)d.acc + idx);
+        float *f = in_lds((float *)hc.acc + 2 * ((uint64_t)d.acc + idx));
         atomicAdd(f, w);
         atomicAdd(f + 1, w2);
     } else {
-        unsafeAtomicAdd(&d.w[idx], w);
-        unsafeAtomicAdd(&d.w2[idx], w2);
+        unsafeAtomicAdd(in_global(&d.w[idx]), w);
+        unsafeAtomicAdd(in_global(&d.w2[idx]), w2);
     }
     hc.events++;
 }
@@ -512,13 +527,13 @@ DEV void hist_acc_end(const HistCtx &hc, int n_det, uint32_t bins)
             if (d.acc < 0 || i < (uint32_t)d.acc || i >= (uint32_t)d.acc + d.n) continue;
             const uint64_t b = i - (uint32_t)d.acc;
             if (hc.mode == RAMP_HIST_DETERMINISTIC) {
-                const unsigned long long *q = (const unsigned long long *)hc.acc + 2 * (uint64_t)i;
-                if (q[0]) atomicAdd(&d.qw[b], q[0]);
-                if (q[1]) atomicAdd(&d.qw2[b], q[1]);
+                const unsigned long long *q = in_lds((const unsigned long long *)hc.acc + 2 * (uint64_t)i);
+                if (q[0]) atomicAdd(in_global(&d.qw[b]), q[0]);
+                if (q[1]) atomicAdd(in_global(&d.qw2[b]), q[1]);
             } else {
-                const float *f = (const float *)hc.acc + 2 * (uint64_t)i;
-                if (f[0] != 0.0f) unsafeAtomicAdd(&d.w[b], f[0]);
-                if (f[1] != 0.0f) unsafeAtomicAdd(&d.w2[b], f[1]);
+                const float *f = in_lds((const float *)hc.acc + 2 * (uint64_t)i);
+                if (f[0] != 0.0f) unsafeAtomicAdd(in_global(&d.w[b]), f[0]);
+                if (f[1] != 0.0f) unsafeAtomicAdd(in_global(&d.w2[b]), f[1]);
             }
             break;
         }
