@@ -52,6 +52,15 @@ DEV T *in_lds(T *p)
     return (T *)(__attribute__((address_space(3))) T *)p;
 }
 
+/* a plan table (written before the launch, never by a kernel): loads at
+ * wave-uniform indices, as in loops over a line or CDF table, become
+ * scalar loads */
+template <class T>
+DEV const T *in_const(const T *p)
+{
+    return (const T *)(const __attribute__((address_space(4))) T *)p;
+}
+
 /* feature sets compiled into a segment-kernel instance: component types
  * outside the set are compiled out, which keeps register use (occupancy)
  * at what the instrument actually needs */
@@ -248,8 +257,8 @@ DEV void src_generate(NS &n, const DPart &s, int num_sim, int seed, float sb_old
     }
     case RAMP_SRC_ISIS: if constexpr ((F & FEAT_ISIS) != 0) {
         /* isis_mod.cl:15-105 (scans bounded by the table, index clamped) */
-        const float *flux = s.tab[ST_ISIS_FLUX], *tb = s.tab[ST_ISIS_TBINS];
-        const float *eb = s.tab[ST_ISIS_EBINS], *Eint = s.tab[ST_ISIS_EINT];
+        const float *flux = in_const(s.tab[ST_ISIS_FLUX]), *tb = in_const(s.tab[ST_ISIS_TBINS]);
+        const float *eb = in_const(s.tab[ST_ISIS_EBINS]), *Eint = in_const(s.tab[ST_ISIS_EINT]);
         int nt = (int)s.ni[SI_ISIS_NT], ne = (int)s.ni[SI_ISIS_NE];
         int nflux = (int)s.tab_len[ST_ISIS_FLUX];
         float deviate = s.p[SP_ISIS_TOTAL] * drw(n);

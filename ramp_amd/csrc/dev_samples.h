@@ -89,7 +89,7 @@ DEV void s_isotropic(NS &n, const DComp &c)
     float omega = 0.0f, Q = 0.0f;
     if (flag >= 1) {
         int ch = (int)flag - 1;
-        iso_sample(n, s.tab[ch], s.ni[ISI_NQ(ch)], s.ni[ISI_NW(ch)], omega, Q);
+        iso_sample(n, in_const(s.tab[ch]), s.ni[ISI_NQ(ch)], s.ni[ISI_NW(ch)], omega, Q);
     }
     if (ki * ki < (RC_E2KS * omega)) {
         n.dead = true;
@@ -213,7 +213,7 @@ DEV void s_powder1(NS &n, const DComp &c)
 DEV void s_powderN(NS &n, const DComp &c)
 {
     const DPart &s = c.s;
-    const float *ssv2 = s.tab[PNT_SSCAT_V2], *qs = s.tab[PNT_Q];
+    const float *ssv2 = in_const(s.tab[PNT_SSCAT_V2]), *qs = in_const(s.tab[PNT_Q]);
     uint32_t nl = s.ni[PNI_NLINES];
     rf3 path = sample_path(n);
     float full = v3_len(path);
@@ -300,7 +300,7 @@ template <class HC>
 DEV void s_powderdiff(NS &n, const DComp &c, HC &hc)
 {
     const DPart &s = c.s;
-    const float *Lt = s.tab[PDT_LINES];
+    const float *Lt = in_const(s.tab[PDT_LINES]);
     const uint32_t nl = s.ni[PDI_NLINES];
     rf3 path = sample_path(n);
     float Lp = v3_len(path);
@@ -508,7 +508,7 @@ DEV bool s_mono_a(NS &n, const DComp &c, uint32_t &aux)
 DEV void s_mono_b(NS &n, const DComp &c, uint32_t aux)
 {
     const DPart &s = c.s;
-    const float *gx = s.tab[MOT_GX], *gw = s.tab[MOT_GW];
+    const float *gx = in_const(s.tab[MOT_GX]), *gw = in_const(s.tab[MOT_GW]);
     uint32_t glen = s.ni[MOI_GLEN];
     float mry = s.d[0], mrz = s.d[1], mrmax = s.d[2], Q = s.d[3];
     MonoBragg g = mono_bragg(n, Q);
