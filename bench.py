@@ -43,6 +43,24 @@ VALU_PEAK_TOPS = 2 * VALU_SINGLE_ISSUE_TOPS
 #   lanes = SQ_THREAD_CYCLES_VALU / (64 SQ_INSTS_VALU), the active-lane share
 #   fetch = FETCH_SIZE / histories, write = WRITE_SIZE / histories (bytes,
 #           after the plain-copy calibration of the counters: x2.0, x1.0)
+# measured SQ_INSTS_VALU of the timed batch of the 256-setting scan, and per
+# history of the powder forward and adjoint kernels
+TAS_VALU_PER_SCAN = 1.43062e10
+POWDER_VALU_PER_HISTORY = {"forward": 82.36, "adjoint": 95.60}
+
+
+def valu_roofline(kernel, lane_ops, launch_ms, basis, extra=None):
+    """VALU roofline of a kernel: lane-op slots issued (measured instruction
+    count x 64) per second of its launch, against the dual-issue peak."""
+    ach = lane_ops / (launch_ms * 1e-3) / 1e12 if launch_ms > 0 else 0.0
+    r = {"bound": "valu", "kernel": kernel, "achieved": ach, "peak": VALU_PEAK_TOPS, "unit": "TOP/s",
+         "frac": ach / VALU_PEAK_TOPS, "single_issue_peak": VALU_SINGLE_ISSUE_TOPS, "traffic": None,
+         "launch_ms": launch_ms, "basis": basis}
+    if extra:
+        r.update(extra)
+    return r
+
+
 PMC_PER_HISTORY = {
     "let_like": {"valu": 75.78, "lanes": 0.647, "fetch": 0.0076, "write": 1.2598},
     "imaging": {"valu": 454.00, "lanes": 0.175, "fetch": 59.98, "write": 63.93},
@@ -339,7 +357,8 @@ def bench_tas_scan(a, wl, sync):
                        "batch_speedup": (total * K / dt) / (total / dt_seq),
                        "settings_with_counts": int(ok.sum()),
                        "max_rel_error_mismatch": err_dev},
-        "roofline": None,
+        "roofline": valu_roofline("k_batch<MONO> (the whole scan in one launch)", TAS_VALU_PER_SCAN * 64,
+                                  dev_ms / K, "wave64 VALU instructions of one scan, measured (SQ_INSTS_VALU)"),
     }
     b.close()
     if not a.no_cpu:
@@ -437,7 +456,12 @@ def bench_powder_adjoint(a, wl, sync):
         "finite_differences": {"forwards": 2 * di.n_params,
                                "ms": 2 * di.n_params * dt_fwd / K * 1e3,
                                "adjoint_speedup": (2 * di.n_params * dt_fwd / K) / (dt / K)},
-        "roofline": None,
+        "roofline": valu_roofline("k_segment<false,true,2> (forward source segment)",
+                                  POWDER_VALU_PER_HISTORY["forward"] * 64 * N, fwd_ms / K,
+                                  "wave64 VALU instructions per history, measured (SQ_INSTS_VALU)",
+                                  extra={"adjoint": {"kernel": "k_adjoint<2>", "launch_ms": adj_ms / K,
+                                                     "achieved": POWDER_VALU_PER_HISTORY["adjoint"] * 64 * N
+                                                     / (adj_ms / K * 1e-3) / 1e12}}),
     }
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline_adjoint(di, t, t2, eps, a.cpu_seconds)
