@@ -1,0 +1,407 @@
+/* seg_kernel.h - the fused segment kernel k_segment<FULL, FIRST, F> and the
+ * per-wave machinery it shares with the batched and adjoint kernels
+ * (monochromator deferral queue, step loop, queue tail).  Instances are
+ * compiled per feature set in kern_v.hip. */
+#ifndef RAMP_SEG_KERNEL_H
+#define RAMP_SEG_KERNEL_H
+
+#include "dev_components.h"
+
+/* ======================================================================== */
+/* segment kernel                                                           */
+/* ======================================================================== */
+struct SegArgs {
+    const DComp *__restrict__ comps;
+    const DStep *__restrict__ steps;
+    const DPart *__restrict__ src;
+    const DDet *__restrict__ dets;
+    unsigned long long *cnt;
+    int s0, s1;
+    int last;
+    int pilot;
+    int hist_mode;
+    int carry;
+    int keep;
+    int defer_k;                /* step whose monochromator reflections are deferred (-1: none) */
+    int num_sim;
+    int seed;
+    uint64_t n;                 /* items of the first segment */
+    /* input of later segments: wave regions of the previous segment */
+    const uint64_t *in_prefix;  /* [n_in + 1] exclusive prefix of region counts */
+    const uint64_t *in_meta;    /* [0] region stride, [1] number of regions */
+    /* output: one region per wave, stride written to out_meta[0] */
+    uint32_t *out_counts;
+    uint64_t *out_meta;
+    DQueue in, out;
+    float *slot_ctr;
+    uint32_t *slot_iidx;
+    float *st16, *st8;
+    uint32_t *stii;
+    /* MCPL steps: SMCPLOut records [slot][dump_cap][MCR_N] with their chunk
+     * tags; revive_k is the last SMCPLIn step (-1: none), which every lane
+     * must reach even when all of its neutrons are dead */
+    float *dump;
+    uint32_t *dump_tag;
+    uint64_t dump_cap;
+    uint32_t chunk_tag;
+    int revive_k;
+    /* SRescal: a finished neutron with se > 0 tags its events [slot][ev_cap] */
+    uint32_t *ev_tag;
+    uint64_t ev_cap;
+    uint32_t n_rescal;
+    /* small detectors accumulated in LDS (DDet.acc >= 0): their bins and
+     * the bytes after the deferral queues; the detector count */
+    uint32_t acc_bins, acc_bytes;
+    int n_dets;
+    /* trace points [slot][4][trace_cap] (full-state, RAMP_RUN_TRACE), or
+     * null; trace_base is the run-wide index of the chunk's first history */
+    float *trace;
+    uint64_t trace_cap, trace_base;
+    /* a shard of a chunk: the first history's key (its chunk-local index) */
+    uint64_t key_base;
+};
+
+#include "dev_queue.h"
+
+/* Deferral of the monochromator's reflection branch.  s_mono is cheap
+ * for the many neutrons it does not reflect and long (mosaic search,
+ * 15-point quadrature, ~100 transcendentals) for the few it reflects, so
+ * in a wave of 64 that branch runs at one or two live lanes in most groups.
+ * At the deferred step (SegArgs.defer_k) the reflecting neutrons instead
+ * park in a per-wave LDS queue (state in the slab frame, slab in aux);
+ * when 64 are waiting the wave runs s_mono_b and the remaining steps for
+ * all of them at once.  Each neutron's arithmetic and draws are unchanged,
+ * only the order in which a wave processes them differs. */
+#define DQ_CAP 128
+#define DQ_WORDS 12
+#define DQ_BYTES_PER_WAVE (DQ_CAP * DQ_WORDS * 4)
+
+struct DeferQ {
+    float *b;     /* this wave's DQ_WORDS x DQ_CAP words of LDS (wave-uniform) */
+    uint32_t n;   /* parked neutrons, wave-uniform, < DQ_CAP */
+};
+
+/* order this wave's LDS accesses: the compiler may not move loads of
+ * other lanes' entries across it (a wave executes LDS operations in order) */
+DEV void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+DEV void defer_init(DeferQ &d, float *lds)
+{
+    const uint32_t wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    d.b = lds + wid * (DQ_CAP * DQ_WORDS);
+    d.n = 0;
+}
+
+/* the record of a parked neutron: the segment-queue record (x y z vx vy vz
+ * w t ctr key status) and the slab in aux */
+DEV void defer_store(const NS &n, float *b, uint32_t i)
+{
+    b[0 * DQ_CAP + i] = n.x; b[1 * DQ_CAP + i] = n.y; b[2 * DQ_CAP + i] = n.z;
+    b[3 * DQ_CAP + i] = n.vx; b[4 * DQ_CAP + i] = n.vy; b[5 * DQ_CAP + i] = n.vz;
+    b[6 * DQ_CAP + i] = n.w; b[7 * DQ_CAP + i] = n.t; b[8 * DQ_CAP + i] = n.ctr;
+    uint32_t *u = (uint32_t *)b;
+    u[9 * DQ_CAP + i] = n.key;
+    u[10 * DQ_CAP + i] = n.iidx & 0x00ffffffu;
+    u[11 * DQ_CAP + i] = n.aux;
+}
+
+DEV void defer_load(NS &n, const float *b, uint32_t i)
+{
+    n.x = b[0 * DQ_CAP + i]; n.y = b[1 * DQ_CAP + i]; n.z = b[2 * DQ_CAP + i];
+    n.vx = b[3 * DQ_CAP + i]; n.vy = b[4 * DQ_CAP + i]; n.vz = b[5 * DQ_CAP + i];
+    n.w = b[6 * DQ_CAP + i]; n.t = b[7 * DQ_CAP + i]; n.ctr = b[8 * DQ_CAP + i];
+    const uint32_t *u = (const uint32_t *)b;
+    n.key = u[9 * DQ_CAP + i];
+    n.iidx = u[10 * DQ_CAP + i];
+    n.aux = u[11 * DQ_CAP + i];
+    n.dead = false;
+    n.pend = true;
+    isect_reset(n);
+    n.px = n.py = n.pz = n.sc = n.sd = n.se = 0.0f;
+}
+
+/* Steps [k0, k1); with defer, the step defer_k parks reflecting neutrons
+ * in dq (they leave the range dead with pend set). */
+/* the plan through scalar loads (plan_at), or through vector loads where
+ * the SGPRs that takes would push a kernel into scratch (the batched
+ * monochromator kernel) */
+template <bool SP, class T>
+DEV const T &plan_get(const T *p, size_t i)
+{
+    if constexpr (SP) return plan_at(p, i);
+    else return p[i];
+}
+
+template <bool FULL, uint32_t F, bool SP = true>
+DEV void run_range(NS &n, const SegArgs &a, HistCtx &hc, int k0, int k1, DeferQ &dq, bool defer)
+{
+    for (int k = k0; k < k1; k++) {
+        /* SMCPLIn revives dead slots; its plans run in full-state mode only */
+        if (!a.pilot && (!FULL || k > a.revive_k) && !__any(!n.dead)) break;
+        const DStep st = plan_get<SP>(a.steps, k);
+        if (st.kind == 0) {
+            const DComp &c = plan_get<SP>(a.comps, st.first);
+            tr_in<FULL>(n, c);
+            geom_dispatch<F>(n, c);
+            terminate(n, c.restore);
+            bool traced = false, mcpl = false;
+            if constexpr (FULL) {
+                traced = a.trace && !a.pilot && c.trace >= 0 && n.key != 0xffffffffu;
+                if (traced) trace_entry(n, a.trace, a.trace_cap, a.trace_base + n.key, c.trace);
+                /* plans holding MCPL components run in full-state mode only */
+                mcpl = c.s.type == RAMP_S_MCPL_IN || c.s.type == RAMP_S_MCPL_OUT;
+                if (mcpl) {
+                    if (c.s.type == RAMP_S_MCPL_IN) mcpl_in_step(n, c, (uint32_t)a.seed);
+                    else if (!a.pilot) mcpl_out_step(n, c, a.dump, a.dump_tag, a.dump_cap, a.chunk_tag);
+                    tr_out<FULL>(n, c);
+                }
+            }
+            /* (the pilot never defers: the deferred step's live count is
+             * taken at its end like every other step's) */
+            if constexpr (!FULL && (F & FEAT_MONO) != 0) {
+                if (defer && k == a.defer_k) {
+                    bool park = false;
+                    if (n.iidx == c.comp_idx && !n.dead) park = s_mono_a(n, c, n.aux);
+                    if (!park) tr_out<FULL>(n, c);
+                    const unsigned long long m = __ballot(park);
+                    if (m) {
+                        if (park) {
+                            const unsigned long long below = (1ull << __lane_id()) - 1ull;
+                            defer_store(n, dq.b, dq.n + (uint32_t)__popcll(m & below));
+                            n.pend = true;
+                            n.dead = true;
+                        }
+                        dq.n += (uint32_t)__popcll(m);
+                    }
+                    continue;
+                }
+            }
+            if (!mcpl) {
+                scat_dispatch<FULL, F>(n, c, hc);
+                tr_out<FULL>(n, c);
+            }
+            if constexpr (FULL) {
+                if (traced) trace_alive(n, a.trace, a.trace_cap, a.trace_base + n.key, c.trace);
+            }
+        } else {
+            for (uint32_t ev = 0; ev < st.multi; ev++) {
+                for (uint32_t j = 0; j < st.count; j++) {
+                    const DComp &c = plan_get<SP>(a.comps, st.first + j);
+                    tr_in<FULL>(n, c);
+                    geom_dispatch<F>(n, c);
+                    tr_out<FULL>(n, c);
+                }
+                for (uint32_t j = 0; j < st.count; j++) {
+                    const DComp &c = plan_get<SP>(a.comps, st.first + j);
+                    tr_in<FULL>(n, c);
+                    scat_dispatch<FULL, F>(n, c, hc);
+                    tr_out<FULL>(n, c);
+                }
+            }
+        }
+        if (a.pilot) {
+            unsigned long long m = __ballot(!n.dead);
+            if (__lane_id() == 0 && m) atomicAdd(&a.cnt[CNT_NSTEP0 + k], (unsigned long long)__popcll(m));
+        }
+    }
+}
+
+template <bool FULL, uint32_t F>
+DEV void run_steps(NS &n, const SegArgs &a, HistCtx &hc, DeferQ &dq, bool defer)
+{
+    run_range<FULL, F>(n, a, hc, a.s0, a.s1, dq, defer);
+}
+
+/* Take up to 64 parked neutrons (lane i gets entry i); returns how many. */
+DEV uint32_t defer_pop(DeferQ &dq, NS &r)
+{
+    const uint32_t lane = __lane_id();
+    const uint32_t take = dq.n < 64u ? dq.n : 64u;
+    wave_lds_sync();
+    if (lane < take) {
+        defer_load(r, dq.b, lane);
+    } else {
+        r.dead = true;
+        r.pend = false;
+        r.key = 0;
+        r.iidx = 0;
+    }
+    wave_lds_sync();
+    /* the dq.n - take < 64 entries beyond them move down */
+    if (lane < dq.n - take) {
+        uint32_t *u = (uint32_t *)dq.b;
+        for (int f = 0; f < DQ_WORDS; f++) u[f * DQ_CAP + lane] = u[f * DQ_CAP + take + lane];
+    }
+    wave_lds_sync();
+    dq.n -= take;
+    return take;
+}
+
+/* reflection half of the deferred step, then the rest of the range */
+template <uint32_t F, bool SP = true>
+DEV void defer_finish(NS &r, const SegArgs &a, HistCtx &hc, int k1, DeferQ &dq)
+{
+    if constexpr ((F & FEAT_MONO) != 0) {
+        const DComp &c = plan_get<SP>(a.comps, plan_get<SP>(a.steps, a.defer_k).first);
+        if (r.pend && !r.dead) {
+            s_mono_b(r, c, r.aux);
+            tr_out<false>(r, c);
+        }
+        r.pend = false;
+        run_range<false, F, SP>(r, a, hc, a.defer_k + 1, k1, dq, false);
+    }
+}
+
+template <bool FULL>
+DEV void finalize(const NS &n, const SegArgs &a)
+{
+    if (FULL && a.n_rescal && n.se > 0.0f)
+        for (uint32_t s = 0; s < a.n_rescal; s++) a.ev_tag[s * a.ev_cap + n.key] = a.chunk_tag;
+    if (a.carry) {
+        a.slot_ctr[n.key] = n.ctr;
+        a.slot_iidx[n.key] = n.iidx;
+    }
+    if (FULL && a.keep) {
+        float *o = a.st16 + 16 * (size_t)n.key;
+        o[0] = n.x; o[1] = n.y; o[2] = n.z; o[3] = n.vx; o[4] = n.vy; o[5] = n.vz;
+        o[6] = n.px; o[7] = n.py; o[8] = n.pz; o[9] = n.w; o[10] = n.t; o[11] = n.ctr;
+        o[12] = n.sc; o[13] = n.sd; o[14] = n.se; o[15] = n.dead ? 1.0f : 0.0f;
+        float *q = a.st8 + 8 * (size_t)n.key;
+        for (int i = 0; i < 8; i++) q[i] = n.is[i];
+        a.stii[n.key] = n.iidx;
+    }
+}
+
+DEV uint32_t wave_sum(uint32_t v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+/* end of a group of 64 in a segment: survivors to the wave's output
+ * region, finished neutrons to finalize (parked ones are neither) */
+template <bool FULL>
+DEV void seg_tail(const NS &n, bool valid, const SegArgs &a, uint64_t beg, uint32_t &wcount)
+{
+    const bool live = valid && !n.dead && !a.last;
+    if (!a.last) {
+        const unsigned long long m = __ballot(live);
+        if (live) {
+            const unsigned long long below = m & ((1ull << __lane_id()) - 1ull);
+            queue_store<FULL>(n, a.out, beg + wcount + (uint64_t)__popcll(below));
+        }
+        wcount += (uint32_t)__popcll(m);
+    }
+    if (valid && !live && !n.pend) finalize<FULL>(n, a);
+}
+
+/* One segment.  Work is split statically: wave g of G handles the dense
+ * item range [g*per, (g+1)*per) and appends its survivors to its own region
+ * [g*per, ...) of the output queue, so compaction needs no atomics; the
+ * region counts are prefix-summed by k_scan before the next segment.  A
+ * segment holding the deferred monochromator step drains the wave's
+ * deferral queue whenever it holds 64, and at the end. */
+/* the fast-mode monochromator instances and the plain full-state ones sit
+ * a few registers above the limit of three waves per SIMD; ask the
+ * compiler to fit them there (the other full-state instances would spill) */
+#define SEG_MIN_WAVES(FULL, F) ((((FULL) && (F) == 0u) || (!(FULL) && ((F) & FEAT_MONO) != 0)) ? 3 : 1)
+
+template <bool FULL, bool FIRST, uint32_t F>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SEG_MIN_WAVES(FULL, F))))
+void k_segment(SegArgs a)
+{
+    extern __shared__ float seg_lds[];
+    HistCtx hc;
+    hc.dets = a.dets;
+    hc.mode = a.hist_mode;
+    hc.enabled = !a.pilot;
+    hc.events = hc.oob = hc.overflow = 0;
+    hc.acc = nullptr;
+    if (a.acc_bins) {
+        /* the detectors' LDS copy follows the deferral queues */
+        const size_t defer_bytes = (!FULL && (F & FEAT_MONO) != 0) ? (blockDim.x >> 6) * DQ_BYTES_PER_WAVE : 0;
+        hist_acc_begin(hc, (char *)seg_lds + defer_bytes, a.acc_bytes);
+    }
+    const uint32_t lane = __lane_id();
+    const uint64_t G = (uint64_t)gridDim.x * (blockDim.x >> 6);
+    const uint64_t g = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    uint64_t total, nreg = 0, stride = 0;
+    if (FIRST) {
+        total = a.n;
+    } else {
+        nreg = a.in_meta[1];
+        stride = a.in_meta[0];
+        total = a.in_prefix[nreg];
+    }
+    const uint64_t per = (total + G - 1) / G;
+    if (g == 0 && lane == 0 && !a.pilot) a.out_meta[0] = per;
+    const uint64_t beg = g * per;
+    const uint64_t end = beg + per < total ? beg + per : total;
+    uint32_t wcount = 0;
+    DeferQ dq;
+    dq.b = seg_lds;
+    dq.n = 0;
+    bool defer = false;
+    if constexpr (!FULL && (F & FEAT_MONO) != 0) {
+        defer = !a.pilot && a.defer_k >= a.s0 && a.defer_k < a.s1;
+        if (defer) defer_init(dq, seg_lds);
+    }
+    uint64_t rcur = 0; /* region of the group's first input (later segments) */
+    for (uint64_t base = beg; base < end; base += 64) {
+        uint64_t i = base + lane;
+        bool valid = i < end;
+        const uint64_t last = base + 63 < end ? base + 63 : end - 1;
+        NS n;
+        if (valid) {
+            if (FIRST) {
+                float sb_old = a.carry ? a.slot_ctr[i] : 0.0f;
+                n.iidx = a.carry ? a.slot_iidx[i] : 0u;
+                n.key = (uint32_t)(a.key_base + i);
+                src_generate<FULL, F>(n, plan_at(a.src, 0), a.num_sim, a.seed, sb_old);
+            } else {
+                queue_load<FULL>(n, a.in, region_slot_group(a.in_prefix, nreg, stride, rcur, base, last, i));
+            }
+        } else {
+            n.dead = true;
+            n.pend = false;
+            n.key = 0xffffffffu; /* no history on this lane */
+            n.iidx = 0;
+        }
+        run_steps<FULL, F>(n, a, hc, dq, defer);
+        if (a.pilot) continue;
+        seg_tail<FULL>(n, valid, a, beg, wcount);
+        if constexpr (!FULL && (F & FEAT_MONO) != 0) {
+            if (defer && dq.n >= 64u) {
+                NS r;
+                const uint32_t take = defer_pop(dq, r);
+                defer_finish<F>(r, a, hc, a.s1, dq);
+                seg_tail<FULL>(r, lane < take, a, beg, wcount);
+            }
+        }
+    }
+    if (a.pilot) return;
+    if constexpr (!FULL && (F & FEAT_MONO) != 0) {
+        while (defer && dq.n > 0u) {
+            NS r;
+            const uint32_t take = defer_pop(dq, r);
+            defer_finish<F>(r, a, hc, a.s1, dq);
+            seg_tail<FULL>(r, lane < take, a, beg, wcount);
+        }
+    }
+    if (!a.last && lane == 0) a.out_counts[g] = wcount;
+    if (a.acc_bins) hist_acc_end(hc, a.n_dets, a.acc_bins);
+    uint32_t ev = wave_sum(hc.events), oob = wave_sum(hc.oob), ovf = wave_sum(hc.overflow);
+    if (lane == 0) {
+        if (ev) atomicAdd(&a.cnt[CNT_EVENTS], (unsigned long long)ev);
+        if (oob) atomicAdd(&a.cnt[CNT_OOB], (unsigned long long)oob);
+        if (ovf) atomicAdd(&a.cnt[CNT_OVERFLOW], (unsigned long long)ovf);
+    }
+}
+
+typedef void (*seg_fn)(SegArgs);
+
+#endif

@@ -396,8 +396,9 @@ class Instrument:
                 c.scat._collect(lib, self._plan, slot)
         if trace and N:
             self._trace_lines(lib, N, chunk)
-        if debug:
-            n16, _, _ = self.state(min(chunk, N))
+        if debug and N:
+            # the library keeps the last chunk's states: N - (chunks - 1) * chunk
+            n16, _, _ = self.state(N - (-(-N // chunk) - 1) * chunk)
             live = n16[n16[:, 15] == 0.0]
             print(live)
 
