@@ -197,9 +197,31 @@ int ramp_plan_destroy(ramp_plan *plan);
  * source seed of chunk c (the reference draws randint(0,4096) per chunk,
  * flat_mod.py:32); n_seeds must cover ceil(n_hist/chunk) chunks.
  * Replaces Instrument.execute (mcramp.py:403-455).  Histograms accumulate
- * across runs, as the reference's do (detector1D.py:59-70). */
+ * across runs, as the reference's do (detector1D.py:59-70).  With n_hist =
+ * 0 nothing runs; the plan's histograms of opts->hist_mode are allocated
+ * (zeroed) if they did not exist. */
 int ramp_run(ramp_plan *plan, uint64_t n_hist, uint64_t chunk, const uint32_t *seeds,
              uint32_t n_seeds, const ramp_run_opts *opts);
+
+/* ramp_run without any device or pinned allocation: no counterpart in the
+ * reference.  For callers that cannot allocate (HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The segment queues,
+ * region counts and prefixes and the per-slot carry buffers of several
+ * chunks come from one static device array per GPU (RAMP_NOALLOC_SCRATCH_BYTES,
+ * held until the process ends) instead of plan allocations.  Limits:
+ * chunk (n_hist if chunk = 0) at most RAMP_NOALLOC_MAX_CHUNK (RAMP_E_RANGE
+ * otherwise); fast-mode plans only (no SMCPLIn/SMCPLOut, SRescal, SAnalyser
+ * or plugin components) and no keep_state, trace or brute-force flags
+ * (RAMP_E_INVALID); the plan's histograms of the mode must exist (ramp_run
+ * with n_hist = 0 allocates them; RAMP_E_STATE otherwise); a deterministic
+ * run that would need its sums folded to the host returns RAMP_E_RANGE.
+ * Results equal ramp_run's.  Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (other threads, other contexts' streams)
+ * race and must be serialised by the caller. */
+#define RAMP_NOALLOC_MAX_CHUNK 262144
+#define RAMP_NOALLOC_SCRATCH_BYTES 25985280
+int ramp_run_noalloc(ramp_plan *plan, uint64_t n_hist, uint64_t chunk, const uint32_t *seeds,
+                     uint32_t n_seeds, const ramp_run_opts *opts);
 
 /* Read detector slot `det` (w and w^2 sums, n values each).  Replaces the
  * wrappers' _cached_copy readback (detector1D.py:130-138). */

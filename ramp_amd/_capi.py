@@ -140,6 +140,8 @@ _SIGS = {
     "ramp_plan_destroy": (C.c_int, [C.c_void_p]),
     "ramp_run": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint32), C.c_uint32,
                            C.POINTER(RunOpts)]),
+    "ramp_run_noalloc": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint32), C.c_uint32,
+                                   C.POINTER(RunOpts)]),
     "ramp_hist_read": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_double),
                                  C.POINTER(C.c_double), C.c_uint64]),
     "ramp_hist_reset": (C.c_int, [C.c_void_p]),

@@ -24,6 +24,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include <hip/hiprtc.h>
@@ -188,6 +189,7 @@ struct ramp_ctx {
     hipStream_t stream;
     hipEvent_t ev[8];
     std::string arch;   /* gcnArchName, for run-time compiled plugins */
+    void *scratch = nullptr; /* this device's scratch array (ramp_run_noalloc) */
 };
 
 struct ramp_plugin {
@@ -320,6 +322,52 @@ struct ramp_plan {
     ramp_run_stats stats;
 };
 
+/* ---- scratch of the no-allocation run (ramp_run_noalloc) ----------------
+ * A fast-mode run needs, besides the plan: two segment queues (9 float
+ * fields, key and status: 44 B per record), the per-wave region counts and
+ * their prefix, and with several chunks each slot's RNG counter and
+ * component index.  ramp_run allocates them per plan; ramp_run_noalloc
+ * takes them from `scratch`, one static device array per GPU, sized for
+ * chunks of at most RAMP_NOALLOC_MAX_CHUNK histories and NA_MAX_WAVES
+ * co-resident waves (256 CUs x 32 waves).  Its device address is taken once
+ * per device with hipGetSymbolAddress when the first context on that device
+ * is created; host code uses only that address. */
+static const uint64_t NA_MAX_WAVES = 8192;
+static const uint64_t NA_CAP = RAMP_NOALLOC_MAX_CHUNK + NA_MAX_WAVES;   /* queue records */
+static const size_t NA_ALIGN = 256;
+static constexpr size_t na_round(size_t b) { return (b + NA_ALIGN - 1) / NA_ALIGN * NA_ALIGN; }
+static const size_t NA_QUEUE_FIELD = na_round(NA_CAP * 4);             /* one field of one queue */
+static const size_t NA_OFF_COUNTS = 2 * 11 * NA_QUEUE_FIELD;
+static const size_t NA_OFF_PREFIX = NA_OFF_COUNTS + na_round(NA_MAX_WAVES * 4);
+static const size_t NA_OFF_CTR = NA_OFF_PREFIX + na_round((NA_MAX_WAVES + 1) * 8);
+static const size_t NA_OFF_IIDX = NA_OFF_CTR + na_round(RAMP_NOALLOC_MAX_CHUNK * 4);
+static const size_t NA_BYTES = NA_OFF_IIDX + na_round(RAMP_NOALLOC_MAX_CHUNK * 4);
+static_assert(NA_BYTES == RAMP_NOALLOC_SCRATCH_BYTES, "RAMP_NOALLOC_SCRATCH_BYTES in ramp.h");
+
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+static std::mutex g_scratch_mu;
+static std::vector<void *> g_scratch_ptr;   /* per device: the address of `scratch` */
+
+/* the device address of this device's `scratch` (device current) */
+static int scratch_setup(int device, void **out)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((size_t)device >= g_scratch_ptr.size()) g_scratch_ptr.resize(device + 1, nullptr);
+    if (!g_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(RAMP_E_DEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", device,
+                        hipGetErrorString(e));
+        g_scratch_ptr[device] = ptr;
+    }
+    *out = g_scratch_ptr[device];
+    return 0;
+}
+
+extern "C" int ramp_ctx_destroy(ramp_ctx *ctx);
+
 extern "C" int ramp_ctx_create(int device, ramp_ctx **out)
 {
     if (!out) return fail(RAMP_E_INVALID, "null output pointer");
@@ -335,6 +383,11 @@ extern "C" int ramp_ctx_create(int device, ramp_ctx **out)
     c->arch = prop.gcnArchName;
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (int i = 0; i < 8; i++) HIPCHK(hipEventCreate(&c->ev[i]));
+    int rc = scratch_setup(device, &c->scratch);
+    if (rc) {
+        (void)ramp_ctx_destroy(c);
+        return rc;
+    }
     *out = c;
     return 0;
 }
@@ -1188,8 +1241,19 @@ extern "C" int ramp_plan_bind_plugin(ramp_plan *p, uint32_t comp, ramp_plugin *p
     return 0;
 }
 
-extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uint32_t *seeds,
-                        uint32_t n_seeds, const ramp_run_opts *opts)
+/* the device buffers a run works in besides the plan's tables and
+ * histograms: the plan's own (ramp_run) or the device's scratch array
+ * (ramp_run_noalloc) */
+struct RunBufs {
+    DQueue q[2];
+    uint32_t *rcounts = nullptr;
+    uint64_t *prefix = nullptr;
+    float *slot_ctr = nullptr;
+    uint32_t *slot_iidx = nullptr;
+};
+
+static int run_plan(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uint32_t *seeds, uint32_t n_seeds,
+                    const ramp_run_opts *opts, bool noalloc)
 {
     if (!p) return fail(RAMP_E_INVALID, "null plan");
     ramp_run_opts o;
@@ -1199,7 +1263,15 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
         return fail(RAMP_E_INVALID, "unknown histogram mode %u", o.hist_mode);
     if (o.flags & ~(uint32_t)(RAMP_RUN_MESH_BRUTE_FORCE | RAMP_RUN_TRACE))
         return fail(RAMP_E_INVALID, "unknown run flags 0x%x", o.flags);
-    if (n_hist == 0) return 0;
+    if (n_hist == 0) {
+        /* no histories: the histograms of the mode exist afterwards (what
+         * ramp_run_noalloc needs), nothing else happens */
+        if (noalloc) return 0;
+        HIPCHK(hipSetDevice(p->ctx->device));
+        int rc = ensure_hist(p, (int)o.hist_mode);
+        if (!rc) HIPCHK(hipStreamSynchronize(p->ctx->stream));
+        return rc;
+    }
     const bool shard = o.shard_total != 0;
     if (!shard && o.key_base) return fail(RAMP_E_INVALID, "key_base without shard_total");
     if (shard) {
@@ -1235,18 +1307,48 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
     /* the MCPL steps need the polarisation and the whole intersection,
      * SRescal's filter the detected flag */
     const bool full = keep || p->need_full || trace;
+    if (noalloc) {
+        if (full || (o.flags & RAMP_RUN_MESH_BRUTE_FORCE))
+            return fail(RAMP_E_INVALID, "ramp_run_noalloc runs fast-mode plans (no MCPL, SRescal, SAnalyser or "
+                                        "plugin components) without keep_state, trace or brute force");
+        if (chunk > RAMP_NOALLOC_MAX_CHUNK)
+            return fail(RAMP_E_RANGE, "ramp_run_noalloc: chunk %llu exceeds RAMP_NOALLOC_MAX_CHUNK (%llu)",
+                        (unsigned long long)chunk, (unsigned long long)RAMP_NOALLOC_MAX_CHUNK);
+        if (!ctx->scratch) return fail(RAMP_E_STATE, "no scratch array on device %d", ctx->device);
+        for (size_t k = 0; k < p->hdets.size(); k++)
+            if ((o.hist_mode == RAMP_HIST_ATOMIC && !p->hw[k].p) ||
+                (o.hist_mode == RAMP_HIST_DETERMINISTIC && !p->hq[k].p))
+                return fail(RAMP_E_STATE, "ramp_run_noalloc: the plan's histograms of this mode do not exist; "
+                                          "call ramp_run with n_hist = 0 first");
+        if (o.hist_mode == RAMP_HIST_DETERMINISTIC) {
+            DetScale t = p->ds;
+            const double wb = o.weight_bound > 0 ? o.weight_bound : 1.0 / (double)chunk;
+            if (det_scale_run(t, wb, (double)(shard ? o.shard_total : n_hist), (double)n_hist))
+                return fail(RAMP_E_RANGE, "ramp_run_noalloc: the deterministic histograms' int64 budget is spent; "
+                                          "read them and reset, or run with ramp_run");
+        }
+    }
     /* a shard takes the fixed-point scale of its whole chunk, so that the
      * shards' sums add up exactly to the unsharded run's */
     int rc = prepare_dets(p, (int)o.hist_mode, o.weight_bound, shard ? o.shard_total : n_hist,
                           shard ? o.shard_total : chunk, n_hist);
     if (rc) return rc;
+    RunBufs bufs;
+    std::memset(bufs.q, 0, sizeof bufs.q);
     bool carry = n_chunks > 1;
     if (carry) {
-        rc = dev_alloc(p->slot_ctr, chunk * sizeof(float));
-        if (!rc) rc = dev_alloc(p->slot_iidx, chunk * sizeof(uint32_t));
-        if (rc) return rc;
-        launch_fill_f32(s, p->slot_ctr.p, chunk);
-        launch_fill_u32(s, p->slot_iidx.p, chunk);
+        if (noalloc) {
+            bufs.slot_ctr = (float *)((char *)ctx->scratch + NA_OFF_CTR);
+            bufs.slot_iidx = (uint32_t *)((char *)ctx->scratch + NA_OFF_IIDX);
+        } else {
+            rc = dev_alloc(p->slot_ctr, chunk * sizeof(float));
+            if (!rc) rc = dev_alloc(p->slot_iidx, chunk * sizeof(uint32_t));
+            if (rc) return rc;
+            bufs.slot_ctr = (float *)p->slot_ctr.p;
+            bufs.slot_iidx = (uint32_t *)p->slot_iidx.p;
+        }
+        launch_fill_f32(s, bufs.slot_ctr, chunk);
+        launch_fill_u32(s, bufs.slot_iidx, chunk);
     }
     if (keep) {
         rc = dev_alloc(p->st16, chunk * 16 * sizeof(float));
@@ -1305,8 +1407,8 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
     a.acc_bins = p->acc_bins;
     a.acc_bytes = p->acc_bytes;
     a.n_dets = (int)p->hdets.size();
-    a.slot_ctr = (float *)p->slot_ctr.p;
-    a.slot_iidx = (uint32_t *)p->slot_iidx.p;
+    a.slot_ctr = bufs.slot_ctr;
+    a.slot_iidx = bufs.slot_iidx;
     a.st16 = (float *)p->st16.p;
     a.st8 = (float *)p->st8.p;
     a.stii = (uint32_t *)p->stii.p;
@@ -1328,18 +1430,39 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
     unsigned grid_first = (unsigned)ctx->n_cu * blocks_per_cu(full, true, p->variant, wg, p->acc_bytes);
     unsigned grid_rest = (unsigned)ctx->n_cu * blocks_per_cu(full, false, p->variant, wg, p->acc_bytes);
     uint64_t max_waves = (uint64_t)std::max(grid_first, grid_rest) * (wg / 64);
-    if (nseg > 1) {
+    if (nseg > 1 && noalloc) {
+        if (max_waves > NA_MAX_WAVES || chunk + max_waves > NA_CAP)
+            return fail(RAMP_E_RANGE, "ramp_run_noalloc: %llu waves and %llu queue records exceed the scratch "
+                                      "array (%llu, %llu)", (unsigned long long)max_waves,
+                        (unsigned long long)(chunk + max_waves), (unsigned long long)NA_MAX_WAVES,
+                        (unsigned long long)NA_CAP);
+        char *base = (char *)ctx->scratch;
+        for (int i = 0; i < 2; i++) {
+            DQueue &q = bufs.q[i];
+            std::memset(&q, 0, sizeof q);
+            float **f[9] = {&q.x, &q.y, &q.z, &q.vx, &q.vy, &q.vz, &q.w, &q.t, &q.ctr};
+            for (int k = 0; k < 9; k++) *f[k] = (float *)(base + (size_t)(i * 11 + k) * NA_QUEUE_FIELD);
+            q.key = (uint32_t *)(base + (size_t)(i * 11 + 9) * NA_QUEUE_FIELD);
+            q.status = (uint32_t *)(base + (size_t)(i * 11 + 10) * NA_QUEUE_FIELD);
+        }
+        bufs.rcounts = (uint32_t *)(base + NA_OFF_COUNTS);
+        bufs.prefix = (uint64_t *)(base + NA_OFF_PREFIX);
+    } else if (nseg > 1) {
         rc = ensure_queues(p, chunk + max_waves, full);
         if (!rc) rc = queues_hold(p, chunk + max_waves, full);
         if (!rc) rc = dev_alloc(p->rcounts, max_waves * sizeof(uint32_t));
         if (!rc) rc = dev_alloc(p->prefix, (max_waves + 1) * sizeof(uint64_t));
         if (rc) return rc;
+        bufs.q[0] = queue_view(p, 0);
+        bufs.q[1] = queue_view(p, 1);
+        bufs.rcounts = (uint32_t *)p->rcounts.p;
+        bufs.prefix = (uint64_t *)p->prefix.p;
     }
     uint64_t *meta = (uint64_t *)p->count.p; /* [0..1] producer, [2..3] consumer */
-    a.out_counts = (uint32_t *)p->rcounts.p;
+    a.out_counts = bufs.rcounts;
     a.out_meta = meta;
     a.in_meta = meta + 2;
-    a.in_prefix = (const uint64_t *)p->prefix.p;
+    a.in_prefix = bufs.prefix;
     launch_fill_u64(s, p->cnt.p, CNT_N);
     HIPCHK(hipEventRecord(ctx->ev[0], s));
     uint64_t segs = 0;
@@ -1362,8 +1485,8 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
                 /* a plugin step rewrites the queue in place */
                 ramp_plugin *pl = p->plugin_of[p->plugin_step[a.s0]];
                 PluginArgs pa;
-                pa.q = queue_view(p, cur);
-                pa.prefix = (const uint64_t *)p->prefix.p;
+                pa.q = bufs.q[cur];
+                pa.prefix = bufs.prefix;
                 pa.meta = meta + 2;
                 pa.comp = (const DComp *)p->comps.p + p->plugin_step[a.s0];
                 pa.trace = a.trace;
@@ -1378,7 +1501,7 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
                                              nullptr));
                 if (timed) {
                     HIPCHK(hipEventRecord(ctx->ev[7], s));
-                    HIPCHK(hipMemcpyAsync(&plugin_items, (const uint64_t *)p->prefix.p + plugin_nreg,
+                    HIPCHK(hipMemcpyAsync(&plugin_items, bufs.prefix + plugin_nreg,
                                           sizeof plugin_items, hipMemcpyDeviceToHost, s));
                     plugin_timed = true;
                 }
@@ -1387,8 +1510,8 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
             }
             a.last = (sg == nseg - 1) ? 1 : 0;
             a.pilot = 0;
-            if (sg >= 1) a.in = queue_view(p, cur);
-            a.out = queue_view(p, sg == 0 ? 0 : cur ^ 1);
+            if (sg >= 1) a.in = bufs.q[cur];
+            a.out = bufs.q[sg == 0 ? 0 : cur ^ 1];
             unsigned grid;
             if (sg == 0) grid = (unsigned)std::min<uint64_t>((torun + wg - 1) / wg, grid_first);
             else grid = grid_rest;
@@ -1401,8 +1524,8 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
             if (sg + 1 < nseg) {
                 uint64_t nreg = (uint64_t)grid * (wg / 64);
                 plugin_nreg = nreg;
-                hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, (const uint32_t *)p->rcounts.p,
-                                   nreg, (const uint64_t *)meta, (uint64_t *)p->prefix.p, meta + 2,
+                hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, (const uint32_t *)bufs.rcounts,
+                                   nreg, (const uint64_t *)meta, bufs.prefix, meta + 2,
                                    (unsigned long long *)p->cnt.p, sg == 0 ? 1 : 0);
                 HIPCHK(hipGetLastError());
             }
@@ -1445,6 +1568,18 @@ extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uin
     p->trace_n = trace ? n_hist : 0;
     p->dump_n = std::min(chunk, n_hist - (n_chunks - 1) * chunk);
     return 0;
+}
+
+extern "C" int ramp_run(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uint32_t *seeds,
+                        uint32_t n_seeds, const ramp_run_opts *opts)
+{
+    return run_plan(p, n_hist, chunk, seeds, n_seeds, opts, false);
+}
+
+extern "C" int ramp_run_noalloc(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uint32_t *seeds,
+                                uint32_t n_seeds, const ramp_run_opts *opts)
+{
+    return run_plan(p, n_hist, chunk, seeds, n_seeds, opts, true);
 }
 
 extern "C" int ramp_trace_read(ramp_plan *p, uint32_t slot, float *out, uint64_t n)
