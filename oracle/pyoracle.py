@@ -47,6 +47,8 @@ def load():
         lib.oracle_set_mcpl_dump.restype = None
         lib.oracle_set_mcpl_dump.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_uint32), C.c_uint64]
         lib.oracle_last_error.restype = C.c_char_p
+        lib.oracle_math_d.restype = None
+        lib.oracle_math_d.argtypes = [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_uint64]
         lib.oracle_threefry4x32.restype = None
         lib.oracle_threefry4x32.argtypes = [C.POINTER(C.c_uint32)] * 3
         _lib = lib

@@ -826,7 +826,7 @@ static void s_isotropic(const octx *cx, on_t *n, const ramp_part_desc *s, uint32
     }
     float kf;
     /* FP64: 0.5*exp(-omega/(kB*temperature)) with kB a double expression */
-    double db = 0.5 * exp(-(double)omega / (RC_KB_D * (double)p[IS_TEMP]));
+    double db = 0.5 * rd_exp(-(double)omega / (RC_KB_D * (double)p[IS_TEMP]));
     if ((double)ornd(n) > db) {
         kf = rm_sqrt(ki * ki - RC_E2KS * omega);
         n->N_SE = omega;
@@ -1008,14 +1008,14 @@ static void s_powderN(const octx *cx, on_t *n, const ramp_part_desc *s, uint32_t
 static float mono_randnorm(on_t *n)
 {
     float u1 = ornd(n), u2 = ornd(n);
-    return (float)((double)rm_sqrt(-2.0f * rm_log(u1)) * cos(2.0f * RC_M_PI_D * (double)u2));
+    return (float)((double)rm_sqrt(-2.0f * rm_log(u1)) * rd_cos(2.0f * RC_M_PI_D * (double)u2));
 }
 
 /* GAUSS macro: monochromator.cl:5-8 (FP64: sqrt(2.0f*M_PI)) */
 static double mono_gauss(float x, float mean, float rms)
 {
     float e = rm_exp(-((x) - (mean)) * ((x) - (mean)) / (2.0f * (rms) * (rms)));
-    return (double)e / (sqrt(2.0f * RC_M_PI_D) * (double)rms);
+    return (double)e / (RC_SQRT_2PI_D * (double)rms);
 }
 
 /* scat/monochromator.cl:22-197 */
@@ -1885,4 +1885,11 @@ void oracle_rotate_axis(float angle, const float *axis, const float *v, float *o
 void oracle_threefry4x32(const uint32_t *in, const uint32_t *key, uint32_t *out)
 {
     rr_threefry4x32_20(in, key, out);
+}
+
+/* the shared FP64 functions of rmath.h (fn 0: rd_exp, 1: rd_cos), for their
+ * accuracy tests */
+void oracle_math_d(int fn, const double *x, double *out, uint64_t n)
+{
+    for (uint64_t i = 0; i < n; i++) out[i] = fn == 0 ? rd_exp(x[i]) : rd_cos(x[i]);
 }

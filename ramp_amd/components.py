@@ -145,10 +145,111 @@ class MCont(_Source):
         return tot / num_sim
 
 
+def read_tally_spectrum(path):
+    """Read a moderator spectrum in the McStas/MCNP tally layout that
+    MISIS consumes (e.g. the reference's examples/Let_Base.mcstas).
+
+    The file is line oriented:
+      - header lines, one of which is the comment 'c TimeOffset <value>';
+      - one block per energy bin, opened by the line
+        'energy bin: <lower> to <upper>' (MeV) and preceded by its
+        'surface' / 'segment:' / 'user bin' lines; after the 'cosine:' and
+        'time' lines come the time rows '<time> <value> (<error>)', time in
+        units of 1e-8 s, and the block ends with 'total <sum> (<error>)';
+      - a closing line ('1end').
+    Every block has the same time column.
+
+    Returns a dict: 'time_offset' (float), 'e_lower' and 'e_upper' (meV,
+    float64 arrays), 'time' (the time column in seconds, float64), 'values'
+    (float64 array [n_blocks, n_times]) and 'totals' (the blocks' total
+    lines, float64)."""
+    time_offset = None
+    e_lo, e_hi, totals, blocks = [], [], [], []
+    rows = None
+    with open(path, "r") as f:
+        for line in f:
+            words = line.split()
+            if not words:
+                continue
+            if time_offset is None and "TimeOffset" in line:
+                time_offset = float(words[-1])
+            elif line.startswith("energy bin"):
+                lo, hi = line.split(":", 1)[1].split("to")
+                e_lo.append(float(lo) * 1e9)
+                e_hi.append(float(hi) * 1e9)
+                rows = []
+                blocks.append(rows)
+            elif words[0] == "total" and rows is not None:
+                totals.append(float(words[1]))
+                rows = None
+            elif rows is not None and len(words) == 3 and words[2].startswith("("):
+                rows.append((float(words[0]), float(words[1])))
+    if time_offset is None or not blocks:
+        raise ValueError(f"{path}: not a tally spectrum (no TimeOffset line or energy-bin blocks)")
+    n_t = len(blocks[0])
+    if any(len(b) != n_t for b in blocks) or len(totals) != len(blocks):
+        raise ValueError(f"{path}: energy-bin blocks of unequal length or without total lines")
+    table = np.array(blocks, dtype=np.float64)          # [block, time, (time, value)]
+    return {"time_offset": time_offset, "e_lower": np.array(e_lo), "e_upper": np.array(e_hi),
+            "time": table[0, :, 0] / 1e8, "values": table[:, :, 1], "totals": np.array(totals)}
+
+
+def _energy_window(e_lower, e_upper, E_min, E_max):
+    """The blocks MISIS keeps for the window [E_min, E_max] and their
+    weights (isis_mod.py:127-150): from the block containing E_min (the one
+    before the first lower edge >= E_min; when E_min is the file's first
+    lower edge, that block itself) up to the block before the first lower
+    edge >= E_max, each weighted by the part of its [lower, upper] inside
+    the window.  Returns (first, stop, weights, edges): blocks [first,
+    stop), and edges E_min, the kept lower edges after the first block,
+    E_max.  The weights are evaluated in float64 against the float32
+    window, as the numpy of the reference's time evaluates Python floats
+    against float32 scalars."""
+    start = int(np.searchsorted(e_lower, E_min))
+    if E_min == e_lower[0]:
+        start += 1
+    stop = int(np.searchsorted(e_lower, E_max))
+    first = start - 1
+    if first < 0:
+        raise ValueError(f"MISIS: E_min {E_min} meV lies below the spectrum's first energy bin")
+    weights = []
+    for lo, hi in zip(e_lower[first:stop], e_upper[first:stop]):
+        width = hi - lo
+        inside = (hi - E_min) / width if E_min > lo else 1.0
+        if E_max < hi:
+            inside -= (hi - E_max) / width
+        weights.append(inside)
+    edges = np.concatenate([[E_min], e_lower[start:stop], [E_max]])
+    return first, stop, np.array(weights), edges
+
+
+def _target_solid_angle_factor(mod_dim, target_dim, target_dist, n=50):
+    """MISIS's str_area (isis_mod.py:43-62): a midpoint-like sum of 1/r^2
+    between grid points of the moderator face (x and y in [-w/2, w/2]) and
+    of one quadrant of the target window (x and y in [0, w/2]), times the
+    two areas (the target's in cm^2).  The reference pairs each moderator x
+    with the target x of the same grid index (a row of its x grid
+    broadcasts against the target's x grid) and runs every moderator y
+    against every target y, and repeats the sum once per row of its x grid
+    (n times).  In the far field it tends to A_mod * A_target[cm^2] / d^2."""
+    mw, mh = mod_dim
+    tw, th = target_dim
+    tx = np.linspace(0, tw / 2.0, num=n)
+    ty = np.linspace(0, th / 2.0, num=n)
+    mx = np.linspace(-mw / 2.0, mw / 2.0, num=n)
+    my = np.linspace(-mh / 2.0, mh / 2.0, num=n)
+    dx2 = (mx - tx) ** 2                                  # [j]: paired by index
+    dy2 = (my[:, None] - ty[None, :]) ** 2                # [moderator y, target y]
+    total = n * np.sum(1.0 / (dx2[None, None, :] + dy2[:, :, None] + target_dist ** 2))
+    return np.float32(total * (mw * mh) / n ** 4 * tw * th * 10000)
+
+
 class MISIS(_Source):
     """Tabulated ISIS moderator spectrum (mcramp/moderator/isis_mod.py:10-185).
-    load_spectrum follows the reference's regex parser of the McStas
-    moderator file (isis_mod.py:64-166)."""
+
+    The spectrum file is read by read_tally_spectrum; the energy window,
+    the cumulative flux table over (energy block, time row), the cumulative
+    sums at block ends and the total follow isis_mod.py:121-166."""
     type_code = capi.RAMP_SRC_ISIS
     draws_per_history = 8
 
@@ -157,7 +258,7 @@ class MISIS(_Source):
         self._init_part()
         self._geom_args(mod_dim, target_dim, target_dist, E_min, E_max)
         self.load_spectrum(spec_file)
-        self.calc_str_area(mod_dim, target_dim, target_dist)
+        self.str_area = _target_solid_angle_factor(mod_dim, target_dim, target_dist)
         self.p[S.SP_ISIS_TOTAL] = self.total
         self.p[S.SP_ISIS_STR_AREA] = self.str_area
         self.p[S.SP_ISIS_TOFFSET] = self.time_offset
@@ -168,86 +269,23 @@ class MISIS(_Source):
         self.tables[S.ST_ISIS_EBINS] = self.e_bins
         self.tables[S.ST_ISIS_EINT] = self.e_int
 
-    @staticmethod
-    def calc_fraction(lower, upper, minimum, maximum):
-        bin_range = upper - lower
-        frac = (upper - minimum) / bin_range if (minimum > lower) else 1.0
-        frac -= (upper - maximum) / bin_range if (maximum < upper) else 0.0
-        return frac
-
-    def calc_str_area(self, mod_dim, target_dim, target_dist):
-        # isis_mod.py:43-62
-        n_steps = 50
-        A = 0.0
-        tx = np.linspace(0, target_dim[0] / 2.0, num=n_steps)
-        ty = np.linspace(0, target_dim[1] / 2.0, num=n_steps)
-        TX, TY = np.meshgrid(tx, ty)
-        mx_ = np.linspace(-mod_dim[0] / 2.0, mod_dim[0] / 2.0, num=n_steps)
-        my_ = np.linspace(-mod_dim[1] / 2.0, mod_dim[1] / 2.0, num=n_steps)
-        MX, MY = np.meshgrid(mx_, my_)
-        for mx in MX:
-            for my in MY:
-                A += np.sum(1 / (np.power(-1.0 * TX + mx, 2.0) + np.power(-1.0 * TY + my, 2.0)
-                                 + target_dist ** 2.0))
-        A *= (mod_dim[0] * mod_dim[1]) / (n_steps ** 4.0)
-        A *= target_dim[0] * target_dim[1] * 10000
-        self.str_area = np.float32(A)
-
     def load_spectrum(self, spec_file):
-        # isis_mod.py:64-166
-        with open(spec_file, "r") as mod_file:
-            lines = mod_file.read()
-            pattern = r"(?:\d*\.\d*[e][+\-]?\d+ \d*\.\d*[e][+\-]?\d+ \(\d*\.\d*[e][+\-]?\d+\) )"
-            result = re.findall(pattern, lines)
-            mod_file.seek(0)
-            line = ""
-            offset_linenum = 0
-            while "TimeOffset" not in line:
-                line = mod_file.readline()
-                offset_linenum += 1
-            self.time_offset = np.float32(line.split()[-1])
-            time_linenum = offset_linenum
-            while "time" not in line:
-                line = mod_file.readline()
-                time_linenum += 1
-            total_linenum = time_linenum
-            while "total" not in line:
-                line = mod_file.readline()
-                total_linenum += 1
-            mod_file.seek(0)
-            e_uppers, e_lowers = [], []
-            for line in mod_file:
-                if "energy bin" in line:
-                    bounds = re.findall(r"(?:\d*\.\d*[e][+\-]?\d+)", line)
-                    e_uppers.append(float(bounds[1]) * 1e9)
-                    e_lowers.append(float(bounds[0]) * 1e9)
-        num_time_bins = total_linenum - time_linenum - 1
-        data = np.array([r.strip().replace("(", "").replace(")", "") for r in result])
-        data = np.reshape(data, (int(np.size(data) / num_time_bins), num_time_bins))
-        e_min_idx = np.searchsorted(e_lowers, self.E_min)
-        e_max_idx = np.searchsorted(e_lowers, self.E_max)
-        if self.E_min == e_lowers[0]:
-            e_min_idx += 1
-        e_bins = [self.E_min] + list(e_lowers[e_min_idx:e_max_idx]) + [self.E_max]
-        e_bins = np.array(e_bins)
-        fractions = [self.calc_fraction(e_lowers[i], e_uppers[i], self.E_min, self.E_max)
-                     for i in range(e_min_idx - 1, e_max_idx)]
-        t_bins = np.array([l.split(" ")[0] for l in data[0]]).astype(float) / 1e8
-        i_vals = np.zeros(data.shape)
-        for i in range(data.shape[0]):
-            i_vals[i] = np.array([l.split(" ")[1] for l in data[i]]).astype(float)
-        crop = i_vals[e_min_idx - 1:e_max_idx, :]
-        fr = np.repeat(np.array([[f] for f in fractions]), crop.shape[1], axis=1)
-        crop = np.multiply(crop, fr)
-        flux = np.cumsum(crop.flatten()).astype(np.float32)
-        e_int = np.insert((flux.reshape(crop.shape)[:, -1]), 0, 0.0).astype(np.float32)
-        self.num_time_bins = np.int32(num_time_bins)
-        self.num_ener_bins = np.int32(len(e_bins))
-        self.flux = flux
-        self.t_bins = t_bins.astype(np.float32)
-        self.e_bins = e_bins.astype(np.float32)
-        self.e_int = e_int
-        self.total = np.float32(np.sum(crop))
+        spec = read_tally_spectrum(spec_file)
+        self.time_offset = np.float32(spec["time_offset"])
+        first, stop, weights, edges = _energy_window(spec["e_lower"], spec["e_upper"],
+                                                     self.E_min, self.E_max)
+        kept = spec["values"][first:stop] * weights[:, None]
+        # cumulative flux over (block, time row) in row-major order, summed
+        # in float64 and stored as float32; the cumulative value at the end
+        # of each block, after a leading zero
+        cum = np.cumsum(kept.ravel())
+        self.flux = cum.astype(np.float32)
+        self.e_int = np.concatenate([[0.0], cum.reshape(kept.shape)[:, -1]]).astype(np.float32)
+        self.total = np.float32(kept.sum())
+        self.num_time_bins = np.int32(kept.shape[1])
+        self.num_ener_bins = np.int32(len(edges))
+        self.t_bins = spec["time"].astype(np.float32)
+        self.e_bins = edges.astype(np.float32)
 
     def weight_bound(self, num_sim):
         return float(self.str_area) * float(self.total) * 6.2415093e12 / num_sim
@@ -584,48 +622,133 @@ class SGuide(_Scat):
         return [[p[0] for p in pts], [p[1] for p in pts], [p[2] for p in pts]]
 
 
-def _simps_avg(y, x, axis=-1):
-    """Composite Simpson integration with the even-count rule of SciPy's
-    former simps(even='avg') (the average of the first-interval and
-    last-interval trapezoid corrections), written out here because simps
-    no longer exists in SciPy; used by the S(Q,w) loader (isotropic.py:
-    201-211).  Parity with the reference's SciPy is unpinned."""
-    y = np.asarray(y, dtype=np.float64)
-    x = np.asarray(x, dtype=np.float64)
-    y = np.moveaxis(y, axis, -1)
-    n = y.shape[-1]
+def _simps_legacy(y, x, axis=-1):
+    """Composite Simpson integration of samples y at abscissae x along
+    `axis`, as scipy.integrate.simps (SciPy <= 1.10, default even='avg';
+    removed in SciPy 1.14) computes it: for an odd sample count, Simpson's
+    rule on pairs of possibly unequal intervals h0, h1,
+      (h0 + h1)/6 * (y0 (2 - 1/(h0/h1)) + y1 (h0 + h1)^2/(h0 h1) + y2 (2 - h0/h1)),
+    summed; for an even count, the average of (Simpson on all but the last
+    sample + the last interval's trapezoid) and (Simpson on all but the
+    first sample + the first interval's trapezoid).  Arithmetic stays in the
+    inputs' dtype (float32 tables stay float32), as in that SciPy.  The
+    reference calls it in its S(q, w) loader (isotropic.py:7, 201-203,
+    211); the installed SciPy no longer has it."""
+    y = np.asarray(y)
+    axis = axis % y.ndim
+    n = y.shape[axis]
+    # x as a column along `axis`, so that its differences broadcast
+    shape = [1] * y.ndim
+    shape[axis] = n
+    x = np.asarray(x).reshape(shape)
 
-    def simpson_odd(yy, xx):
-        m = yy.shape[-1]
-        if m == 1:
-            return np.zeros(yy.shape[:-1])
-        h = np.diff(xx)
-        tot = np.zeros(yy.shape[:-1])
-        for i in range(0, m - 2, 2):
-            h0, h1 = h[i], h[i + 1]
-            hs = h0 + h1
-            tot = tot + hs / 6.0 * (yy[..., i] * (2 - h1 / h0) + yy[..., i + 1] * hs * hs / (h0 * h1)
-                                    + yy[..., i + 2] * (2 - h0 / h1))
-        return tot
+    def take(a, start, stop, step=2):
+        return a[(slice(None),) * axis + (slice(start, stop, step),)]
 
-    if n < 2:
-        return np.zeros(y.shape[:-1])
+    def pairs(lo, hi):
+        # Simpson over samples lo .. hi (hi - lo even); the sum runs along
+        # `axis` in place, as that SciPy's np.sum did
+        h = np.diff(x, axis=axis)
+        h0, h1 = take(h, lo, hi), take(h, lo + 1, hi + 1)
+        hsum = h0 + h1
+        ratio = h0 / h1
+        terms = hsum / 6.0 * (take(y, lo, hi) * (2 - 1.0 / ratio)
+                              + take(y, lo + 1, hi + 1) * hsum * hsum / (h0 * h1)
+                              + take(y, lo + 2, hi + 2) * (2 - ratio))
+        return np.sum(terms, axis=axis)
+
     if n % 2 == 1:
-        return simpson_odd(y, x)
-    first = simpson_odd(y[..., :-1], x[:-1]) + 0.5 * (x[-1] - x[-2]) * (y[..., -1] + y[..., -2])
-    last = simpson_odd(y[..., 1:], x[1:]) + 0.5 * (x[1] - x[0]) * (y[..., 0] + y[..., 1])
-    return 0.5 * (first + last)
+        return pairs(0, n - 2)
+
+    def one(a, i):
+        return a[(slice(None),) * axis + (i,)]
+    ends = 0.5 * (one(x, -1) - one(x, -2)) * (one(y, -1) + one(y, -2))
+    ends = ends + 0.5 * (one(x, 1) - one(x, 0)) * (one(y, 1) + one(y, 0))
+    return (pairs(0, n - 3) + pairs(1, n - 2)) / 2.0 + ends / 2.0
 
 
-def _cdf_from_pd(x, pd):
-    # isotropic.py:210-212
-    un = np.array([_simps_avg(pd[:i], x[:i]) for i in range(1, len(x) + 1)], dtype=np.float32)
-    return un / un[-1]
+def _running_cdf(x, density):
+    """The normalised running integral of `density` over x at every sample
+    (isotropic.py:210-212): the legacy Simpson rule over samples 0..i for
+    each i (zero for i = 0), in float32, divided by its last value."""
+    partial = np.array([_simps_legacy(density[:i + 1], x[:i + 1]) for i in range(len(x))], dtype=np.float32)
+    return partial / partial[-1]
+
+
+_SQW_KEY_CHARS = set("ABCDEFGHIJKLMNOPQRSTUVWXYZabcdefghijklmnopqrstuvwxyz_[\\]^`")
+
+
+def _sqw_header_entry(line):
+    """'# <key> <number>' of an S(q, w) file, or None.  The key runs over
+    letters, '_' and the ASCII characters between 'Z' and 'a' (the
+    reference's character range admits them); the value is the longest
+    leading decimal '<digits>.<digits>' or integer of the next word."""
+    at = line.find("#")
+    while at >= 0:
+        rest = line[at + 1:]
+        words = rest.split()
+        if rest[:1].isspace() and len(words) >= 2 and all(ch in _SQW_KEY_CHARS for ch in words[0]):
+            m = re.match(r"[-+]?(\d*\.\d+|\d+)", words[1])
+            gap = rest.lstrip()[len(words[0]):]
+            if m and gap[:1].isspace():
+                return words[0], m.group(0)
+        at = line.find("#", at + 1)
+    return None
+
+
+def read_sqw(path):
+    """Read an S(q, w) table in the layout of the reference's SIsotropic
+    files (McStas Isotropic_Sqw style), keeping what its loader keeps
+    (isotropic.py:171-196):
+      - '# <key> <value>' lines set parameters (V_rho, sigma_coh, sigma_abs
+        are used);
+      - the first other line is the q grid, whose last value is dropped;
+      - the second is the w grid;
+      - the third is read and discarded;
+      - the remaining lines are S rows, one per kept q, w along the row.
+    S is returned as float32 plus the float64 machine epsilon rounded to
+    float32 (2.220446e-16), as the loader adds it.
+    Returns (header dict of strings, q, w, S[q, w])."""
+    header = {}
+    grids = []
+    with open(path, "r") as f:
+        lines = f.readlines()
+    pos = 0
+    for pos, line in enumerate(lines):
+        entry = _sqw_header_entry(line)
+        if entry:
+            header[entry[0]] = entry[1]
+        if line.startswith("#"):
+            continue
+        if len(grids) == 2:
+            break   # this line is consumed, not kept
+        grids.append(np.array(line.split(), dtype=np.float32))
+    q, w = grids[0][:-1], grids[1]
+    # data rows: text after '#' is a comment, blank lines are skipped
+    rows = [np.array(l.split("#")[0].split(), dtype=np.float64) for l in lines[pos + 1:]
+            if l.split("#")[0].strip()]
+    sqw = np.array(rows).astype(np.float32) + np.float32(np.finfo(np.float64).eps)
+    return header, q, w, sqw
+
+
+def _sqw_distributions(q, w, sqw):
+    """The sampling tables of SIsotropic (isotropic.py:199-207):
+      p(w) proportional to the integral over q of q S(q, w), normalised
+      over w, and its running CDF over w;
+      for each w_j, p(q | w_j) = q S(q, w_j) / S(q) with S(q) the integral
+      of S(q, w) over w, and its running CDF over q.
+    Returns (pw_cdf[nw], pq_cdf[nw, nq]), float32."""
+    per_w = _simps_legacy(q * sqw.T, q, axis=1)       # over q, one per w
+    per_w /= _simps_legacy(per_w, w)
+    s_of_q = _simps_legacy(sqw.T, w, axis=0)          # over w, one per q
+    pw_cdf = _running_cdf(w, per_w)
+    pq_cdf = np.array([_running_cdf(q, q * sqw[:, j] / s_of_q) for j in range(len(w))], dtype=np.float32)
+    return pw_cdf, pq_cdf
 
 
 class SIsotropic(_Scat):
-    """S(Q,w) isotropic scatterer (scat/isotropic.py:12-212).  _LoadSQW
-    follows the reference text; its integrations use _simps_avg."""
+    """S(Q,w) isotropic scatterer (scat/isotropic.py:12-212).  Files are read
+    by read_sqw and turned into sampling tables by _sqw_distributions."""
     type_code = capi.RAMP_S_ISOTROPIC
 
     def __init__(self, fn_coh="", fn_inc="", fn_mag="", temperature=0, transmit=1, idx=0,
@@ -638,14 +761,17 @@ class SIsotropic(_Scat):
         self.idx = np.uint32(idx)
         for ch, fn in enumerate((fn_coh, fn_inc, fn_mag)):
             if fn == "":
-                # dummy tables of the reference (isotropic.py:70-81)
+                # a channel without a file: one-element zero tables (isotropic.py:70-81)
                 q = w = pw = pq = np.zeros(1, np.float32)
                 nq = nw = 0
                 rho = sa = ss = np.float32(0)
             else:
-                q, w, rho, sa, ss, pw, pq, _ = self._LoadSQW(fn)
-                pq = pq.flatten()
+                header, q, w, sqw = read_sqw(fn)
+                pw, pq = _sqw_distributions(q, w, sqw)
+                pq = pq.ravel()
                 nq, nw = len(q), len(w)
+                # every channel scatters with its file's sigma_coh (isotropic.py:196)
+                rho, sa, ss = (np.float32(header[k]) for k in ("V_rho", "sigma_abs", "sigma_coh"))
             self.p[S.slot("IS_RHO", ch)] = np.float32(rho)
             self.p[S.slot("IS_SABS", ch)] = np.float32(sa)
             self.p[S.slot("IS_SSCAT", ch)] = np.float32(ss)
@@ -655,39 +781,6 @@ class SIsotropic(_Scat):
                                               for a in (q, w, pw, pq)])
         self.p[S.IS_TEMP] = self.temperature
         self.ni[S.ISI_TRANSMIT] = self.transmit
-
-    @staticmethod
-    def _LoadSQW(fn):
-        header = {}
-        flag = 0
-        with open(fn, "r") as fin:
-            for line in fin:
-                d = re.findall(r"#\s+([A-za-z_]+)\s+([-+]?\d*\.\d+|\d+)", line)
-                if d:
-                    header[d[0][0]] = d[0][1]
-                if not line[0] == "#":
-                    if flag == 0:
-                        q = np.array(line.split(), dtype=np.float32)[:-1]
-                        flag += 1
-                        continue
-                    elif flag == 1:
-                        w = np.array(line.split(), dtype=np.float32)
-                        flag += 1
-                        continue
-                    else:
-                        break
-            sqw = np.genfromtxt(fin.readlines()).astype(np.float32) + \
-                np.finfo(float).eps.astype(np.float32)
-        sigma_abs = np.float32(header["sigma_abs"])
-        sigma_coh = np.float32(header["sigma_coh"])
-        rho = np.float32(header["V_rho"])
-        sigma_scat = sigma_coh
-        pw = _simps_avg(q * sqw.T, q, axis=1)
-        pw /= _simps_avg(pw, w)
-        pq = np.array([q * sqw[:, i] / _simps_avg(sqw.T, w, axis=0) for i in range(0, len(w))])
-        pw_cdf = _cdf_from_pd(w, pw)
-        pq_cdf = np.array([_cdf_from_pd(q, pq[j, :]) for j in range(0, len(w))], dtype=np.float32)
-        return (q, w, rho, sigma_abs, sigma_scat, pw_cdf, pq_cdf, sqw)
 
 
 def _powder_sigma(d_spacing, pack, vc, sigma_abs, multiplicity, DW, F2):

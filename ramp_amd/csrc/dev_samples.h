@@ -98,8 +98,10 @@ DEV void s_isotropic(NS &n, const DComp &c)
         return;
     }
     float kf;
-    float db = 0.5f * rm_exp(-omega / (s.d[2] * s.p[IS_TEMP]));
-    if (drw(n) > db) {
+    /* isotropic.cl:247 in double: kB is a double expression (consts.h:36);
+     * the exponential is the shared rd_exp */
+    double db = 0.5 * rd_exp(-(double)omega / (RC_KB_D * (double)s.p[IS_TEMP]));
+    if ((double)drw(n) > db) {
         kf = rm_sqrt(ki * ki - RC_E2KS * omega);
         n.se = omega;
     } else {
@@ -275,17 +277,20 @@ DEV void s_powderN(NS &n, const DComp &c)
     s_done(n, c.comp_idx);
 }
 
-/* GAUSS of monochromator.cl:5-8 as an FP32 approximation; d[8] holds
- * sqrt(2*M_PI) rounded to float */
-DEV float mono_gauss(float x, float rms, float sq2pi)
+/* GAUSS of monochromator.cl:5-8 with mean 0: the exponential in float,
+ * divided in double by sqrt(2.0f*M_PI) (double) times rms, as the
+ * reference's double promotion evaluates it */
+DEV double mono_gauss(float x, float rms)
 {
-    return rm_exp(-(x) * (x) / (2.0f * (rms) * (rms))) / (sq2pi * rms);
+    float e = rm_exp(-(x) * (x) / (2.0f * (rms) * (rms)));
+    return (double)e / (RC_SQRT_2PI_D * (double)rms);
 }
 
 /* scat/monochromator.cl:22-197.  d[0] mos_rms_y, d[1] mos_rms_z,
- * d[2] mos_rms_max, d[3] Q, d[4] zmax, d[5] ymax, d[8] sqrt(2 pi),
- * d[9]/d[10] 2*pi split.  Box-Muller's cos and the GAUSS weights are FP32
- * approximations of the reference's double evaluation. */
+ * d[2] mos_rms_max, d[3] Q, d[4] zmax, d[5] ymax.  The reference's double
+ * sites (GAUSS's normalisation and the products and quotient of GAUSS
+ * values, Box-Muller's cos(2.0f*M_PI*u2)) are evaluated in FP64, the cos
+ * with the shared rd_cos: only reflected neutrons reach them. */
 /* Differentiable powder + incoherent sample (no reference counterpart).
  * Depth, channel and grain d-spacing are drawn from the proposal; the
  * weight is multiplied by model / proposal probability:
@@ -531,24 +536,23 @@ DEV void s_mono_b(NS &n, const DComp &c, uint32_t aux)
         if (qv.z * qv.z + qv.y * qv.y < (25.0f / (2.0f / 3.0f)) * (qv.x * qv.x)) break;
         mos_sample *= (2.0f / 3.0f);
     }
-    float sq2pi = s.d[8];
     for (uint32_t i = 0; i < glen; i++) {
         float phi = w * gx[i];
         float cphi = rm_cos(phi), sphi = rm_sin(phi);
         qv = v3_add(v3_add(c1, v3_scale(cphi, a)), v3_scale(sphi, b));
-        float pr = mono_gauss(qv.z / qv.x, mry, sq2pi) * mono_gauss(qv.y / qv.x, mrz, sq2pi);
+        float pr = (float)(mono_gauss(qv.z / qv.x, mry) * mono_gauss(qv.y / qv.x, mrz));
         total += gw[i] * pr;
     }
     total *= w;
+    /* randnorm (monochromator.cl:10-20) */
     float u1 = drw(n), u2 = drw(n);
-    rdf tp; tp.hi = s.d[9]; tp.lo = s.d[10];
-    float rn = rm_sqrt(-2.0f * rm_log(u1)) * rm_cos(df_mulc(u2, tp));
+    float rn = (float)((double)rm_sqrt(-2.0f * rm_log(u1)) * rd_cos(2.0f * RC_M_PI_D * (double)u2));
     float phi = mos_sample * rn;
     float cphi = rm_cos(phi), sphi = rm_sin(phi);
     qv = v3_add(v3_add(c1, v3_scale(cphi, a)), v3_scale(sphi, b));
-    float p_reflect = mono_gauss(qv.z / qv.x, mry, sq2pi) * mono_gauss(qv.y / qv.x, mrz, sq2pi);
+    float p_reflect = (float)(mono_gauss(qv.z / qv.x, mry) * mono_gauss(qv.y / qv.x, mrz));
     setvel(n, v3_scale(RC_K2V, v3_add(ki, qv)));
-    p_reflect = p_reflect / (total * mono_gauss(phi, mos_sample, sq2pi));
+    p_reflect = (float)((double)p_reflect / ((double)total * mono_gauss(phi, mos_sample)));
     if (p_reflect > 1.0f) p_reflect = 1.0f;
     n.w *= p_reflect;
     /* the slab frame is rebuilt here rather than kept live through the

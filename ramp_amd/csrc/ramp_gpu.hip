@@ -515,7 +515,6 @@ static void derive_scat(DPart &s)
     }
     case RAMP_S_ISOTROPIC:
         set_df(s.d, 0, 2 * RC_M_PI_D);
-        s.d[2] = (float)RC_KB_D;
         break;
     case RAMP_S_POWDER1:
     case RAMP_S_POWDERN: {
@@ -535,8 +534,6 @@ static void derive_scat(DPart &s)
         float sw = s.p[MO_SLAB_W], sh = s.p[MO_SLAB_H], gap = s.p[MO_GAP];
         s.d[4] = (((float)s.ni[MOI_NH] * (sw + gap)) - gap) / 2.0f;
         s.d[5] = (((float)s.ni[MOI_NV] * (sh + gap)) - gap) / 2.0f;
-        s.d[8] = (float)sqrt(2.0f * RC_M_PI_D);
-        set_df(s.d, 9, 2.0f * RC_M_PI_D);
         break;
     }
     case RAMP_S_DET1D:

@@ -18,6 +18,7 @@
 #define RC_M_PI_D 3.14159265358979323846 /* double M_PI of OpenCL */
 #define RC_MIN2RAD_D (RC_M_PI_D / (double)(180.0f * 60.0f))
 #define RC_KB_D (8.6173 * (double)10e-2f)
+#define RC_SQRT_2PI_D 0x1.40d931ff62705p+1 /* sqrt(2.0f*M_PI) in double (monochromator.cl:7) */
 #define RC_MEV2J 1.6021765e-22f /* cont_mod.cl:38 */
 #define RC_ISIS_WSCALE 6.2415093e+12f /* isis_mod.cl:94 */
 #define RC_SENTINEL 100000.0f  /* intersection sentinel t (e.g. scat/arm.cl:27) */

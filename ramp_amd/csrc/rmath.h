@@ -453,4 +453,115 @@ RHD float rm_fmod(float x, float y)
     return rm_u2f(out | sx);
 }
 
+/* ---- FP64 exp and cos ----------------------------------------------------
+ * For the reference's double-precision sites that call a transcendental
+ * (the monochromator's Box-Muller cos(2.0f*M_PI*u), the isotropic sample's
+ * detailed-balance exp(-w/(kB*T))): one implementation of basic IEEE double
+ * operations, so the oracle and the kernels agree bit for bit.
+ *   exp: x = k ln2 + r, |r| <= ln2/2, with ln2 split into a 42-bit head (k
+ *        times it is exact for |k| < 2^11) and a tail; e^r by its Taylor
+ *        series to degree 13 (next term < 4e-18); exact scaling by 2^k.
+ *   cos: x = k pi/2 + r, |r| <= pi/4, three-part pi/2 (33 + 33 bits + tail,
+ *        exact products for |k| < 2^20, i.e. |x| below about 1.6e6); the
+ *        Taylor series of cos r (degree 20) or sin r (degree 21) chosen by
+ *        k mod 4 (next terms < 5e-24).
+ * The series coefficients are 1/n! rounded to double. */
+#define RD_INV_LN2 0x1.71547652b82fep+0
+#define RD_LN2_HI 0x1.62e42fefa38p-1
+#define RD_LN2_LO 0x1.ef35793c7673p-45
+#define RD_EXP_MAX 0x1.62e42fefa39efp+9
+#define RD_2_OVER_PI 0x1.45f306dc9c883p-1
+#define RD_PIO2_1 0x1.921fb544p+0
+#define RD_PIO2_2 0x1.0b4611a6p-34
+#define RD_PIO2_3 0x1.3198a2e037073p-69
+#define RD_F2 0x1.0000000000000p-1
+#define RD_F3 0x1.5555555555555p-3
+#define RD_F4 0x1.5555555555555p-5
+#define RD_F5 0x1.1111111111111p-7
+#define RD_F6 0x1.6c16c16c16c17p-10
+#define RD_F7 0x1.a01a01a01a01ap-13
+#define RD_F8 0x1.a01a01a01a01ap-16
+#define RD_F9 0x1.71de3a556c734p-19
+#define RD_F10 0x1.27e4fb7789f5cp-22
+#define RD_F11 0x1.ae64567f544e4p-26
+#define RD_F12 0x1.1eed8eff8d898p-29
+#define RD_F13 0x1.6124613a86d09p-33
+#define RD_F14 0x1.93974a8c07c9dp-37
+#define RD_F15 0x1.ae7f3e733b81fp-41
+#define RD_F16 0x1.ae7f3e733b81fp-45
+#define RD_F17 0x1.952c77030ad4ap-49
+#define RD_F18 0x1.6827863b97d97p-53
+#define RD_F19 0x1.2f49b46814157p-57
+#define RD_F20 0x1.e542ba4020225p-62
+#define RD_F21 0x1.71b8ef6dcf572p-66
+
+RHD double rd_exp(double x)
+{
+    if (!(x == x)) return x;
+    if (x > RD_EXP_MAX) return 1.0 / 0.0;
+    if (x < -745.2) return 0.0;
+    double kd = __builtin_rint(x * RD_INV_LN2);
+    double r = (x - kd * RD_LN2_HI) - kd * RD_LN2_LO;
+    double p = RD_F13;
+    p = p * r + RD_F12;
+    p = p * r + RD_F11;
+    p = p * r + RD_F10;
+    p = p * r + RD_F9;
+    p = p * r + RD_F8;
+    p = p * r + RD_F7;
+    p = p * r + RD_F6;
+    p = p * r + RD_F5;
+    p = p * r + RD_F4;
+    p = p * r + RD_F3;
+    p = p * r + RD_F2;
+    p = p * r + 1.0;
+    p = p * r + 1.0;
+    return __builtin_ldexp(p, (int)kd);
+}
+
+RHD double rd_cos_kernel(double r)
+{
+    double z = r * r;
+    double p = RD_F20;
+    p = p * z - RD_F18;
+    p = p * z + RD_F16;
+    p = p * z - RD_F14;
+    p = p * z + RD_F12;
+    p = p * z - RD_F10;
+    p = p * z + RD_F8;
+    p = p * z - RD_F6;
+    p = p * z + RD_F4;
+    p = p * z - RD_F2;
+    return p * z + 1.0;
+}
+
+RHD double rd_sin_kernel(double r)
+{
+    double z = r * r;
+    double p = RD_F21;
+    p = p * z - RD_F19;
+    p = p * z + RD_F17;
+    p = p * z - RD_F15;
+    p = p * z + RD_F13;
+    p = p * z - RD_F11;
+    p = p * z + RD_F9;
+    p = p * z - RD_F7;
+    p = p * z + RD_F5;
+    p = p * z - RD_F3;
+    return r + (r * z) * p;
+}
+
+RHD double rd_cos(double x)
+{
+    if (!(x - x == 0.0)) return x - x;   /* NaN for NaN and infinities */
+    double kd = __builtin_rint(x * RD_2_OVER_PI);
+    double r = ((x - kd * RD_PIO2_1) - kd * RD_PIO2_2) - kd * RD_PIO2_3;
+    switch ((int)((long long)kd & 3)) {
+    case 0: return rd_cos_kernel(r);
+    case 1: return -rd_sin_kernel(r);
+    case 2: return -rd_cos_kernel(r);
+    default: return rd_sin_kernel(r);
+    }
+}
+
 #endif

@@ -2,7 +2,8 @@
 used as the headline benchmark workload.
 
 - Source: MCont, T1 = T2 = T3 = 40 K, I1 = 1, I2 = I3 = 0, with the 1/E
-  epithermal tail joined at 5 k_B T; E in [1.1, 9.0] meV; 0.04 x 0.09 m face
+  epithermal tail joined at 2 k_B T (6.89 meV, inside the window); E in
+  [1.1, 9.0] meV; 0.04 x 0.09 m face
   aimed at the 0.04 x 0.09 m guide entrance at 1.7 m.
 - Guide: 20 straight supermirror segments between z = 1.7 m and 26.7 m, 2 mm
   gaps between segments, width tapering linearly 0.04 -> 0.02 m, height
@@ -56,7 +57,7 @@ def build():
             "name": "MCont", "mod_dim": [0.04, 0.09], "target_dim": [0.04, 0.09],
             "target_dist": 1.7, "E_min": 1.1, "E_max": 9.0,
             "T1": 40.0, "I1": 1.0, "T2": 40.0, "I2": 0.0, "T3": 40.0, "I3": 0.0,
-            "tail_join": 5.0,
+            "tail_join": 2.0,
         },
     }
     b = boundaries()

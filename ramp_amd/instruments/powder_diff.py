@@ -20,8 +20,10 @@ def theta0(n=40, seed=5):
     return np.concatenate([[300.0, 0.2, 0.1, 0.005], d])
 
 
-def variables(theta, F2):
-    """JSON variables of powder_diff.json for parameter vector theta."""
+def variables(theta, F2, tail_join=0.0):
+    """JSON variables of powder_diff.json for parameter vector theta;
+    tail_join > 0 adds the source's 1/E tail above tail_join * k_B * T."""
     theta = np.asarray(theta, dtype=np.float64)
     return {"T": float(theta[0]), "sigma_inc": float(theta[1]), "sigma_abs": float(theta[2]),
-            "U": float(theta[3]), "d": [float(x) for x in theta[4:]], "F2": [float(x) for x in F2]}
+            "U": float(theta[3]), "d": [float(x) for x in theta[4:]], "F2": [float(x) for x in F2],
+            "tail_join": float(tail_join)}

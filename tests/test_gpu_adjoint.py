@@ -16,10 +16,11 @@ import powder_diff  # noqa: E402
 PD = os.path.join(INSTR, "powder_diff.json")
 
 
-def _dinst(n, seeds=(11,)):
+def _dinst(n, seeds=(11,), tail_join=0.0):
     from ramp_amd.diff import DiffInstrument
     d, F2 = powder_diff.lines()
-    return DiffInstrument(PD, n_hist=n, seeds=seeds, **powder_diff.variables(powder_diff.theta0(), F2))
+    return DiffInstrument(PD, n_hist=n, seeds=seeds,
+                          **powder_diff.variables(powder_diff.theta0(), F2, tail_join))
 
 
 def _target(di, seed):
@@ -34,11 +35,13 @@ def _target(di, seed):
 
 
 @pytest.mark.gpu
-def test_adjoint_equals_fp64_cpu_reference():
+@pytest.mark.parametrize("tail_join", [0.0, 2.0])
+def test_adjoint_equals_fp64_cpu_reference(tail_join):
+    # tail_join 2: the source's 1/E tail above 2 k_B T (52 meV at 300 K)
     from oracle import pyoracle
     n = 200_000
-    di = _dinst(n)
-    print(f"adjoint vs FP64 oracle N={n}", flush=True)
+    di = _dinst(n, tail_join=tail_join)
+    print(f"adjoint vs FP64 oracle N={n} tail_join={tail_join}", flush=True)
     rng = np.random.default_rng(0)
     dl = rng.normal(size=di.n_bins).astype(np.float32)
     g = di.adjoint(di.theta0, dl)

@@ -273,7 +273,7 @@ def test_incoherent_focusing_preserves_weight_into_the_cone(tmp_path, oracle):
     assert wi > 0 and abs(wf / wi - 1.0) < rel
 
 
-def _pd_build(N):
+def _pd_build(N, tail_join=0.0):
     import sys
     import os as _os
     sys.path.insert(0, _os.path.join(_os.path.dirname(_os.path.dirname(_os.path.abspath(__file__))),
@@ -283,7 +283,7 @@ def _pd_build(N):
     path = _os.path.join(_os.path.dirname(_os.path.dirname(_os.path.abspath(__file__))), "ramp_amd",
                          "instruments", "powder_diff.json")
     d, F2 = powder_diff.lines()
-    v = powder_diff.variables(powder_diff.theta0(), F2)
+    v = powder_diff.variables(powder_diff.theta0(), F2, tail_join)
     di = DiffInstrument.__new__(DiffInstrument)
     di.fn, di.variables, di.sample_name, di.chunk = path, v, "sample", N
     base = ramp_amd.Instrument(path, **v)
@@ -294,11 +294,14 @@ def _pd_build(N):
     return di, powder_diff.theta0()
 
 
-def test_oracle_adjoint_is_the_derivative_of_the_forward(oracle):
+@pytest.mark.parametrize("tail_join", [0.0, 2.0])
+def test_oracle_adjoint_is_the_derivative_of_the_forward(oracle, tail_join):
     # the FP64 reference gradient equals same-seed central differences of
-    # the oracle's own forward histogram (weights are smooth in theta)
+    # the oracle's own forward histogram (weights are smooth in theta); with
+    # the source's 1/E tail above 2 k_B T (52 meV at 300 K) the temperature
+    # derivative of the tail weight is 1/T
     N = 200000
-    di, th0 = _pd_build(N)
+    di, th0 = _pd_build(N, tail_join)
     rng = np.random.default_rng(1)
     dl = rng.normal(size=2048 * 256).astype(np.float32)
 

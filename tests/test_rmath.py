@@ -145,3 +145,37 @@ def test_sin_cos_of_signed_zero_are_exact(lib):
                                        y.ctypes.data_as(C.POINTER(C.c_float)),
                                        o.ctypes.data_as(C.POINTER(C.c_float)), 2, 0))
         assert (o.view(np.uint32) == want).all()
+
+
+def _math_d(oracle, fn, x):
+    x = np.ascontiguousarray(x, np.float64)
+    out = np.zeros_like(x)
+    oracle.load().oracle_math_d(fn, x.ctypes.data_as(C.POINTER(C.c_double)),
+                                out.ctypes.data_as(C.POINTER(C.c_double)), x.size)
+    return out
+
+
+def _ulp64(got, ref):
+    return np.abs(got - ref) / np.spacing(np.abs(ref))
+
+
+def test_fp64_exp_and_cos_accuracy(oracle):
+    # rd_exp and rd_cos (rmath.h): the reference's double-precision sites
+    # (detailed balance, Box-Muller); within 2 ulp of numpy's float64
+    rng = np.random.default_rng(7)
+    x = np.concatenate([rng.uniform(-700, 700, 100000), rng.uniform(-2, 2, 100000),
+                        rng.uniform(-1e-6, 1e-6, 1000)])
+    e = _ulp64(_math_d(oracle, 0, x), np.exp(x))
+    assert e.max() <= 2.0, (e.max(), x[np.argmax(e)])
+    # Box-Muller's argument 2 pi u, u in [0, 1]; and a wider range
+    u = rng.uniform(0, 1, 200000)
+    a = np.concatenate([2.0 * np.pi * u, rng.uniform(-1e3, 1e3, 100000)])
+    got, ref = _math_d(oracle, 1, a), np.cos(a)
+    # absolute error near the zeros of cos, relative elsewhere
+    err = np.abs(got - ref) / np.maximum(np.spacing(np.abs(ref)), np.spacing(1.0) * 0.5)
+    assert err.max() <= 2.0, (err.max(), a[np.argmax(err)])
+    sp = _math_d(oracle, 0, np.array([0.0, 1.0, -np.inf, np.inf, np.nan, 710.0, -746.0]))
+    assert sp[0] == 1.0 and abs(sp[1] - np.e) <= np.spacing(np.e) and sp[2] == 0.0 and np.isinf(sp[3]) and np.isnan(sp[4])
+    assert np.isinf(sp[5]) and sp[6] == 0.0
+    cp = _math_d(oracle, 1, np.array([0.0, np.inf, np.nan, np.pi]))
+    assert cp[0] == 1.0 and np.isnan(cp[1]) and np.isnan(cp[2]) and cp[3] == -1.0
