@@ -11,7 +11,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "liboracle.so")
+# RAMP_ORACLE_LIB selects another build of the same source (the sanitizer
+# build, liboracle_san.so)
+LIB = os.environ.get("RAMP_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 _lib = None
 
 

@@ -8,21 +8,36 @@
  * types; sites where the reference promotes to double (M_PI, unsuffixed
  * literals) are evaluated in double here and marked "FP64".
  *
- * Parity status of each function (how it is pinned):
+ * Parity status of each function (how it is pinned; tests/test_oracle.py
+ * and tests/test_oracle_pins.py unless noted):
  *   rand stream            golden vectors from the reference's Random123
  *                          header (tests/test_rng.py)
- *   frame transforms       orthonormality / round-trip closed forms
- *   geometry kernels       analytic ray-shape solutions (tests/test_oracle.py)
- *   sources                sampled moments; weight closed forms
- *   chopper                analytic open windows
- *   guide                  R=1 transmission; reflectivity closed form
- *   powder                 Bragg 2theta = 2 asin(lambda/2d)
- *   incoherent/absorber    Beer-Lambert closed forms
- *   detectors              binning arithmetic, weight conservation
- *   isotropic, mono, ISIS  parity unpinned beyond moments / conservation
+ *   frame transforms       quarter turns, orthonormality, round trips
+ *   geometry kernels       analytic chords (Beer-Lambert through sphere,
+ *                          cylinder, box); a mesh cube equals GBox
+ *   MFlat / MCont          energy moments; every weight against the
+ *                          Maxwellian closed form, its peak and kT values,
+ *                          the 1/E tail; NaN at T = 0
+ *   MISIS                  a cut of the reference's Let_Base.mcstas: block
+ *                          totals, edges, window weights, sampled time and
+ *                          energy marginals against the table
+ *   chopper                duty cycle
+ *   guide                  unit reflectivity; single-bounce weights against
+ *                          the McStas supermirror formula, every branch
+ *   powder 1 / N           Bragg angles; line intensity ratios
+ *   monochromator          Bragg angle, mosaic reflectivity, quadrature
+ *                          normalisation, Gauss-Legendre table
+ *   isotropic              table construction (legacy Simpson rule exact
+ *                          cases, separable S), kinematics, Q sampling,
+ *                          detailed-balance branch
+ *   incoherent/absorber    forced-scattering weight, Beer-Lambert
+ *   detectors, beamstop    binning arithmetic (1D floor, 2D round/alias,
+ *                          tubes floor); the beamstop's exact shadow
  *   MCPL in/out            header and record layout of the reference's
  *                          files; unpack(pack(u)) = u on the McStas file's
  *                          records; the reference writer's packing quirks
+ *                          (tests/test_mcpl.py); last-ulp conversion
+ *                          parity unpinned
  */
 #include <math.h>
 #include <stdio.h>
