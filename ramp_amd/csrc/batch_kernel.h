@@ -46,7 +46,7 @@ void k_batch(BatchArgs b)
     DeferQ dq;
     dq.b = batch_lds;
     dq.n = 0;
-    if constexpr ((F & FEAT_MONO) != 0) defer_init(dq, batch_lds);
+    if constexpr ((F & FEAT_MONO) != 0) defer_init(dq, batch_lds, DQ_CAP * DQ_WORDS);
     /* every wave leaves once the counter has passed ntiles */
     for (;;) {
         uint32_t t = 0;
@@ -76,7 +76,7 @@ void k_batch(BatchArgs b)
             if constexpr ((F & FEAT_MONO) != 0) {
                 if (defer && dq.n >= 64u) {
                     NS r;
-                    defer_pop(dq, r);
+                    defer_pop<DQ_CAP, DQ_WORDS>(dq, r);
                     defer_finish<F, BATCH_SP(F)>(r, a, hc, a.s1, dq);
                 }
             }
@@ -84,7 +84,7 @@ void k_batch(BatchArgs b)
         if constexpr ((F & FEAT_MONO) != 0) {
             while (defer && dq.n > 0u) {
                 NS r;
-                defer_pop(dq, r);
+                defer_pop<DQ_CAP, DQ_WORDS>(dq, r);
                 defer_finish<F, BATCH_SP(F)>(r, a, hc, a.s1, dq);
             }
         }

@@ -11,11 +11,19 @@
  * the triangle index.
  *
  * A last pass writes the tree in depth-first preorder as 32-byte BvhFlat
- * nodes: the left child of an internal node is the next node, and `esc` is
- * the node after the subtree (preorder index + subtree size), so traversal
- * needs neither child pointers nor a stack.  A node's preorder index is its
- * depth plus, over its ancestors whose right subtree holds it, the sizes of
- * their left subtrees.
+ * nodes: the first-visited child of an internal node is the next node, and
+ * `esc` is the node after the subtree (the layout's start + preorder index
+ * + subtree size), so traversal needs neither child pointers nor a stack.
+ * The same pass writes the tree of the host SAH builder (bvh_sah.h), whose
+ * leaves may hold two triangles; the leaves' triangles are copied in leaf
+ * order into BvhTri records (k_bvh_leaf_tris here, on the host there).  A node's preorder
+ * index is its depth plus, over its ancestors whose second-visited subtree
+ * holds it, the sizes of their first-visited subtrees.  Six such layouts
+ * are written, one per ray direction class (+x, -x, +y, -y, +z, -z): in
+ * layout 2a + s the first-visited child is the one whose box centre lies
+ * nearer along axis a for a ray running in that direction, so a ray
+ * traversed in the layout of its dominant direction meets nearer triangles
+ * first and its nearest-hit bounds prune the farther subtrees early.
  *
  * Every leaf box is the triangle's box widened by a margin of 1e-3 of the
  * mesh extent (and at least 1e-6 m), far above the rounding of both the
@@ -107,6 +115,7 @@ __global__ void k_bvh_boxes(const float *pts, const uint32_t *order, int n, floa
     lf.left = -1;
     lf.right = k;
     lf.count = 1;               /* subtree size */
+    lf.ntri = 1;
     if (n == 1) return;
     __threadfence();
     int p = parent[node];
@@ -135,29 +144,54 @@ __global__ void k_bvh_boxes(const float *pts, const uint32_t *order, int n, floa
     }
 }
 
-/* Preorder flattening, one thread per build-layout node (2n-1 of them).
- * Launched after k_bvh_boxes has completed, so plain loads see its writes. */
-__global__ void k_bvh_flatten(const BvhNode *nodes, const int *parent, const uint32_t *order, int n,
-                              BvhFlat *flat)
+/* the child of internal node p visited first in layout `lay` (axis lay/2,
+ * positive direction for even lay): the one whose box centre is nearer */
+__device__ static inline int bvh_first_child(const BvhNode *nodes, int p, int lay)
+{
+    const int a = lay >> 1;
+    const int l = nodes[p].left, r = nodes[p].right;
+    const float cl = nodes[l].lo[a] + nodes[l].hi[a], cr = nodes[r].lo[a] + nodes[r].hi[a];
+    if (lay & 1) return cl >= cr ? l : r;
+    return cl <= cr ? l : r;
+}
+
+/* Preorder flattening of layout `lay`, one thread per build-layout node
+ * (nn of them).  Launched after k_bvh_boxes has completed, so plain loads
+ * see its writes. */
+__global__ void k_bvh_flatten(const BvhNode *nodes, const int *parent, int nn, int lay, BvhFlat *flat)
 {
     int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= 2 * n - 1) return;
+    if (x >= nn) return;
     int pre = 0;
     int y = x;
     /* depth is at most 30 Morton bits + 32 index bits of splits */
     for (int guard = 0; guard < 96; guard++) {
         int p = parent[y];
         if (p < 0) break;
-        int l = nodes[p].left;
-        pre += (y == l) ? 1 : 1 + nodes[l].count;
+        int f = bvh_first_child(nodes, p, lay);
+        pre += (y == f) ? 1 : 1 + nodes[f].count;
         y = p;
     }
     const BvhNode &nd = nodes[x];
     BvhFlat f;
     for (int a = 0; a < 3; a++) { f.lo[a] = nd.lo[a]; f.hi[a] = nd.hi[a]; }
-    f.esc = pre + nd.count;
-    f.tri = (nd.left < 0) ? (int32_t)order[nd.right] : -1;
+    f.esc = lay * nn + pre + nd.count;   /* absolute in the six layouts */
+    const int cnt = nd.ntri > 0 ? nd.ntri : 1;
+    f.tri = (nd.left < 0) ? (int32_t)((uint32_t)nd.right | ((uint32_t)(cnt - 1) << BVH_LEAF_POS_BITS)) : -1;
     flat[pre] = f;
+}
+
+/* the leaf-order triangle records: lt[k] holds triangle order[k] */
+__global__ void k_bvh_leaf_tris(const float *pts, const uint32_t *order, int n, BvhTri *lt)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t t = order[k];
+    BvhTri r;
+    for (int i = 0; i < 9; i++) r.v[i] = pts[9 * (size_t)t + i];
+    r.idx = t;
+    r.pad[0] = r.pad[1] = 0u;
+    lt[k] = r;
 }
 
 #endif

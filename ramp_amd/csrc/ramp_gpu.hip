@@ -32,6 +32,7 @@
 #include "../../include/ramp.h"
 #include "dev_components.h"
 #include "bvh_build.h"
+#include "bvh_sah.h"
 #include "det_scale.h"
 
 /* ======================================================================== */
@@ -94,6 +95,7 @@ __global__ __launch_bounds__(1024) void k_scan(const uint32_t *counts, uint64_t 
         prefix[nreg] = part[1023];
         cons_meta[0] = prod_meta[0];
         cons_meta[1] = nreg;
+        if (!cnt) return;   /* a walk list: no queue counters */
         cnt[CNT_QOUT] += part[1023];
         cnt[CNT_QIN] += part[1023];
         if (first) cnt[CNT_QOUT0] += part[1023];
@@ -177,6 +179,133 @@ __global__ void k_rng(const uint32_t *tid, const float *ctr, float *out, uint64_
          i += (uint64_t)gridDim.x * blockDim.x) {
         float c = ctr[i];
         out[i] = rr_draw(&c, tid[i]);
+    }
+}
+
+/* ---- mesh walks in their own launch ----------------------------------------
+ * The walk of a ray over a mesh's BVH varies several-fold in length from
+ * ray to ray, so walks run in lockstep inside a segment leave most lanes of
+ * a wave idle.  For the plan's first linear GMesh step, the segment before
+ * it prepares every output neutron (walk_emit: the mesh frame, the box
+ * precheck, 1/v) and lists the rays that need a walk; k_mesh_walk walks
+ * them with lane refill, and the segment starting at the step takes the
+ * hits from the neutron's queue slot.  Every ray runs the same walk and
+ * arithmetic as in g_mesh.
+ *
+ * A lane whose walk is over takes the next ray from its wave's pool of
+ * claimed rays, which the wave refills from a device counter, one atomic
+ * per WALK_POOL rays (device-scope atomics on one address are serialised
+ * across the XCDs).  Lanes alternate between box steps and triangle tests:
+ * a lane whose box step meets a leaf parks the leaf and waits; the wave
+ * tests the parked leaves' triangles together once WALK_TRI_BATCH lanes
+ * hold one (or no lane can step), so the long triangle test runs with most
+ * lanes active instead of once per step for the few lanes at a leaf.  Refills
+ * likewise run once WALK_REFILL lanes are idle. */
+struct WalkArgs {
+    const float *list;         /* walk_list of the producing segment */
+    const uint64_t *prefix;    /* its region prefix [nreg + 1] */
+    const uint64_t *meta;      /* [0] region stride, [1] number of regions */
+    const DComp *comp;         /* the mesh component */
+    float *res;                /* [3][cap]: t1, t2, outcome per queue slot */
+    uint64_t cap;
+    unsigned long long *next;  /* work counter, zero at launch */
+};
+
+#define WALK_POOL 1024u     /* rays a wave claims per atomic on the work counter */
+#define WALK_REFILL 16u     /* idle lanes that trigger a refill pass */
+#define WALK_TRI_BATCH 24u  /* lanes with a parked leaf that trigger the tests */
+#define IDLE 0xffffffffu    /* a lane without a ray (queue slots stay below 2^32 - 1) */
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_mesh_walk(WalkArgs a)
+{
+    const DComp &c = plan_at(a.comp, 0);
+    const float *pts = c.g.tab[GT_POINTS];
+    const BvhTri *lt = (const BvhTri *)c.g.aux2;
+    const BvhFlat *nodes = (const BvhFlat *)c.g.aux;
+    const bool brute = nodes == nullptr;
+    const uint32_t ntri = c.g.ni[GI_NTRI], interior = c.g.ni[GI_INTERIOR];
+    const uint32_t lane = __lane_id();
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const uint64_t nreg = a.meta[1], stride = a.meta[0];
+    const uint64_t total = a.prefix[nreg];
+    const uint64_t C = a.cap;
+    uint32_t *code = (uint32_t *)(a.res + 2 * C);
+    /* wave-uniform: the region cursor, the wave's pool of claimed rays
+     * [pool, pool_end) and whether the counter has passed the total */
+    uint64_t cur = 0, pool = 0, pool_end = 0;
+    bool drained = false;
+    uint32_t my = IDLE;        /* this lane's queue slot */
+    int32_t tri = -1;          /* its parked leaf (brute force: triangle); -1 none */
+    MeshWalk w;
+    BvhFlat nd;                /* the lane's next node, loaded one step ahead */
+    for (;;) {
+        unsigned long long im = __ballot(my == IDLE);
+        if (!drained && im && ((uint32_t)__popcll(im) >= WALK_REFILL || im == __ballot(true))) {
+            if (pool >= pool_end) {
+                unsigned long long base = 0;
+                if (lane == 0) base = atomicAdd(a.next, (unsigned long long)WALK_POOL);
+                base = __shfl(base, 0);
+                if (base >= total) {
+                    drained = true;
+                } else {
+                    pool = base;
+                    pool_end = base + WALK_POOL < total ? base + WALK_POOL : total;
+                }
+            }
+            if (!drained) {
+                const uint64_t avail = pool_end - pool;
+                const uint32_t cnt = (uint32_t)__popcll(im);
+                const uint32_t take = (uint64_t)cnt < avail ? cnt : (uint32_t)avail;
+                const uint32_t rank = (uint32_t)__popcll(im & below);
+                cur = region_advance(a.prefix, nreg, cur, pool);
+                const uint64_t r1 = region_advance(a.prefix, nreg, cur, pool + take - 1);
+                if (((im >> lane) & 1ull) != 0ull && rank < take) {
+                    const uint64_t item = pool + rank;
+                    const uint64_t r = region_find(a.prefix, cur, r1 + 1, item);
+                    const uint64_t e = r * stride + (item - a.prefix[r]);
+                    const float *L = a.list;
+                    const rf3 pos = rf3_make(L[C + e], L[2 * C + e], L[3 * C + e]);
+                    const rf3 vel = rf3_make(L[4 * C + e], L[5 * C + e], L[6 * C + e]);
+                    const rf3 inv = rf3_make(L[7 * C + e], L[8 * C + e], L[9 * C + e]);
+                    mesh_walk_start(w, c.g, pos, vel, inv);
+                    if (!brute && w.cur < w.end) nd = nodes[w.cur];
+                    my = ((const uint32_t *)L)[e];
+                    tri = -1;
+                }
+                pool += take;
+            }
+        }
+        const unsigned long long walking = __ballot(my != IDLE);
+        if (!walking) {
+            if (drained) break;
+            continue;
+        }
+        /* the parked triangles */
+        const unsigned long long parked = __ballot(tri >= 0);
+        if ((uint32_t)__popcll(parked) >= WALK_TRI_BATCH || (parked && parked == walking)) {
+            if (tri >= 0) {
+                if (brute) mesh_walk_brute(w, pts, (uint32_t)tri);
+                else mesh_walk_leaf(w, lt, tri);
+                tri = -1;
+            }
+        }
+        /* one box step of every walking lane without a parked triangle */
+        if (my != IDLE && tri < 0) {
+            if (w.cur < w.end) {
+                if (!brute) {
+                    tri = mesh_walk_node(w, nd, ntri, interior != 0);
+                    if (w.cur < w.end) nd = nodes[w.cur];   /* consumed at the lane's next step */
+                } else {
+                    tri = w.cur++;
+                }
+            }
+            if (tri < 0 && w.cur >= w.end) {
+                a.res[my] = w.t[0];
+                a.res[C + my] = w.t[1];
+                code[my] = mesh_walk_code(w, interior);
+                my = IDLE;
+            }
+        }
     }
 }
 
@@ -292,6 +421,10 @@ struct ramp_plan {
     bool has_atomic = false, has_det = false;
     std::vector<int> bounds;  /* segment boundaries (step indices) */
     int defer_k = -1;         /* first linear monochromator step (reflections deferred) */
+    int walk_k = -1;          /* first linear GMesh step: its walks run in k_mesh_walk (fast mode) */
+    /* the walks: hits [3][queue slots], the walk list [10][queue slots], its
+     * region counts, and [prefix (waves + 1), meta (2), work counter (1)] */
+    DevBuf walk_res, walk_list, walk_counts, walk_aux;
     int revive_k = -1;        /* last SMCPLIn step: segment 0 runs at least through it */
     bool has_mcpl = false;    /* SMCPLIn/SMCPLOut present */
     bool need_full = false;   /* MCPL, SRescal or SAnalyser present: full-state execution */
@@ -564,14 +697,98 @@ __global__ void k_fill_i32(int *p, uint64_t n, int v)
         p[i] = v;
 }
 
-static int build_mesh(ramp_plan *p, DPart &g, const float *dpts, uint32_t ntri)
+/* The BVH of a GMesh: a binned-SAH tree built on the host (bvh_sah.h) up
+ * to BVH_SAH_MAX_TRIS triangles, the GPU LBVH (bvh_build.h) above that or
+ * when the host triangles are not at hand; RAMP_BVH_BUILDER=lbvh or =sah
+ * forces one.  Either tree is written as the six preorder layouts. */
+#define BVH_SAH_MAX_TRIS (1u << 21)
+
+/* the six preorder layouts of a build-layout tree of nn nodes (on the
+ * device), into a new buffer held by the plan */
+static int bvh_layouts(ramp_plan *p, DPart &g, const BvhNode *nodes, const int *parent, int nn)
+{
+    hipStream_t s = p->ctx->stream;
+    DevBuf flat;
+    int rc = dev_alloc(flat, BVH_LAYOUTS * (uint64_t)nn * sizeof(BvhFlat));
+    if (rc) return rc;
+    for (int lay = 0; lay < BVH_LAYOUTS; lay++)
+        hipLaunchKernelGGL(k_bvh_flatten, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, nodes, parent, nn,
+                           lay, (BvhFlat *)flat.p + (size_t)lay * nn);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        dev_free(flat);
+        return fail(RAMP_E_DEVICE, "BVH layouts: %s", hipGetErrorString(e));
+    }
+    g.aux = flat.p;
+    g.ni[GI_NNODES] = (uint32_t)nn;
+    p->mesh_nodes.push_back(flat);
+    return 0;
+}
+
+static int build_mesh_sah(ramp_plan *p, DPart &g, const float *hpts, uint32_t ntri, float margin)
+{
+    hipStream_t s = p->ctx->stream;
+    const int n = (int)ntri;
+    SahBuilder b;
+    b.P = hpts;
+    b.n = n;
+    b.margin = margin;
+    b.init();
+    int nn = 0;
+    b.build(0, n, -1, nn, 0);
+    /* the leaf-order triangle records */
+    std::vector<BvhTri> lt(n);
+    for (int k = 0; k < n; k++) {
+        const uint32_t t = b.order[k];
+        for (int i = 0; i < 9; i++) lt[k].v[i] = hpts[9 * (size_t)t + i];
+        lt[k].idx = t;
+        lt[k].pad[0] = lt[k].pad[1] = 0u;
+    }
+    DevBuf nodes, parent, tris;
+    int rc = dev_alloc(nodes, (uint64_t)nn * sizeof(BvhNode));
+    if (!rc) rc = dev_alloc(parent, (uint64_t)nn * sizeof(int));
+    if (!rc) rc = dev_alloc(tris, (uint64_t)n * sizeof(BvhTri));
+    hipError_t e = hipSuccess;
+    if (!rc) {
+        e = hipMemcpyAsync(nodes.p, b.nodes.data(), (size_t)nn * sizeof(BvhNode), hipMemcpyHostToDevice, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(parent.p, b.parent.data(), (size_t)nn * sizeof(int), hipMemcpyHostToDevice, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(tris.p, lt.data(), (size_t)n * sizeof(BvhTri), hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) rc = fail(RAMP_E_DEVICE, "BVH upload: %s", hipGetErrorString(e));
+    }
+    if (!rc) rc = bvh_layouts(p, g, (const BvhNode *)nodes.p, (const int *)parent.p, nn);
+    dev_free(nodes);
+    dev_free(parent);
+    if (rc) {
+        dev_free(tris);
+        return rc;
+    }
+    g.aux2 = tris.p;
+    p->mesh_nodes.push_back(tris);
+    return 0;
+}
+
+static int build_mesh(ramp_plan *p, DPart &g, const float *dpts, const float *hpts, uint32_t ntri)
 {
     g.aux = nullptr;
     g.aux2 = nullptr;
     if (ntri == 0) return 0;
+    if (ntri > BVH_LEAF_POS_MASK) return fail(RAMP_E_RANGE, "mesh of %u triangles", ntri);
     hipStream_t s = p->ctx->stream;
     int n = (int)ntri;
-    DevBuf keys, vals, skeys, order, nodes, parent, flags, temp;
+    bool sah = hpts != nullptr && ntri <= BVH_SAH_MAX_TRIS;
+    if (const char *f = std::getenv("RAMP_BVH_BUILDER")) {
+        if (std::strcmp(f, "lbvh") == 0) sah = false;
+        else if (std::strcmp(f, "sah") == 0 && hpts != nullptr) sah = true;
+    }
+    if (sah) {
+        float ext = 0.0f;
+        for (int a = 0; a < 3; a++) ext = std::max(ext, g.p[GP_XMAX + 2 * a] - g.p[GP_XMIN + 2 * a]);
+        return build_mesh_sah(p, g, hpts, ntri, std::max(1e-3f * ext, 1e-6f));
+    }
+    DevBuf keys, vals, skeys, order, nodes, parent, flags, temp, tris;
     int rc = dev_alloc(keys, n * 4ull);
     if (!rc) rc = dev_alloc(vals, n * 4ull);
     if (!rc) rc = dev_alloc(skeys, n * 4ull);
@@ -579,10 +796,12 @@ static int build_mesh(ramp_plan *p, DPart &g, const float *dpts, uint32_t ntri)
     if (!rc) rc = dev_alloc(nodes, (2ull * n - 1) * sizeof(BvhNode));
     if (!rc) rc = dev_alloc(parent, (2ull * n - 1) * 4ull);
     if (!rc) rc = dev_alloc(flags, (uint64_t)std::max(1, n - 1) * 4ull);
+    if (!rc) rc = dev_alloc(tris, (uint64_t)n * sizeof(BvhTri));
     auto cleanup = [&]() {
         dev_free(keys); dev_free(vals); dev_free(skeys); dev_free(parent); dev_free(flags); dev_free(temp);
+        dev_free(order); dev_free(nodes);
     };
-    if (rc) { cleanup(); dev_free(order); dev_free(nodes); return rc; }
+    if (rc) { cleanup(); dev_free(tris); return rc; }
     float lo[3] = {g.p[GP_XMIN], g.p[GP_YMIN], g.p[GP_ZMIN]};
     float hi[3] = {g.p[GP_XMAX], g.p[GP_YMAX], g.p[GP_ZMAX]};
     float ext = 0.0f, ie[3];
@@ -599,7 +818,7 @@ static int build_mesh(ramp_plan *p, DPart &g, const float *dpts, uint32_t ntri)
     HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tbytes, (uint32_t *)keys.p, (uint32_t *)skeys.p,
                                               (uint32_t *)vals.p, (uint32_t *)order.p, n, 0, 30, s));
     rc = dev_alloc(temp, tbytes + 16);
-    if (rc) { cleanup(); dev_free(order); dev_free(nodes); return rc; }
+    if (rc) { cleanup(); dev_free(tris); return rc; }
     HIPCHK(hipcub::DeviceRadixSort::SortPairs(temp.p, tbytes, (uint32_t *)keys.p, (uint32_t *)skeys.p,
                                               (uint32_t *)vals.p, (uint32_t *)order.p, n, 0, 30, s));
     hipLaunchKernelGGL(k_fill_i32, dim3(64), dim3(256), 0, s, (int *)parent.p, 2ull * n - 1, -1);
@@ -609,24 +828,17 @@ static int build_mesh(ramp_plan *p, DPart &g, const float *dpts, uint32_t ntri)
                            n, (BvhNode *)nodes.p, (int *)parent.p);
     hipLaunchKernelGGL(k_bvh_boxes, dim3(gblk), dim3(b), 0, s, dpts, (const uint32_t *)order.p, n, margin,
                        (BvhNode *)nodes.p, (const int *)parent.p, (int *)flags.p);
-    DevBuf flat;
-    rc = dev_alloc(flat, (2ull * n - 1) * sizeof(BvhFlat));
-    if (rc) { cleanup(); dev_free(order); dev_free(nodes); return rc; }
-    hipLaunchKernelGGL(k_bvh_flatten, dim3((unsigned)((2 * n - 1 + 255) / 256)), dim3(256), 0, s,
-                       (const BvhNode *)nodes.p, (const int *)parent.p, (const uint32_t *)order.p, n,
-                       (BvhFlat *)flat.p);
-    hipError_t e = hipStreamSynchronize(s);
+    hipLaunchKernelGGL(k_bvh_leaf_tris, dim3(gblk), dim3(b), 0, s, dpts, (const uint32_t *)order.p, n,
+                       (BvhTri *)tris.p);
+    /* six preorder layouts, one per ray direction class (bvh_build.h) */
+    rc = bvh_layouts(p, g, (const BvhNode *)nodes.p, (const int *)parent.p, 2 * n - 1);
     cleanup();
-    dev_free(order);
-    dev_free(nodes);
-    if (e != hipSuccess) {
-        dev_free(flat);
-        return fail(RAMP_E_DEVICE, "BVH build: %s", hipGetErrorString(e));
+    if (rc) {
+        dev_free(tris);
+        return rc;
     }
-    /* preorder node 0 is the root (for one triangle, the leaf itself) */
-    g.aux = flat.p;
-    g.aux2 = nullptr;
-    p->mesh_nodes.push_back(flat);
+    g.aux2 = tris.p;
+    p->mesh_nodes.push_back(tris);
     return 0;
 }
 
@@ -773,7 +985,8 @@ extern "C" int ramp_plan_create(ramp_ctx *ctx, const ramp_plan_desc *d, ramp_pla
             p->pd_lines.assign(d->tables + c.scat.tab_off[PDT_LINES],
                                d->tables + c.scat.tab_off[PDT_LINES] + (size_t)c.scat.ni[PDI_NLINES] * PD_LW);
         if (c.geom.type == RAMP_G_MESH)
-            PCHK(build_mesh(p, dc.g, dtab + c.geom.tab_off[GT_POINTS], c.geom.ni[GI_NTRI]));
+            PCHK(build_mesh(p, dc.g, dtab + c.geom.tab_off[GT_POINTS],
+                            d->tables ? d->tables + c.geom.tab_off[GT_POINTS] : nullptr, c.geom.ni[GI_NTRI]));
         dc.comp_idx = c.comp_idx;
         dc.restore = c.restore;
         dc.det = c.det;
@@ -824,6 +1037,9 @@ extern "C" int ramp_plan_create(ramp_ctx *ctx, const ramp_plan_desc *d, ramp_pla
     for (size_t k = 0; k < p->hsteps.size() && p->defer_k < 0; k++)
         if (p->hsteps[k].kind == 0 && p->hcomps[p->hsteps[k].first].s.type == RAMP_S_MONO)
             p->defer_k = (int)k;
+    for (size_t k = 0; k < p->hsteps.size() && p->walk_k < 0; k++)
+        if (p->hsteps[k].kind == 0 && p->hcomps[p->hsteps[k].first].g.type == RAMP_G_MESH)
+            p->walk_k = (int)k;
     for (size_t k = 0; k < p->hsteps.size(); k++)
         if (p->hsteps[k].kind == 0 && p->hcomps[p->hsteps[k].first].s.type == RAMP_S_MCPL_IN)
             p->revive_k = (int)k;
@@ -875,7 +1091,7 @@ extern "C" int ramp_plan_destroy(ramp_plan *p)
     }
     DevBuf *single[] = {&p->comps, &p->steps, &p->src, &p->dets, &p->tables, &p->cnt, &p->count,
                         &p->slot_ctr, &p->slot_iidx, &p->st16, &p->st8, &p->stii,
-                        &p->rcounts, &p->prefix, &p->comps_brute, &p->dump, &p->dump_tag,
+                        &p->rcounts, &p->prefix, &p->comps_brute, &p->dump, &p->dump_tag, &p->walk_res, &p->walk_list, &p->walk_counts, &p->walk_aux,
                         &p->sel_counts, &p->sel_prefix, &p->sel_out, &p->ev, &p->ev_tag,
                         &p->ev_list, &p->ev_list_n, &p->trace};
     for (DevBuf *b : single) dev_free(*b);
@@ -1056,25 +1272,26 @@ static seg_fn pick_kernel(bool full, bool first, int v)
     return kern_table(v).seg[full ? 1 : 0][first ? 1 : 0];
 }
 
-/* LDS of a segment kernel instance: the per-wave deferral queues */
-static size_t seg_lds_bytes(bool full, int v, unsigned wg)
+/* LDS per wave of a plan's monochromator deferral queue in fast mode */
+static uint32_t defer_lds_per_wave(const ramp_plan *p, bool full)
 {
-    return (!full && (kVariants[v] & FEAT_MONO)) ? (size_t)(wg / 64) * DQ_BYTES_PER_WAVE : 0;
+    return (full || p->defer_k < 0) ? 0u : (uint32_t)DQ_BYTES_PER_WAVE;
 }
 
 static void launch_segment(const SegArgs &a, bool full, bool first, int v, unsigned grid,
                            unsigned wg, hipStream_t s)
 {
     hipLaunchKernelGGL(pick_kernel(full, first, v), dim3(grid), dim3(wg),
-                       seg_lds_bytes(full, v, wg) + a.acc_bytes, s, a);
+                       (size_t)(wg / 64) * a.defer_lds + a.acc_bytes, s, a);
 }
 
 /* resident blocks per CU of a kernel instance (for a persistent grid) */
-static unsigned blocks_per_cu(bool full, bool first, int v, unsigned wg, uint32_t acc_bytes)
+static unsigned blocks_per_cu(bool full, bool first, int v, unsigned wg, uint32_t defer_lds, uint32_t acc_bytes)
 {
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pick_kernel(full, first, v), (int)wg,
-                                                     seg_lds_bytes(full, v, wg) + acc_bytes) != hipSuccess || nb < 1)
+                                                     (size_t)(wg / 64) * defer_lds + acc_bytes) != hipSuccess ||
+        nb < 1)
         nb = 1;
     return (unsigned)nb;
 }
@@ -1099,6 +1316,7 @@ static int choose_bounds(ramp_plan *p, const SegArgs &base, uint64_t chunk, unsi
         a.s0 = 0;
         a.s1 = ns;
         a.pilot = 1;
+        a.walk_role = 0;
         a.last = 1;
         a.carry = 0;
         a.keep = 0;
@@ -1127,6 +1345,10 @@ static int choose_bounds(ramp_plan *p, const SegArgs &base, uint64_t chunk, unsi
         }
     }
     p->bounds.push_back(ns);
+    /* a segment starts at the first mesh step: its walks run before it in
+     * k_mesh_walk over the queue of the segment before */
+    bool first_walk = max_seg != 1 && p->walk_k == 0;
+    if (max_seg != 1 && p->walk_k > 0) p->bounds.push_back(p->walk_k);
     /* a plugin step is a segment of its own (run by its plugin kernel) */
     bool first_plugin = false;
     for (int k = 0; k < ns; k++)
@@ -1137,7 +1359,7 @@ static int choose_bounds(ramp_plan *p, const SegArgs &base, uint64_t chunk, unsi
         }
     std::sort(p->bounds.begin(), p->bounds.end());
     p->bounds.erase(std::unique(p->bounds.begin(), p->bounds.end()), p->bounds.end());
-    if (first_plugin) p->bounds.insert(p->bounds.begin(), 0);  /* the source alone first */
+    if (first_plugin || first_walk) p->bounds.insert(p->bounds.begin(), 0);  /* the source alone first */
     if (p->bounds.size() == 1) p->bounds.push_back(ns);  /* no steps: the source segment alone */
     const size_t nb = p->bounds.size();
     if (nb >= 2 && p->bounds[nb - 1] == p->bounds[nb - 2] + 1 && p->plugin_step[p->bounds[nb - 2]] >= 0)
@@ -1394,6 +1616,7 @@ static int run_plan(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uint32_
     a.carry = carry ? 1 : 0;
     a.keep = keep ? 1 : 0;
     a.defer_k = full ? -1 : p->defer_k;
+    a.defer_lds = defer_lds_per_wave(p, full);
     a.revive_k = p->revive_k;
     a.dump = (float *)p->dump.p;
     a.dump_tag = (uint32_t *)p->dump_tag.p;
@@ -1424,8 +1647,8 @@ static int run_plan(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uint32_
     }
     int nseg = (int)p->bounds.size() - 1;
     /* persistent grids: as many blocks as are co-resident */
-    unsigned grid_first = (unsigned)ctx->n_cu * blocks_per_cu(full, true, p->variant, wg, p->acc_bytes);
-    unsigned grid_rest = (unsigned)ctx->n_cu * blocks_per_cu(full, false, p->variant, wg, p->acc_bytes);
+    unsigned grid_first = (unsigned)ctx->n_cu * blocks_per_cu(full, true, p->variant, wg, a.defer_lds, p->acc_bytes);
+    unsigned grid_rest = (unsigned)ctx->n_cu * blocks_per_cu(full, false, p->variant, wg, a.defer_lds, p->acc_bytes);
     uint64_t max_waves = (uint64_t)std::max(grid_first, grid_rest) * (wg / 64);
     if (nseg > 1 && noalloc) {
         if (max_waves > NA_MAX_WAVES || chunk + max_waves > NA_CAP)
@@ -1455,6 +1678,42 @@ static int run_plan(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uint32_
         bufs.rcounts = (uint32_t *)p->rcounts.p;
         bufs.prefix = (uint64_t *)p->prefix.p;
     }
+    /* mesh walks in their own launch (fast mode): the segment before the
+     * first mesh step lists the rays, k_mesh_walk walks them, the segment
+     * starting at the step takes their hits.  Queue slots index the hits
+     * (32-bit, in the neutron's aux), and the listing segment must be an
+     * ordinary one (not a plugin's). */
+    int walk_seg = -1;
+    unsigned walk_grid = 0;
+    const uint64_t walk_cap = chunk + max_waves;   /* the queues' slots */
+    const bool walk_ok = !full && !noalloc && p->walk_k >= 0 && p->plugin_step[p->walk_k] < 0 &&
+                         walk_cap < (1ull << 32);
+    if (walk_ok && p->walk_k > 0 && nseg > 1) {
+        for (int sg = 1; sg < nseg; sg++)
+            if (p->bounds[sg] == p->walk_k) walk_seg = sg;
+    } else if (walk_ok && p->walk_k == 0 && nseg > 1 && p->bounds[1] == 0) {
+        walk_seg = 1;
+    }
+    if (walk_seg > 1) {
+        const int ps = walk_seg - 1;
+        if (p->bounds[ps + 1] == p->bounds[ps] + 1 && p->plugin_step[p->bounds[ps]] >= 0) walk_seg = -1;
+    }
+    if (walk_seg > 0) {
+        rc = dev_alloc(p->walk_res, 3 * walk_cap * sizeof(float));
+        if (!rc) rc = dev_alloc(p->walk_list, 10 * walk_cap * sizeof(float));
+        if (!rc) rc = dev_alloc(p->walk_counts, max_waves * sizeof(uint32_t));
+        if (!rc) rc = dev_alloc(p->walk_aux, (max_waves + 4) * sizeof(uint64_t));
+        if (rc) return rc;
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_mesh_walk, 256, 0) != hipSuccess || nb < 1) nb = 1;
+        walk_grid = (unsigned)ctx->n_cu * (unsigned)nb;
+    }
+    a.walk_role = 0;
+    a.walk_ci = walk_seg > 0 ? (uint32_t)p->hsteps[p->walk_k].first : 0u;
+    a.walk_res = walk_seg > 0 ? (float *)p->walk_res.p : nullptr;
+    a.walk_list = walk_seg > 0 ? (float *)p->walk_list.p : nullptr;
+    a.walk_counts = walk_seg > 0 ? (uint32_t *)p->walk_counts.p : nullptr;
+    a.walk_cap = walk_cap;
     uint64_t *meta = (uint64_t *)p->count.p; /* [0..1] producer, [2..3] consumer */
     a.out_counts = bufs.rcounts;
     a.out_meta = meta;
@@ -1507,6 +1766,7 @@ static int run_plan(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uint32_
             }
             a.last = (sg == nseg - 1) ? 1 : 0;
             a.pilot = 0;
+            a.walk_role = walk_seg < 0 ? 0u : sg == walk_seg ? WALK_USE : sg + 1 == walk_seg ? WALK_EMIT : 0u;
             if (sg >= 1) a.in = bufs.q[cur];
             a.out = bufs.q[sg == 0 ? 0 : cur ^ 1];
             unsigned grid;
@@ -1525,6 +1785,24 @@ static int run_plan(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uint32_
                                    nreg, (const uint64_t *)meta, bufs.prefix, meta + 2,
                                    (unsigned long long *)p->cnt.p, sg == 0 ? 1 : 0);
                 HIPCHK(hipGetLastError());
+                if (sg + 1 == walk_seg) {
+                    uint64_t *wp = (uint64_t *)p->walk_aux.p;   /* prefix, meta, counter */
+                    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, (const uint32_t *)p->walk_counts.p,
+                                       nreg, (const uint64_t *)meta, wp, wp + max_waves + 1,
+                                       (unsigned long long *)nullptr, 0);
+                    HIPCHK(hipGetLastError());
+                    launch_fill_u64(s, wp + max_waves + 3, 1);
+                    WalkArgs wa;
+                    wa.list = (const float *)p->walk_list.p;
+                    wa.prefix = wp;
+                    wa.meta = wp + max_waves + 1;
+                    wa.comp = a.comps + a.walk_ci;   /* the brute-force copy when flagged */
+                    wa.res = (float *)p->walk_res.p;
+                    wa.cap = walk_cap;
+                    wa.next = (unsigned long long *)(wp + max_waves + 3);
+                    hipLaunchKernelGGL(k_mesh_walk, dim3(walk_grid), dim3(256), 0, s, wa);
+                    HIPCHK(hipGetLastError());
+                }
             }
         }
         if (p->n_rescal) {

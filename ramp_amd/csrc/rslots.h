@@ -53,6 +53,7 @@
 #define GP_ZMAX 5
 #define GI_NTRI 0
 #define GI_INTERIOR 1
+#define GI_NNODES 2           /* derived: nodes of one BVH layout (filled by libramp) */
 #define GT_POINTS 0
 #define GT_BVH 1              /* device-side BVH (filled by libramp, unused by the oracle) */
 

@@ -10,6 +10,9 @@
 /* ======================================================================== */
 /* segment kernel                                                           */
 /* ======================================================================== */
+#define WALK_EMIT 1u
+#define WALK_USE 2u
+
 struct SegArgs {
     const DComp *__restrict__ comps;
     const DStep *__restrict__ steps;
@@ -23,6 +26,21 @@ struct SegArgs {
     int carry;
     int keep;
     int defer_k;                /* step whose monochromator reflections are deferred (-1: none) */
+    uint32_t defer_lds;         /* LDS bytes per wave of the deferral queue (0: none) */
+    /* mesh walks in their own launch (k_mesh_walk, fast mode).  WALK_EMIT:
+     * the segment prepares each output neutron for the walk against
+     * component walk_ci (its walk-list entries per wave in walk_counts;
+     * a ray that misses the mesh's box gets outcome 3 at once).  WALK_USE:
+     * the segment starts at that GMesh step and takes the hits instead of
+     * walking.  walk_res [0][slot] t1, [1][slot] t2, [2][slot] outcome, per
+     * queue slot; walk_list [10][entry]: queue slot, pos, vel and 1/vel in
+     * the mesh's frame.  Zero-initialised arguments switch both off. */
+    uint32_t walk_role;
+    uint32_t walk_ci;
+    float *walk_res;
+    float *walk_list;
+    uint32_t *walk_counts;
+    uint64_t walk_cap;
     int num_sim;
     int seed;
     uint64_t n;                 /* items of the first segment */
@@ -77,8 +95,8 @@ struct SegArgs {
 #define DQ_BYTES_PER_WAVE (DQ_CAP * DQ_WORDS * 4)
 
 struct DeferQ {
-    float *b;     /* this wave's DQ_WORDS x DQ_CAP words of LDS (wave-uniform) */
-    uint32_t n;   /* parked neutrons, wave-uniform, < DQ_CAP */
+    float *b;     /* this wave's words x capacity of LDS (wave-uniform) */
+    uint32_t n;   /* parked neutrons, wave-uniform, below the capacity */
 };
 
 /* order this wave's LDS accesses: the compiler may not move loads of
@@ -89,35 +107,38 @@ DEV void wave_lds_sync()
     __builtin_amdgcn_wave_barrier();
 }
 
-DEV void defer_init(DeferQ &d, float *lds)
+DEV void defer_init(DeferQ &d, float *lds, uint32_t words_per_wave)
 {
     const uint32_t wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    d.b = lds + wid * (DQ_CAP * DQ_WORDS);
+    d.b = lds + wid * words_per_wave;
     d.n = 0;
 }
 
 /* the record of a parked neutron: the segment-queue record (x y z vx vy vz
- * w t ctr key status) and the slab in aux */
+ * w t ctr key status) and aux (the slab of a monochromator; for a mesh,
+ * bit 0: the intersection was all zeros, bits 1-2 the walk's outcome) */
+template <uint32_t CAP>
 DEV void defer_store(const NS &n, float *b, uint32_t i)
 {
-    b[0 * DQ_CAP + i] = n.x; b[1 * DQ_CAP + i] = n.y; b[2 * DQ_CAP + i] = n.z;
-    b[3 * DQ_CAP + i] = n.vx; b[4 * DQ_CAP + i] = n.vy; b[5 * DQ_CAP + i] = n.vz;
-    b[6 * DQ_CAP + i] = n.w; b[7 * DQ_CAP + i] = n.t; b[8 * DQ_CAP + i] = n.ctr;
+    b[0 * CAP + i] = n.x; b[1 * CAP + i] = n.y; b[2 * CAP + i] = n.z;
+    b[3 * CAP + i] = n.vx; b[4 * CAP + i] = n.vy; b[5 * CAP + i] = n.vz;
+    b[6 * CAP + i] = n.w; b[7 * CAP + i] = n.t; b[8 * CAP + i] = n.ctr;
     uint32_t *u = (uint32_t *)b;
-    u[9 * DQ_CAP + i] = n.key;
-    u[10 * DQ_CAP + i] = n.iidx & 0x00ffffffu;
-    u[11 * DQ_CAP + i] = n.aux;
+    u[9 * CAP + i] = n.key;
+    u[10 * CAP + i] = n.iidx & 0x00ffffffu;
+    u[11 * CAP + i] = n.aux;
 }
 
+template <uint32_t CAP>
 DEV void defer_load(NS &n, const float *b, uint32_t i)
 {
-    n.x = b[0 * DQ_CAP + i]; n.y = b[1 * DQ_CAP + i]; n.z = b[2 * DQ_CAP + i];
-    n.vx = b[3 * DQ_CAP + i]; n.vy = b[4 * DQ_CAP + i]; n.vz = b[5 * DQ_CAP + i];
-    n.w = b[6 * DQ_CAP + i]; n.t = b[7 * DQ_CAP + i]; n.ctr = b[8 * DQ_CAP + i];
+    n.x = b[0 * CAP + i]; n.y = b[1 * CAP + i]; n.z = b[2 * CAP + i];
+    n.vx = b[3 * CAP + i]; n.vy = b[4 * CAP + i]; n.vz = b[5 * CAP + i];
+    n.w = b[6 * CAP + i]; n.t = b[7 * CAP + i]; n.ctr = b[8 * CAP + i];
     const uint32_t *u = (const uint32_t *)b;
-    n.key = u[9 * DQ_CAP + i];
-    n.iidx = u[10 * DQ_CAP + i];
-    n.aux = u[11 * DQ_CAP + i];
+    n.key = u[9 * CAP + i];
+    n.iidx = u[10 * CAP + i];
+    n.aux = u[11 * CAP + i];
     n.dead = false;
     n.pend = true;
     isect_reset(n);
@@ -146,7 +167,20 @@ DEV void run_range(NS &n, const SegArgs &a, HistCtx &hc, int k0, int k1, DeferQ 
         if (st.kind == 0) {
             const DComp &c = plan_get<SP>(a.comps, st.first);
             tr_in<FULL>(n, c);
-            geom_dispatch<F>(n, c);
+            /* a mesh step at the start of a segment after a walk launch
+             * takes the walk's hits (at the neutron's input queue slot,
+             * held in aux) instead of walking (k_mesh_walk) */
+            bool walked = false;
+            if constexpr ((F & FEAT_MESH) != 0) {
+                if ((a.walk_role & WALK_USE) != 0u && k == a.s0 && !n.dead) {
+                    const float *wr = a.walk_res;
+                    const uint32_t code = ((const uint32_t *)wr)[2 * a.walk_cap + n.aux];
+                    if (code == 1u || code == 2u)
+                        mesh_commit(n, npos(n), nvel(n), wr[n.aux], wr[a.walk_cap + n.aux], code, c.comp_idx);
+                    walked = true;
+                }
+            }
+            if (!walked) geom_dispatch<F>(n, c);
             terminate(n, c.restore);
             bool traced = false, mcpl = false;
             if constexpr (FULL) {
@@ -171,7 +205,7 @@ DEV void run_range(NS &n, const SegArgs &a, HistCtx &hc, int k0, int k1, DeferQ 
                     if (m) {
                         if (park) {
                             const unsigned long long below = (1ull << __lane_id()) - 1ull;
-                            defer_store(n, dq.b, dq.n + (uint32_t)__popcll(m & below));
+                            defer_store<DQ_CAP>(n, dq.b, dq.n + (uint32_t)__popcll(m & below));
                             n.pend = true;
                             n.dead = true;
                         }
@@ -216,14 +250,16 @@ DEV void run_steps(NS &n, const SegArgs &a, HistCtx &hc, DeferQ &dq, bool defer)
     run_range<FULL, F>(n, a, hc, a.s0, a.s1, dq, defer);
 }
 
-/* Take up to 64 parked neutrons (lane i gets entry i); returns how many. */
+/* Take up to 64 parked neutrons (lane i gets entry i); returns how many.
+ * The entries beyond them move down, 64 per round. */
+template <uint32_t CAP, uint32_t WORDS>
 DEV uint32_t defer_pop(DeferQ &dq, NS &r)
 {
     const uint32_t lane = __lane_id();
     const uint32_t take = dq.n < 64u ? dq.n : 64u;
     wave_lds_sync();
     if (lane < take) {
-        defer_load(r, dq.b, lane);
+        defer_load<CAP>(r, dq.b, lane);
     } else {
         r.dead = true;
         r.pend = false;
@@ -231,12 +267,13 @@ DEV uint32_t defer_pop(DeferQ &dq, NS &r)
         r.iidx = 0;
     }
     wave_lds_sync();
-    /* the dq.n - take < 64 entries beyond them move down */
-    if (lane < dq.n - take) {
-        uint32_t *u = (uint32_t *)dq.b;
-        for (int f = 0; f < DQ_WORDS; f++) u[f * DQ_CAP + lane] = u[f * DQ_CAP + take + lane];
+    uint32_t *u = (uint32_t *)dq.b;
+    for (uint32_t j0 = 0; j0 < dq.n - take; j0 += 64u) {
+        const uint32_t j = j0 + lane;
+        if (j < dq.n - take)
+            for (uint32_t f = 0; f < WORDS; f++) u[f * CAP + j] = u[f * CAP + take + j];
+        wave_lds_sync();
     }
-    wave_lds_sync();
     dq.n -= take;
     return take;
 }
@@ -282,19 +319,49 @@ DEV uint32_t wave_sum(uint32_t v)
     return v;
 }
 
+/* Prepare a stored neutron (queue slot `slot`) for k_mesh_walk: bring it
+ * into the mesh's frame as the next segment's transform does (the general
+ * rotation, which equals its identity shortcut wherever that applies),
+ * run the reference's box precheck (mesh.cl:23-42), and append the rays
+ * that pass to the wave's walk-list region; the others get outcome 3. */
+DEV void walk_emit(const NS &n, bool live, const SegArgs &a, uint64_t beg, uint64_t slot, uint32_t &wl_count)
+{
+    const DComp &c = plan_at(a.comps, (size_t)a.walk_ci);
+    bool pass = false;
+    rf3 pos = rf3_make(0.0f, 0.0f, 0.0f), vel = pos;
+    if (live) {
+        pos = mat_derot(c.R, rf3_make(n.x - c.pos[0], n.y - c.pos[1], n.z - c.pos[2]));
+        vel = mat_derot(c.R, nvel(n));
+        pass = mesh_box_precheck(c.g, pos, vel);
+        if (!pass) ((uint32_t *)a.walk_res)[2 * a.walk_cap + slot] = 3u;
+    }
+    const unsigned long long m = __ballot(pass);
+    if (pass) {
+        const uint64_t e = beg + wl_count + (uint64_t)__popcll(m & ((1ull << __lane_id()) - 1ull));
+        float *L = a.walk_list;
+        const uint64_t C = a.walk_cap;
+        ((uint32_t *)L)[e] = (uint32_t)slot;
+        L[C + e] = pos.x; L[2 * C + e] = pos.y; L[3 * C + e] = pos.z;
+        L[4 * C + e] = vel.x; L[5 * C + e] = vel.y; L[6 * C + e] = vel.z;
+        L[7 * C + e] = 1.0f / vel.x; L[8 * C + e] = 1.0f / vel.y; L[9 * C + e] = 1.0f / vel.z;
+    }
+    wl_count += (uint32_t)__popcll(m);
+}
+
 /* end of a group of 64 in a segment: survivors to the wave's output
  * region, finished neutrons to finalize (parked ones are neither) */
-template <bool FULL>
-DEV void seg_tail(const NS &n, bool valid, const SegArgs &a, uint64_t beg, uint32_t &wcount)
+template <bool FULL, uint32_t F>
+DEV void seg_tail(const NS &n, bool valid, const SegArgs &a, uint64_t beg, uint32_t &wcount, uint32_t &wl_count)
 {
     const bool live = valid && !n.dead && !a.last;
     if (!a.last) {
         const unsigned long long m = __ballot(live);
-        if (live) {
-            const unsigned long long below = m & ((1ull << __lane_id()) - 1ull);
-            queue_store<FULL>(n, a.out, beg + wcount + (uint64_t)__popcll(below));
-        }
+        const uint64_t slot = beg + wcount + (uint64_t)__popcll(m & ((1ull << __lane_id()) - 1ull));
+        if (live) queue_store<FULL>(n, a.out, slot);
         wcount += (uint32_t)__popcll(m);
+        if constexpr (!FULL && (F & FEAT_MESH) != 0) {
+            if (a.walk_role & WALK_EMIT) walk_emit(n, live, a, beg, slot, wl_count);
+        }
     }
     if (valid && !live && !n.pend) finalize<FULL>(n, a);
 }
@@ -323,7 +390,7 @@ void k_segment(SegArgs a)
     hc.acc = nullptr;
     if (a.acc_bins) {
         /* the detectors' LDS copy follows the deferral queues */
-        const size_t defer_bytes = (!FULL && (F & FEAT_MONO) != 0) ? (blockDim.x >> 6) * DQ_BYTES_PER_WAVE : 0;
+        const size_t defer_bytes = FULL ? 0 : (size_t)(blockDim.x >> 6) * a.defer_lds;
         hist_acc_begin(hc, (char *)seg_lds + defer_bytes, a.acc_bytes);
     }
     const uint32_t lane = __lane_id();
@@ -341,14 +408,14 @@ void k_segment(SegArgs a)
     if (g == 0 && lane == 0 && !a.pilot) a.out_meta[0] = per;
     const uint64_t beg = g * per;
     const uint64_t end = beg + per < total ? beg + per : total;
-    uint32_t wcount = 0;
+    uint32_t wcount = 0, wl_count = 0;
     DeferQ dq;
     dq.b = seg_lds;
     dq.n = 0;
     bool defer = false;
     if constexpr (!FULL && (F & FEAT_MONO) != 0) {
         defer = !a.pilot && a.defer_k >= a.s0 && a.defer_k < a.s1;
-        if (defer) defer_init(dq, seg_lds);
+        if (defer) defer_init(dq, seg_lds, DQ_CAP * DQ_WORDS);
     }
     uint64_t rcur = 0; /* region of the group's first input (later segments) */
     for (uint64_t base = beg; base < end; base += 64) {
@@ -363,7 +430,9 @@ void k_segment(SegArgs a)
                 n.key = (uint32_t)(a.key_base + i);
                 src_generate<FULL, F>(n, plan_at(a.src, 0), a.num_sim, a.seed, sb_old);
             } else {
-                queue_load<FULL>(n, a.in, region_slot_group(a.in_prefix, nreg, stride, rcur, base, last, i));
+                const uint64_t slot = region_slot_group(a.in_prefix, nreg, stride, rcur, base, last, i);
+                queue_load<FULL>(n, a.in, slot);
+                if (a.walk_role & WALK_USE) n.aux = (uint32_t)slot;   /* where its walk result is */
             }
         } else {
             n.dead = true;
@@ -373,13 +442,13 @@ void k_segment(SegArgs a)
         }
         run_steps<FULL, F>(n, a, hc, dq, defer);
         if (a.pilot) continue;
-        seg_tail<FULL>(n, valid, a, beg, wcount);
+        seg_tail<FULL, F>(n, valid, a, beg, wcount, wl_count);
         if constexpr (!FULL && (F & FEAT_MONO) != 0) {
             if (defer && dq.n >= 64u) {
                 NS r;
-                const uint32_t take = defer_pop(dq, r);
+                const uint32_t take = defer_pop<DQ_CAP, DQ_WORDS>(dq, r);
                 defer_finish<F>(r, a, hc, a.s1, dq);
-                seg_tail<FULL>(r, lane < take, a, beg, wcount);
+                seg_tail<FULL, F>(r, lane < take, a, beg, wcount, wl_count);
             }
         }
     }
@@ -387,12 +456,15 @@ void k_segment(SegArgs a)
     if constexpr (!FULL && (F & FEAT_MONO) != 0) {
         while (defer && dq.n > 0u) {
             NS r;
-            const uint32_t take = defer_pop(dq, r);
+            const uint32_t take = defer_pop<DQ_CAP, DQ_WORDS>(dq, r);
             defer_finish<F>(r, a, hc, a.s1, dq);
-            seg_tail<FULL>(r, lane < take, a, beg, wcount);
+            seg_tail<FULL, F>(r, lane < take, a, beg, wcount, wl_count);
         }
     }
     if (!a.last && lane == 0) a.out_counts[g] = wcount;
+    if constexpr (!FULL && (F & FEAT_MESH) != 0) {
+        if ((a.walk_role & WALK_EMIT) && lane == 0) a.walk_counts[g] = wl_count;
+    }
     if (a.acc_bins) hist_acc_end(hc, a.n_dets, a.acc_bins);
     uint32_t ev = wave_sum(hc.events), oob = wave_sum(hc.oob), ovf = wave_sum(hc.overflow);
     if (lane == 0) {

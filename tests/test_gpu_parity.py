@@ -95,13 +95,13 @@ def test_full_size_mesh_bvh_equals_brute_force(tmp_path):
     _hist_close(gh, o["hist"], 1e-6)
 
 
-def _run_pair(path, kw, n, seed, **ex):
+def _run_pair(path, kw, n, seed, keep_state=True, **ex):
     import ramp_amd
     out = []
     for brute in (False, True):
         inst = ramp_amd.Instrument(path, hist_mode="deterministic", chunk=n, **kw)
-        inst.execute(n, keep_state=True, seeds=[seed], mesh_brute_force=brute, **ex)
-        st = inst.state(n)
+        inst.execute(n, keep_state=keep_state, seeds=[seed], mesh_brute_force=brute, **ex)
+        st = inst.state(n) if keep_state else None
         hists = [inst.hist64(c.name) for c in inst._comp_list if getattr(c.scat, "n_bins", 0)]
         inst.close()
         out.append((st, hists))
@@ -109,14 +109,20 @@ def _run_pair(path, kw, n, seed, **ex):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("builder", ["sah", "lbvh"])
 @pytest.mark.parametrize("case,n", [("imaging", 4_000_000), ("mesh_multi", 1_000_000)])
-def test_bvh_equals_gpu_brute_force_at_scale(case, n, tmp_path):
+def test_bvh_equals_gpu_brute_force_at_scale(case, n, builder, tmp_path, monkeypatch):
     # the reference's loop over every triangle (mesh.cl:52-112) runs on the
     # GPU with the same Moller-Trumbore arithmetic as the BVH traversal (both
     # match the oracle bit for bit above); at millions of rays the two must
-    # give identical final states and identical fixed-point histograms
+    # give identical final states and identical fixed-point histograms, with
+    # either tree (host SAH or GPU LBVH).  Full-state runs walk inside the
+    # segment; fast-mode runs walk in the separate k_mesh_walk launch (the
+    # brute-force loop there too), and their histograms must equal the
+    # full-state ones.
     from ramp_amd import meshes
     from ramp_amd.components import write_stl_binary
+    monkeypatch.setenv("RAMP_BVH_BUILDER", builder)
     if case == "imaging":
         mesh = tmp_path / "ico50.stl"
         write_stl_binary(str(mesh), meshes.icosphere(50, radius=0.015, sigma=0.001, seed=3))
@@ -124,12 +130,17 @@ def test_bvh_equals_gpu_brute_force_at_scale(case, n, tmp_path):
         kw = {"mesh": repr(str(mesh))}
     else:
         path, kw = CASES[case]
-    print(f"{case} BVH vs brute force N={n}", flush=True)
+    print(f"{case} {builder} BVH vs brute force N={n} (full state)", flush=True)
     (sb, hb), (sf, hf) = _run_pair(path, kw, n, 91)
     for a, b in zip(sb, sf):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
     for (w1, q1), (w2, q2) in zip(hb, hf):
         assert np.array_equal(w1, w2) and np.array_equal(q1, q2)
+    print(f"{case} {builder} BVH vs brute force N={n} (fast mode)", flush=True)
+    (_, fb), (_, ff) = _run_pair(path, kw, n, 91, keep_state=False)
+    for (w1, q1), (w2, q2), (w3, q3) in zip(fb, ff, hb):
+        assert np.array_equal(w1, w2) and np.array_equal(q1, q2)
+        assert np.array_equal(w1, w3) and np.array_equal(q1, q3)
 
 
 @pytest.mark.gpu
