@@ -62,8 +62,9 @@ def valu_roofline(kernel, lane_ops, launch_ms, basis, extra=None):
 
 
 PMC_PER_HISTORY = {
-    "let_like": {"valu": 75.78, "lanes": 0.647, "fetch": 0.0076, "write": 1.2598},
-    "imaging": {"valu": 454.00, "lanes": 0.175, "fetch": 59.98, "write": 63.93},
+    "let_like": {"kernel": "k_segment<false,true>", "valu": 75.78, "lanes": 0.647, "fetch": 0.0076,
+                 "write": 1.2598, "commit": "55b7f20"},
+    "imaging": None,
     "minimal": None,
 }
 
@@ -194,12 +195,15 @@ def main():
     for w in range(a.warmup):
         inst.execute(N, seeds=[(seed + w) % 4097])
     sync()
-    seg_ms, out0, ev, qin, qout = 0.0, 0, 0, 0, 0
+    seg_ms, walk_ms, sort_ms, rays, out0, ev, qin, qout = 0.0, 0.0, 0.0, 0, 0, 0, 0, 0
     t0 = time.perf_counter()
     for k in range(a.steps):
         inst.execute(N, seeds=[(seed + 100 + k) % 4097])
         st = inst.last_stats
         seg_ms += st["ms_source_segment"]
+        walk_ms += st["ms_mesh_walk"]
+        sort_ms += st["ms_walk_sort"]
+        rays += st["walk_rays"]
         out0 += st["state_out_first"]
         ev += st["n_events"]
         qin += st["state_in"]
@@ -209,6 +213,7 @@ def main():
     dt = t1 - t0
     K = max(a.steps, 1)
     seg_avg_ms = seg_ms / K
+    walk_avg_ms = walk_ms / K
     # algorithmic HBM bytes of the source segment per launch: the 44-byte
     # SoA record (x,y,z,vx,vy,vz,w,t,ctr,key,status) of every survivor it
     # appends to the queue (it reads no state: histories are generated)
@@ -220,14 +225,22 @@ def main():
             "traffic": None, "launch_ms": seg_avg_ms,
             "achieved": None, "peak": VALU_PEAK_TOPS, "unit": "TOP/s", "frac": None}
     pmc = None if a.prefix else PMC_PER_HISTORY.get(a.workload)
-    if pmc and seg_avg_ms > 0:
+    if pmc and pmc.get("kernel") == "k_mesh_walk":
+        # the dominant kernel is the separate mesh-walk launch
+        roof.update({"kernel": "k_mesh_walk", "launch_ms": walk_avg_ms,
+                     "walk_rays_per_step": rays / K, "ms_walk_sort": sort_ms / K,
+                     "source_segment_ms": seg_avg_ms})
+        launch_ms = walk_avg_ms
+    else:
+        launch_ms = seg_avg_ms
+    if pmc and launch_ms > 0:
         # lane-op slots issued per second by the launch (measured instruction
         # count per history x 64 x histories / measured launch time)
-        ach = pmc["valu"] * 64 * N / (seg_avg_ms * 1e-3) / 1e12
+        ach = pmc["valu"] * 64 * N / (launch_ms * 1e-3) / 1e12
         roof.update({"achieved": ach, "frac": ach / VALU_PEAK_TOPS,
                      "single_issue_peak": VALU_SINGLE_ISSUE_TOPS,
                      "valu_insts_per_history": pmc["valu"], "active_lane_share": pmc["lanes"],
-                     "traffic": (pmc["fetch"] + pmc["write"]) * N})
+                     "traffic": (pmc["fetch"] + pmc["write"]) * N, "pmc_commit": pmc["commit"]})
     out = {
         "metric": METRIC,
         "value": N * K / dt,

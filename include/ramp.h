@@ -177,6 +177,9 @@ typedef struct {
     double ms_rescal_append; /* device time of the first chunk's SRescal event append (count, scan, append) */
     double ms_plugin;        /* device time of the first chunk's first plugin step */
     uint64_t plugin_items;   /* neutrons that step processed (the queue it rewrote) */
+    double ms_mesh_walk;     /* device time of the first chunk's separate mesh-walk launch (k_mesh_walk) */
+    double ms_walk_sort;     /* device time of the first chunk's walk-list ordering (keys, sort, gather) */
+    uint64_t walk_rays;      /* rays that walk launch walked */
 } ramp_run_stats;
 
 typedef struct ramp_ctx ramp_ctx;

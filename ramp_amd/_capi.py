@@ -118,6 +118,9 @@ class RunStats(C.Structure):
         ("ms_rescal_append", C.c_double),
         ("ms_plugin", C.c_double),
         ("plugin_items", C.c_uint64),
+        ("ms_mesh_walk", C.c_double),
+        ("ms_walk_sort", C.c_double),
+        ("walk_rays", C.c_uint64),
     ]
 
 

@@ -181,17 +181,23 @@ __global__ void k_bvh_flatten(const BvhNode *nodes, const int *parent, int nn, i
     flat[pre] = f;
 }
 
+/* a leaf record of triangle t (vertices P), host and device */
+__host__ __device__ inline BvhTri bvh_tri_record(const float *P, uint32_t t)
+{
+    BvhTri r;
+    for (int i = 0; i < 9; i++) r.v[i] = P[i];
+    r.idx = t;
+    r.pad[0] = r.pad[1] = 0u;
+    return r;
+}
+
 /* the leaf-order triangle records: lt[k] holds triangle order[k] */
 __global__ void k_bvh_leaf_tris(const float *pts, const uint32_t *order, int n, BvhTri *lt)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const uint32_t t = order[k];
-    BvhTri r;
-    for (int i = 0; i < 9; i++) r.v[i] = pts[9 * (size_t)t + i];
-    r.idx = t;
-    r.pad[0] = r.pad[1] = 0u;
-    lt[k] = r;
+    lt[k] = bvh_tri_record(pts + 9 * (size_t)t, t);
 }
 
 #endif

@@ -22,16 +22,16 @@
 
 #include "bvh_types.h"
 
-/* One Moller-Trumbore test of the reference (mesh.cl:54-111).  Only the
+/* One Moller-Trumbore test of the reference (mesh.cl:54-111), from the
+ * triangle's first vertex v0, edges e1 = v1 - v0, e2 = v2 - v0 and normal
+ * nrm = e1 x e2.  Only the
  * distances of the nearest entering (t[0]) and exiting (t[1]) hits and
  * their triangles are kept: a hit point is pos + vel t, formed from them
  * when the intersection is written (mesh_commit). */
-DEV void mesh_tri(const float *P, rf3 pos, rf3 vel, float *tb, uint32_t *out_idx, uint32_t tri, uint32_t &nint)
+DEV void mesh_tri_core(rf3 v0, rf3 e1, rf3 e2, rf3 nrm, rf3 pos, rf3 vel, float *tb, uint32_t *out_idx,
+                       uint32_t tri, uint32_t &nint)
 {
     const float eps = 0.000000001f;
-    rf3 v0 = rf3_make(P[0], P[1], P[2]), v1 = rf3_make(P[3], P[4], P[5]);
-    rf3 v2 = rf3_make(P[6], P[7], P[8]);
-    rf3 e1 = v3_sub(v1, v0), e2 = v3_sub(v2, v0);
     rf3 h = v3_cross(vel, e2);
     float a = v3_dot(e1, h);
     if (rm_fabs(a) < eps) return;
@@ -44,7 +44,6 @@ DEV void mesh_tri(const float *P, rf3 pos, rf3 vel, float *tb, uint32_t *out_idx
     if (v < 0.0f || u + v > 1.0f) return;
     float t = f * v3_dot(e2, q);
     if (t > eps) nint++;
-    rf3 nrm = v3_cross(e1, e2);
     float nv = v3_dot(nrm, vel);
     /* strict '<' in index order: an equal t wins only with a lower index */
     if ((t > eps) && (nv < 0.0f) && (t < tb[0] || (t == tb[0] && tri < out_idx[0]))) {
@@ -55,6 +54,15 @@ DEV void mesh_tri(const float *P, rf3 pos, rf3 vel, float *tb, uint32_t *out_idx
         tb[1] = t;
         out_idx[1] = tri;
     }
+}
+
+/* the test from the triangle's three vertices P[0..8] */
+DEV void mesh_tri(const float *P, rf3 pos, rf3 vel, float *tb, uint32_t *out_idx, uint32_t tri, uint32_t &nint)
+{
+    rf3 v0 = rf3_make(P[0], P[1], P[2]), v1 = rf3_make(P[3], P[4], P[5]);
+    rf3 v2 = rf3_make(P[6], P[7], P[8]);
+    rf3 e1 = v3_sub(v1, v0), e2 = v3_sub(v2, v0);
+    mesh_tri_core(v0, e1, e2, v3_cross(e1, e2), pos, vel, tb, out_idx, tri, nint);
 }
 
 /* conservative slab test: may the ray meet the box at some t > 0 that

@@ -187,10 +187,13 @@ __global__ void k_rng(const uint32_t *tid, const float *ctr, float *out, uint64_
  * ray to ray, so walks run in lockstep inside a segment leave most lanes of
  * a wave idle.  For the plan's first linear GMesh step, the segment before
  * it prepares every output neutron (walk_emit: the mesh frame, the box
- * precheck, 1/v) and lists the rays that need a walk; k_mesh_walk walks
+ * precheck) and lists the rays that need a walk; k_mesh_walk walks
  * them with lane refill, and the segment starting at the step takes the
  * hits from the neutron's queue slot.  Every ray runs the same walk and
- * arithmetic as in g_mesh.
+ * arithmetic as in g_mesh.  The listed rays are walked in the order of
+ * their keys (walk_key: direction class, then position across the dominant
+ * axis), radix-sorted and gathered between the two launches, so the rays
+ * side by side in a wave visit nearly the same nodes and share their loads.
  *
  * A lane whose walk is over takes the next ray from its wave's pool of
  * claimed rays, which the wave refills from a device counter, one atomic
@@ -202,18 +205,50 @@ __global__ void k_rng(const uint32_t *tid, const float *ctr, float *out, uint64_
  * lanes active instead of once per step for the few lanes at a leaf.  Refills
  * likewise run once WALK_REFILL lanes are idle. */
 struct WalkArgs {
-    const float *list;         /* walk_list of the producing segment */
-    const uint64_t *prefix;    /* its region prefix [nreg + 1] */
-    const uint64_t *meta;      /* [0] region stride, [1] number of regions */
+    const WalkEntry *list;     /* the listed rays in key order, [n] */
+    uint64_t n;                /* rays listed */
     const DComp *comp;         /* the mesh component */
     float *res;                /* [3][cap]: t1, t2, outcome per queue slot */
     uint64_t cap;
     unsigned long long *next;  /* work counter, zero at launch */
 };
 
+/* the listed rays in key order (order: their list indices after the sort) */
+__global__ void k_walk_gather(const WalkEntry *list, const uint32_t *order, uint64_t n, WalkEntry *out)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        out[i] = list[order[i]];
+}
+
+/* The keys in dense order: one block per producing wave, item i = prefix[r]
+ * + k of region r is list entry e = r * stride + k. */
+__global__ void k_walk_order(const uint32_t *keys, const uint64_t *prefix, const uint64_t *meta, uint32_t *okeys,
+                             uint32_t *oidx)
+{
+    const uint64_t r = blockIdx.x, stride = meta[0];
+    if (r >= meta[1]) return;
+    const uint64_t b = prefix[r], cnt = prefix[r + 1] - b;
+    for (uint64_t k = threadIdx.x; k < cnt; k += blockDim.x) {
+        const uint64_t e = r * stride + k;
+        okeys[b + k] = keys[e];
+        oidx[b + k] = (uint32_t)e;
+    }
+}
+
+#define WALK_KEY_BITS 19     /* walk_key: 3 bits of direction class, 16 of position */
+
+#ifndef WALK_POOL
 #define WALK_POOL 1024u     /* rays a wave claims per atomic on the work counter */
-#define WALK_REFILL 16u     /* idle lanes that trigger a refill pass */
+#endif
+#ifndef WALK_REFILL
+#define WALK_REFILL 24u     /* idle lanes that trigger a refill pass */
+#endif
+#ifndef WALK_TRI_BATCH
 #define WALK_TRI_BATCH 24u  /* lanes with a parked leaf that trigger the tests */
+#endif
+#ifndef WALK_TRI_Q4
+#define WALK_TRI_Q4 3u      /* ... or this many quarters of the walking lanes */
+#endif
 #define IDLE 0xffffffffu    /* a lane without a ray (queue slots stay below 2^32 - 1) */
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_mesh_walk(WalkArgs a)
@@ -226,25 +261,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
     const uint32_t ntri = c.g.ni[GI_NTRI], interior = c.g.ni[GI_INTERIOR];
     const uint32_t lane = __lane_id();
     const unsigned long long below = (1ull << lane) - 1ull;
-    const uint64_t nreg = a.meta[1], stride = a.meta[0];
-    const uint64_t total = a.prefix[nreg];
+    const uint64_t total = a.n;
     const uint64_t C = a.cap;
     uint32_t *code = (uint32_t *)(a.res + 2 * C);
-    /* wave-uniform: the region cursor, the wave's pool of claimed rays
-     * [pool, pool_end) and whether the counter has passed the total */
-    uint64_t cur = 0, pool = 0, pool_end = 0;
+    /* wave-uniform: the wave's pool of claimed rays [pool, pool_end) and
+     * whether the counter has passed the total */
+    uint64_t pool = 0, pool_end = 0;
     bool drained = false;
     uint32_t my = IDLE;        /* this lane's queue slot */
     int32_t tri = -1;          /* its parked leaf (brute force: triangle); -1 none */
+    /* A refilled lane loads its entry (fresh 2), then starts the walk with
+     * its first node load (0): each load is used an iteration after it is
+     * issued, so the wave never waits on one. */
+    uint32_t fresh = 0;
     MeshWalk w;
     BvhFlat nd;                /* the lane's next node, loaded one step ahead */
     for (;;) {
-        unsigned long long im = __ballot(my == IDLE);
+        const unsigned long long im = __ballot(my == IDLE);
         if (!drained && im && ((uint32_t)__popcll(im) >= WALK_REFILL || im == __ballot(true))) {
             if (pool >= pool_end) {
                 unsigned long long base = 0;
                 if (lane == 0) base = atomicAdd(a.next, (unsigned long long)WALK_POOL);
-                base = __shfl(base, 0);
+                /* every lane runs here: lane 0 is the first active one, and
+                 * the claimed base lives in scalar registers */
+                base = ((unsigned long long)(uint32_t)__builtin_amdgcn_readfirstlane((int)(base >> 32)) << 32) |
+                       (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)base);
                 if (base >= total) {
                     drained = true;
                 } else {
@@ -257,20 +298,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
                 const uint32_t cnt = (uint32_t)__popcll(im);
                 const uint32_t take = (uint64_t)cnt < avail ? cnt : (uint32_t)avail;
                 const uint32_t rank = (uint32_t)__popcll(im & below);
-                cur = region_advance(a.prefix, nreg, cur, pool);
-                const uint64_t r1 = region_advance(a.prefix, nreg, cur, pool + take - 1);
                 if (((im >> lane) & 1ull) != 0ull && rank < take) {
-                    const uint64_t item = pool + rank;
-                    const uint64_t r = region_find(a.prefix, cur, r1 + 1, item);
-                    const uint64_t e = r * stride + (item - a.prefix[r]);
-                    const float *L = a.list;
-                    const rf3 pos = rf3_make(L[C + e], L[2 * C + e], L[3 * C + e]);
-                    const rf3 vel = rf3_make(L[4 * C + e], L[5 * C + e], L[6 * C + e]);
-                    const rf3 inv = rf3_make(L[7 * C + e], L[8 * C + e], L[9 * C + e]);
-                    mesh_walk_start(w, c.g, pos, vel, inv);
-                    if (!brute && w.cur < w.end) nd = nodes[w.cur];
-                    my = ((const uint32_t *)L)[e];
+                    const WalkEntry &we = a.list[pool + rank];
+                    w.pos = rf3_make(we.pos[0], we.pos[1], we.pos[2]);
+                    w.vel = rf3_make(we.vel[0], we.vel[1], we.vel[2]);
+                    my = we.slot;
                     tri = -1;
+                    fresh = 2u;
                 }
                 pool += take;
             }
@@ -282,7 +316,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
         }
         /* the parked triangles */
         const unsigned long long parked = __ballot(tri >= 0);
-        if ((uint32_t)__popcll(parked) >= WALK_TRI_BATCH || (parked && parked == walking)) {
+        const uint32_t npark = (uint32_t)__popcll(parked);
+        if (npark >= WALK_TRI_BATCH || (parked && 4u * npark >= WALK_TRI_Q4 * (uint32_t)__popcll(walking))) {
             if (tri >= 0) {
                 if (brute) mesh_walk_brute(w, pts, (uint32_t)tri);
                 else mesh_walk_leaf(w, lt, tri);
@@ -290,7 +325,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
             }
         }
         /* one box step of every walking lane without a parked triangle */
-        if (my != IDLE && tri < 0) {
+        if (fresh == 2u) {
+            mesh_walk_init(w, c.g, w.pos, w.vel);
+            if (!brute && w.cur < w.end) nd = nodes[w.cur];
+            fresh = 0u;
+        } else if (my != IDLE && tri < 0) {
             if (w.cur < w.end) {
                 if (!brute) {
                     tri = mesh_walk_node(w, nd, ntri, interior != 0);
@@ -316,7 +355,7 @@ struct ramp_ctx {
     int device;
     int n_cu;
     hipStream_t stream;
-    hipEvent_t ev[8];
+    hipEvent_t ev[12];
     std::string arch;   /* gcnArchName, for run-time compiled plugins */
     void *scratch = nullptr; /* this device's scratch array (ramp_run_noalloc) */
 };
@@ -422,9 +461,11 @@ struct ramp_plan {
     std::vector<int> bounds;  /* segment boundaries (step indices) */
     int defer_k = -1;         /* first linear monochromator step (reflections deferred) */
     int walk_k = -1;          /* first linear GMesh step: its walks run in k_mesh_walk (fast mode) */
-    /* the walks: hits [3][queue slots], the walk list [10][queue slots], its
-     * region counts, and [prefix (waves + 1), meta (2), work counter (1)] */
-    DevBuf walk_res, walk_list, walk_counts, walk_aux;
+    /* the walks: hits [3][queue slots], the walk list and its keys [queue
+     * slots], its region counts, [prefix (waves + 1), meta (2), work counter
+     * (1)], the keys and list indices in dense order and sorted, and the
+     * sort's scratch */
+    DevBuf walk_res, walk_list, walk_keys, walk_counts, walk_aux, walk_sort[4], walk_temp, walk_sorted;
     int revive_k = -1;        /* last SMCPLIn step: segment 0 runs at least through it */
     bool has_mcpl = false;    /* SMCPLIn/SMCPLOut present */
     bool need_full = false;   /* MCPL, SRescal or SAnalyser present: full-state execution */
@@ -515,7 +556,7 @@ extern "C" int ramp_ctx_create(int device, ramp_ctx **out)
     c->n_cu = prop.multiProcessorCount;
     c->arch = prop.gcnArchName;
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    for (int i = 0; i < 8; i++) HIPCHK(hipEventCreate(&c->ev[i]));
+    for (int i = 0; i < 12; i++) HIPCHK(hipEventCreate(&c->ev[i]));
     int rc = scratch_setup(device, &c->scratch);
     if (rc) {
         (void)ramp_ctx_destroy(c);
@@ -530,7 +571,7 @@ extern "C" int ramp_ctx_destroy(ramp_ctx *ctx)
     if (!ctx) return 0;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    for (int i = 0; i < 8; i++) (void)hipEventDestroy(ctx->ev[i]);
+    for (int i = 0; i < 12; i++) (void)hipEventDestroy(ctx->ev[i]);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return 0;
@@ -739,12 +780,7 @@ static int build_mesh_sah(ramp_plan *p, DPart &g, const float *hpts, uint32_t nt
     b.build(0, n, -1, nn, 0);
     /* the leaf-order triangle records */
     std::vector<BvhTri> lt(n);
-    for (int k = 0; k < n; k++) {
-        const uint32_t t = b.order[k];
-        for (int i = 0; i < 9; i++) lt[k].v[i] = hpts[9 * (size_t)t + i];
-        lt[k].idx = t;
-        lt[k].pad[0] = lt[k].pad[1] = 0u;
-    }
+    for (int k = 0; k < n; k++) lt[k] = bvh_tri_record(hpts + 9 * (size_t)b.order[k], b.order[k]);
     DevBuf nodes, parent, tris;
     int rc = dev_alloc(nodes, (uint64_t)nn * sizeof(BvhNode));
     if (!rc) rc = dev_alloc(parent, (uint64_t)nn * sizeof(int));
@@ -1091,7 +1127,9 @@ extern "C" int ramp_plan_destroy(ramp_plan *p)
     }
     DevBuf *single[] = {&p->comps, &p->steps, &p->src, &p->dets, &p->tables, &p->cnt, &p->count,
                         &p->slot_ctr, &p->slot_iidx, &p->st16, &p->st8, &p->stii,
-                        &p->rcounts, &p->prefix, &p->comps_brute, &p->dump, &p->dump_tag, &p->walk_res, &p->walk_list, &p->walk_counts, &p->walk_aux,
+                        &p->rcounts, &p->prefix, &p->comps_brute, &p->dump, &p->dump_tag, &p->walk_res, &p->walk_list, &p->walk_keys, &p->walk_counts, &p->walk_aux,
+                        &p->walk_sort[0], &p->walk_sort[1], &p->walk_sort[2], &p->walk_sort[3], &p->walk_temp,
+                        &p->walk_sorted,
                         &p->sel_counts, &p->sel_prefix, &p->sel_out, &p->ev, &p->ev_tag,
                         &p->ev_list, &p->ev_list_n, &p->trace};
     for (DevBuf *b : single) dev_free(*b);
@@ -1687,7 +1725,7 @@ static int run_plan(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uint32_
     unsigned walk_grid = 0;
     const uint64_t walk_cap = chunk + max_waves;   /* the queues' slots */
     const bool walk_ok = !full && !noalloc && p->walk_k >= 0 && p->plugin_step[p->walk_k] < 0 &&
-                         walk_cap < (1ull << 32);
+                         walk_cap < (1ull << 31);
     if (walk_ok && p->walk_k > 0 && nseg > 1) {
         for (int sg = 1; sg < nseg; sg++)
             if (p->bounds[sg] == p->walk_k) walk_seg = sg;
@@ -1698,11 +1736,20 @@ static int run_plan(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uint32_
         const int ps = walk_seg - 1;
         if (p->bounds[ps + 1] == p->bounds[ps] + 1 && p->plugin_step[p->bounds[ps]] >= 0) walk_seg = -1;
     }
+    size_t walk_tbytes = 0;
     if (walk_seg > 0) {
         rc = dev_alloc(p->walk_res, 3 * walk_cap * sizeof(float));
-        if (!rc) rc = dev_alloc(p->walk_list, 10 * walk_cap * sizeof(float));
+        if (!rc) rc = dev_alloc(p->walk_list, walk_cap * sizeof(WalkEntry));
+        if (!rc) rc = dev_alloc(p->walk_keys, walk_cap * sizeof(uint32_t));
         if (!rc) rc = dev_alloc(p->walk_counts, max_waves * sizeof(uint32_t));
         if (!rc) rc = dev_alloc(p->walk_aux, (max_waves + 4) * sizeof(uint64_t));
+        for (int i = 0; i < 4 && !rc; i++) rc = dev_alloc(p->walk_sort[i], walk_cap * sizeof(uint32_t));
+        if (rc) return rc;
+        HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, walk_tbytes, (const uint32_t *)p->walk_sort[0].p,
+                                                  (uint32_t *)p->walk_sort[1].p, (const uint32_t *)p->walk_sort[2].p,
+                                                  (uint32_t *)p->walk_sort[3].p, (int)walk_cap, 0, WALK_KEY_BITS, s));
+        rc = dev_alloc(p->walk_temp, walk_tbytes + 16);
+        if (!rc) rc = dev_alloc(p->walk_sorted, walk_cap * sizeof(WalkEntry));
         if (rc) return rc;
         int nb = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_mesh_walk, 256, 0) != hipSuccess || nb < 1) nb = 1;
@@ -1711,7 +1758,8 @@ static int run_plan(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uint32_
     a.walk_role = 0;
     a.walk_ci = walk_seg > 0 ? (uint32_t)p->hsteps[p->walk_k].first : 0u;
     a.walk_res = walk_seg > 0 ? (float *)p->walk_res.p : nullptr;
-    a.walk_list = walk_seg > 0 ? (float *)p->walk_list.p : nullptr;
+    a.walk_list = walk_seg > 0 ? (WalkEntry *)p->walk_list.p : nullptr;
+    a.walk_keys = walk_seg > 0 ? (uint32_t *)p->walk_keys.p : nullptr;
     a.walk_counts = walk_seg > 0 ? (uint32_t *)p->walk_counts.p : nullptr;
     a.walk_cap = walk_cap;
     uint64_t *meta = (uint64_t *)p->count.p; /* [0..1] producer, [2..3] consumer */
@@ -1722,7 +1770,8 @@ static int run_plan(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uint32_
     launch_fill_u64(s, p->cnt.p, CNT_N);
     HIPCHK(hipEventRecord(ctx->ev[0], s));
     uint64_t segs = 0;
-    bool plugin_timed = false;
+    bool plugin_timed = false, walk_timed = false;
+    uint64_t walk_rays = 0;
     uint64_t plugin_items = 0, plugin_nreg = 0;
     for (uint64_t ck = 0; ck < n_chunks; ck++) {
         uint64_t torun = std::min(chunk, n_hist - ck * chunk);
@@ -1791,17 +1840,46 @@ static int run_plan(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uint32_
                                        nreg, (const uint64_t *)meta, wp, wp + max_waves + 1,
                                        (unsigned long long *)nullptr, 0);
                     HIPCHK(hipGetLastError());
-                    launch_fill_u64(s, wp + max_waves + 3, 1);
-                    WalkArgs wa;
-                    wa.list = (const float *)p->walk_list.p;
-                    wa.prefix = wp;
-                    wa.meta = wp + max_waves + 1;
-                    wa.comp = a.comps + a.walk_ci;   /* the brute-force copy when flagged */
-                    wa.res = (float *)p->walk_res.p;
-                    wa.cap = walk_cap;
-                    wa.next = (unsigned long long *)(wp + max_waves + 3);
-                    hipLaunchKernelGGL(k_mesh_walk, dim3(walk_grid), dim3(256), 0, s, wa);
+                    if (ck == 0) HIPCHK(hipEventRecord(ctx->ev[10], s));
+                    hipLaunchKernelGGL(k_walk_order, dim3((unsigned)nreg), dim3(256), 0, s,
+                                       (const uint32_t *)p->walk_keys.p, (const uint64_t *)wp,
+                                       (const uint64_t *)(wp + max_waves + 1), (uint32_t *)p->walk_sort[0].p,
+                                       (uint32_t *)p->walk_sort[2].p);
                     HIPCHK(hipGetLastError());
+                    launch_fill_u64(s, wp + max_waves + 3, 1);
+                    /* the sort needs the number of listed rays on the host */
+                    uint64_t nwalk = 0;
+                    HIPCHK(hipMemcpyAsync(&nwalk, wp + nreg, sizeof nwalk, hipMemcpyDeviceToHost, s));
+                    HIPCHK(hipStreamSynchronize(s));
+                    if (nwalk > walk_cap) return fail(RAMP_E_DEVICE, "walk list of %llu rays", (unsigned long long)nwalk);
+                    if (nwalk > 0) {
+                        size_t tb = walk_tbytes;
+                        HIPCHK(hipcub::DeviceRadixSort::SortPairs(p->walk_temp.p, tb, (const uint32_t *)p->walk_sort[0].p,
+                                                                  (uint32_t *)p->walk_sort[1].p,
+                                                                  (const uint32_t *)p->walk_sort[2].p,
+                                                                  (uint32_t *)p->walk_sort[3].p, (int)nwalk, 0,
+                                                                  WALK_KEY_BITS, s));
+                        hipLaunchKernelGGL(k_walk_gather, dim3(4096), dim3(256), 0, s,
+                                           (const WalkEntry *)p->walk_list.p, (const uint32_t *)p->walk_sort[3].p,
+                                           nwalk, (WalkEntry *)p->walk_sorted.p);
+                        HIPCHK(hipGetLastError());
+                        if (ck == 0) HIPCHK(hipEventRecord(ctx->ev[11], s));
+                        WalkArgs wa;
+                        wa.list = (const WalkEntry *)p->walk_sorted.p;
+                        wa.n = nwalk;
+                        wa.comp = a.comps + a.walk_ci;   /* the brute-force copy when flagged */
+                        wa.res = (float *)p->walk_res.p;
+                        wa.cap = walk_cap;
+                        wa.next = (unsigned long long *)(wp + max_waves + 3);
+                        if (ck == 0) HIPCHK(hipEventRecord(ctx->ev[8], s));
+                        hipLaunchKernelGGL(k_mesh_walk, dim3(walk_grid), dim3(256), 0, s, wa);
+                        if (ck == 0) HIPCHK(hipEventRecord(ctx->ev[9], s));
+                        HIPCHK(hipGetLastError());
+                    }
+                    if (ck == 0) {
+                        walk_timed = nwalk > 0;
+                        walk_rays = nwalk;
+                    }
                 }
             }
         }
@@ -1836,6 +1914,14 @@ static int run_plan(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uint32_
     p->stats.ms_rescal_append = ms_ev;
     p->stats.ms_plugin = ms_pl;
     p->stats.plugin_items = plugin_items;
+    float ms_wk = 0, ms_ws = 0;
+    if (walk_timed) {
+        HIPCHK(hipEventElapsedTime(&ms_wk, ctx->ev[8], ctx->ev[9]));
+        HIPCHK(hipEventElapsedTime(&ms_ws, ctx->ev[10], ctx->ev[11]));
+    }
+    p->stats.ms_mesh_walk = ms_wk;
+    p->stats.ms_walk_sort = ms_ws;
+    p->stats.walk_rays = walk_rays;
     if (p->n_rescal)
         HIPCHK(hipMemcpy(p->ev_count.data(), p->ev_list_n.p, p->n_rescal * sizeof(uint64_t),
                          hipMemcpyDeviceToHost));

@@ -13,6 +13,15 @@
 #define WALK_EMIT 1u
 #define WALK_USE 2u
 
+/* a ray listed for k_mesh_walk: position and velocity in the mesh's frame,
+ * and its queue slot (32 bytes, two 16-byte loads) */
+struct WalkEntry {
+    float pos[3];
+    uint32_t slot;
+    float vel[3];
+    uint32_t pad;
+};
+
 struct SegArgs {
     const DComp *__restrict__ comps;
     const DStep *__restrict__ steps;
@@ -33,12 +42,13 @@ struct SegArgs {
      * a ray that misses the mesh's box gets outcome 3 at once).  WALK_USE:
      * the segment starts at that GMesh step and takes the hits instead of
      * walking.  walk_res [0][slot] t1, [1][slot] t2, [2][slot] outcome, per
-     * queue slot; walk_list [10][entry]: queue slot, pos, vel and 1/vel in
-     * the mesh's frame.  Zero-initialised arguments switch both off. */
+     * queue slot; walk_list: a WalkEntry per listed ray, walk_keys its
+     * ordering key.  Zero-initialised arguments switch both off. */
     uint32_t walk_role;
     uint32_t walk_ci;
     float *walk_res;
-    float *walk_list;
+    struct WalkEntry *walk_list;
+    uint32_t *walk_keys;
     uint32_t *walk_counts;
     uint64_t walk_cap;
     int num_sim;
@@ -319,11 +329,35 @@ DEV uint32_t wave_sum(uint32_t v)
     return v;
 }
 
+/* The ordering key of a ray for k_mesh_walk: its direction class (the
+ * BVH layout it walks) above an 8+8-bit Morton code of its position across
+ * the dominant axis, quantised over the mesh's box.  Rays walked in key
+ * order run through nearly the same nodes side by side in a wave; the key
+ * orders work only and does not touch any result. */
+DEV uint32_t walk_key(const DPart &g, rf3 pos, rf3 vel)
+{
+    const float ax = rm_fabs(vel.x), ay = rm_fabs(vel.y), az = rm_fabs(vel.z);
+    const int a = (ax >= ay && ax >= az) ? 0 : (ay >= az) ? 1 : 2;
+    const float neg = a == 0 ? vel.x : a == 1 ? vel.y : vel.z;
+    const int lay = 2 * a + (neg < 0.0f ? 1 : 0);
+    const int u = a == 0 ? 1 : 0, v = a == 2 ? 1 : 2;
+    const float pu = u == 0 ? pos.x : pos.y, pv = v == 1 ? pos.y : pos.z;
+    const float lu = g.p[GP_XMIN + 2 * u], hu = g.p[GP_XMAX + 2 * u];
+    const float lv = g.p[GP_XMIN + 2 * v], hv = g.p[GP_XMAX + 2 * v];
+    const float su = hu > lu ? 256.0f / (hu - lu) : 0.0f, sv = hv > lv ? 256.0f / (hv - lv) : 0.0f;
+    uint32_t qu = (uint32_t)rm_fmin(rm_fmax((pu - lu) * su, 0.0f), 255.0f);
+    uint32_t qv = (uint32_t)rm_fmin(rm_fmax((pv - lv) * sv, 0.0f), 255.0f);
+    uint32_t m = 0;
+    for (int b = 0; b < 8; b++) m |= (((qu >> b) & 1u) << (2 * b)) | (((qv >> b) & 1u) << (2 * b + 1));
+    return ((uint32_t)lay << 16) | m;
+}
+
 /* Prepare a stored neutron (queue slot `slot`) for k_mesh_walk: bring it
  * into the mesh's frame as the next segment's transform does (the general
  * rotation, which equals its identity shortcut wherever that applies),
- * run the reference's box precheck (mesh.cl:23-42), and append the rays
- * that pass to the wave's walk-list region; the others get outcome 3. */
+ * run the reference's box precheck (mesh.cl:23-42), and list the rays that
+ * pass in the wave's walk-list region, with their ordering keys; the
+ * others get outcome 3. */
 DEV void walk_emit(const NS &n, bool live, const SegArgs &a, uint64_t beg, uint64_t slot, uint32_t &wl_count)
 {
     const DComp &c = plan_at(a.comps, (size_t)a.walk_ci);
@@ -338,12 +372,13 @@ DEV void walk_emit(const NS &n, bool live, const SegArgs &a, uint64_t beg, uint6
     const unsigned long long m = __ballot(pass);
     if (pass) {
         const uint64_t e = beg + wl_count + (uint64_t)__popcll(m & ((1ull << __lane_id()) - 1ull));
-        float *L = a.walk_list;
-        const uint64_t C = a.walk_cap;
-        ((uint32_t *)L)[e] = (uint32_t)slot;
-        L[C + e] = pos.x; L[2 * C + e] = pos.y; L[3 * C + e] = pos.z;
-        L[4 * C + e] = vel.x; L[5 * C + e] = vel.y; L[6 * C + e] = vel.z;
-        L[7 * C + e] = 1.0f / vel.x; L[8 * C + e] = 1.0f / vel.y; L[9 * C + e] = 1.0f / vel.z;
+        WalkEntry w;
+        w.pos[0] = pos.x; w.pos[1] = pos.y; w.pos[2] = pos.z;
+        w.slot = (uint32_t)slot;
+        w.vel[0] = vel.x; w.vel[1] = vel.y; w.vel[2] = vel.z;
+        w.pad = 0u;
+        a.walk_list[e] = w;
+        a.walk_keys[e] = walk_key(c.g, pos, vel);
     }
     wl_count += (uint32_t)__popcll(m);
 }
