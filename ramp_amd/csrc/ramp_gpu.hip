@@ -208,8 +208,7 @@ struct WalkArgs {
     const WalkEntry *list;     /* the listed rays in key order, [n] */
     uint64_t n;                /* rays listed */
     const DComp *comp;         /* the mesh component */
-    float *res;                /* [3][cap]: t1, t2, outcome per queue slot */
-    uint64_t cap;
+    WalkHit *res;              /* the outcome per queue slot */
     unsigned long long *next;  /* work counter, zero at launch */
 };
 
@@ -262,8 +261,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
     const uint32_t lane = __lane_id();
     const unsigned long long below = (1ull << lane) - 1ull;
     const uint64_t total = a.n;
-    const uint64_t C = a.cap;
-    uint32_t *code = (uint32_t *)(a.res + 2 * C);
+
     /* wave-uniform: the wave's pool of claimed rays [pool, pool_end) and
      * whether the counter has passed the total */
     uint64_t pool = 0, pool_end = 0;
@@ -339,9 +337,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
                 }
             }
             if (tri < 0 && w.cur >= w.end) {
-                a.res[my] = w.t[0];
-                a.res[C + my] = w.t[1];
-                code[my] = mesh_walk_code(w, interior);
+                WalkHit h;
+                h.t1 = w.t[0];
+                h.t2 = w.t[1];
+                h.code = mesh_walk_code(w, interior);
+                h.pad = 0u;
+                a.res[my] = h;
                 my = IDLE;
             }
         }
@@ -1738,7 +1739,7 @@ static int run_plan(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uint32_
     }
     size_t walk_tbytes = 0;
     if (walk_seg > 0) {
-        rc = dev_alloc(p->walk_res, 3 * walk_cap * sizeof(float));
+        rc = dev_alloc(p->walk_res, walk_cap * sizeof(WalkHit));
         if (!rc) rc = dev_alloc(p->walk_list, walk_cap * sizeof(WalkEntry));
         if (!rc) rc = dev_alloc(p->walk_keys, walk_cap * sizeof(uint32_t));
         if (!rc) rc = dev_alloc(p->walk_counts, max_waves * sizeof(uint32_t));
@@ -1757,11 +1758,10 @@ static int run_plan(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uint32_
     }
     a.walk_role = 0;
     a.walk_ci = walk_seg > 0 ? (uint32_t)p->hsteps[p->walk_k].first : 0u;
-    a.walk_res = walk_seg > 0 ? (float *)p->walk_res.p : nullptr;
+    a.walk_res = walk_seg > 0 ? (WalkHit *)p->walk_res.p : nullptr;
     a.walk_list = walk_seg > 0 ? (WalkEntry *)p->walk_list.p : nullptr;
     a.walk_keys = walk_seg > 0 ? (uint32_t *)p->walk_keys.p : nullptr;
     a.walk_counts = walk_seg > 0 ? (uint32_t *)p->walk_counts.p : nullptr;
-    a.walk_cap = walk_cap;
     uint64_t *meta = (uint64_t *)p->count.p; /* [0..1] producer, [2..3] consumer */
     a.out_counts = bufs.rcounts;
     a.out_meta = meta;
@@ -1868,8 +1868,7 @@ static int run_plan(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uint32_
                         wa.list = (const WalkEntry *)p->walk_sorted.p;
                         wa.n = nwalk;
                         wa.comp = a.comps + a.walk_ci;   /* the brute-force copy when flagged */
-                        wa.res = (float *)p->walk_res.p;
-                        wa.cap = walk_cap;
+                        wa.res = (WalkHit *)p->walk_res.p;
                         wa.next = (unsigned long long *)(wp + max_waves + 3);
                         if (ck == 0) HIPCHK(hipEventRecord(ctx->ev[8], s));
                         hipLaunchKernelGGL(k_mesh_walk, dim3(walk_grid), dim3(256), 0, s, wa);

@@ -13,6 +13,15 @@
 #define WALK_EMIT 1u
 #define WALK_USE 2u
 
+/* a walk's outcome (mesh_walk_code, or 3: the ray missed the mesh's box)
+ * with its entering and exiting hit distances, one 16-byte record per
+ * queue slot */
+struct WalkHit {
+    float t1, t2;
+    uint32_t code;
+    uint32_t pad;
+};
+
 /* a ray listed for k_mesh_walk: position and velocity in the mesh's frame,
  * and its queue slot (32 bytes, two 16-byte loads) */
 struct WalkEntry {
@@ -41,16 +50,15 @@ struct SegArgs {
      * component walk_ci (its walk-list entries per wave in walk_counts;
      * a ray that misses the mesh's box gets outcome 3 at once).  WALK_USE:
      * the segment starts at that GMesh step and takes the hits instead of
-     * walking.  walk_res [0][slot] t1, [1][slot] t2, [2][slot] outcome, per
-     * queue slot; walk_list: a WalkEntry per listed ray, walk_keys its
-     * ordering key.  Zero-initialised arguments switch both off. */
+     * walking.  walk_res: a WalkHit per queue slot; walk_list: a
+     * WalkEntry per listed ray, walk_keys its ordering key.
+     * Zero-initialised arguments switch both off. */
     uint32_t walk_role;
     uint32_t walk_ci;
-    float *walk_res;
+    struct WalkHit *walk_res;
     struct WalkEntry *walk_list;
     uint32_t *walk_keys;
     uint32_t *walk_counts;
-    uint64_t walk_cap;
     int num_sim;
     int seed;
     uint64_t n;                 /* items of the first segment */
@@ -183,10 +191,8 @@ DEV void run_range(NS &n, const SegArgs &a, HistCtx &hc, int k0, int k1, DeferQ 
             bool walked = false;
             if constexpr ((F & FEAT_MESH) != 0) {
                 if ((a.walk_role & WALK_USE) != 0u && k == a.s0 && !n.dead) {
-                    const float *wr = a.walk_res;
-                    const uint32_t code = ((const uint32_t *)wr)[2 * a.walk_cap + n.aux];
-                    if (code == 1u || code == 2u)
-                        mesh_commit(n, npos(n), nvel(n), wr[n.aux], wr[a.walk_cap + n.aux], code, c.comp_idx);
+                    const WalkHit h = a.walk_res[n.aux];
+                    if (h.code == 1u || h.code == 2u) mesh_commit(n, npos(n), nvel(n), h.t1, h.t2, h.code, c.comp_idx);
                     walked = true;
                 }
             }
@@ -367,7 +373,13 @@ DEV void walk_emit(const NS &n, bool live, const SegArgs &a, uint64_t beg, uint6
         pos = mat_derot(c.R, rf3_make(n.x - c.pos[0], n.y - c.pos[1], n.z - c.pos[2]));
         vel = mat_derot(c.R, nvel(n));
         pass = mesh_box_precheck(c.g, pos, vel);
-        if (!pass) ((uint32_t *)a.walk_res)[2 * a.walk_cap + slot] = 3u;
+        if (!pass) {
+            WalkHit h;
+            h.t1 = h.t2 = 1e8f;
+            h.code = 3u;
+            h.pad = 0u;
+            a.walk_res[slot] = h;
+        }
     }
     const unsigned long long m = __ballot(pass);
     if (pass) {
