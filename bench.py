@@ -36,17 +36,18 @@ HBM_PEAK_GBS = 8000.0
 VALU_SINGLE_ISSUE_TOPS = 256 * 64 * 2.4e9 / 1e12
 VALU_PEAK_TOPS = 2 * VALU_SINGLE_ISSUE_TOPS
 
-# Per-history counts of the source-segment kernel (the dominant one), from
-# rocprofv3 --pmc runs of this script (profiles/let_like_1e8_segment_kernels.md,
-# profiles/imaging_mesh50k_1e8_bvh.md):
+# Per-history counts of the dominant kernel (the LET-like source segment; the
+# imaging workload's separate mesh-walk launch), from rocprofv3 --pmc runs of
+# this script at the commit named in each entry
+# (profiles/let_like_1e8_segment_kernels.md, profiles/imaging_mesh50k_1e8_bvh.md):
 #   valu  = SQ_INSTS_VALU / histories (wave64 instructions, so x64 lane-ops)
 #   lanes = SQ_THREAD_CYCLES_VALU / (64 SQ_INSTS_VALU), the active-lane share
 #   fetch = FETCH_SIZE / histories, write = WRITE_SIZE / histories (bytes,
 #           after the plain-copy calibration of the counters: x2.0, x1.0)
 # measured SQ_INSTS_VALU of the timed batch of the 256-setting scan, and per
 # history of the powder forward and adjoint kernels
-TAS_VALU_PER_SCAN = 1.43062e10
-POWDER_VALU_PER_HISTORY = {"forward": 82.36, "adjoint": 95.60}
+TAS_VALU_PER_SCAN = {"valu": 1.43062e10, "commit": "55b7f20"}
+POWDER_VALU_PER_HISTORY = {"forward": 82.36, "adjoint": 95.60, "commit": "55b7f20"}
 
 
 def valu_roofline(kernel, lane_ops, launch_ms, basis, extra=None):
@@ -62,9 +63,10 @@ def valu_roofline(kernel, lane_ops, launch_ms, basis, extra=None):
 
 
 PMC_PER_HISTORY = {
-    "let_like": {"kernel": "k_segment<false,true>", "valu": 75.78, "lanes": 0.647, "fetch": 0.0076,
-                 "write": 1.2598, "commit": "55b7f20"},
-    "imaging": None,
+    "let_like": {"kernel": "k_segment<false,true>", "valu": 76.01, "lanes": 0.643, "fetch": 0.0081,
+                 "write": 1.2562, "commit": "4fb921b"},
+    "imaging": {"kernel": "k_mesh_walk", "valu": 110.90, "lanes": 0.568, "fetch": 32.24, "write": 25.50,
+                "commit": "f4c8cd0"},
     "minimal": None,
 }
 
@@ -116,6 +118,26 @@ def instrument_prefix(path, kw, k, tmpdir):
     out = os.path.join(tmpdir, f"prefix{k}.json")
     with open(out, "w") as f:
         json.dump({bname: cut}, f)
+    return out, names
+
+
+def instrument_prefix_text(path, k, tmpdir):
+    """A batch instrument file (with per-setting $...$ expressions, so not
+    parsed here) cut as text to its first k component entries: the entries
+    are the keys indented by eight spaces whose value opens a brace."""
+    import re
+    lines = open(path).read().splitlines()
+    starts = [i for i, ln in enumerate(lines) if re.match(r'^ {8}"[^"]+" : \{', ln)]
+    names = [re.match(r'^ {8}"([^"]+)"', lines[i]).group(1) for i in starts][:k]
+    if k >= len(starts):
+        return path, names
+    keep = lines[:starts[k]]
+    while keep and not keep[-1].strip():
+        keep.pop()
+    keep[-1] = keep[-1].rstrip().rstrip(",")
+    out = os.path.join(tmpdir, f"prefix{k}.json")
+    with open(out, "w") as f:
+        f.write("\n".join(keep + ["    }", "}"]) + "\n")
     return out, names
 
 
@@ -226,8 +248,15 @@ def main():
             "achieved": None, "peak": VALU_PEAK_TOPS, "unit": "TOP/s", "frac": None}
     pmc = None if a.prefix else PMC_PER_HISTORY.get(a.workload)
     if pmc and pmc.get("kernel") == "k_mesh_walk":
-        # the dominant kernel is the separate mesh-walk launch
+        # the dominant kernel is the separate mesh-walk launch; its
+        # algorithmic HBM bytes: the 32-byte ray record read and the 16-byte
+        # hit record written per walked ray (the BVH and triangles stay in
+        # cache)
+        walk_bytes = 48.0 * rays / K
+        walk_gbs = walk_bytes / (walk_avg_ms * 1e-3) / 1e9 if walk_avg_ms > 0 else 0.0
         roof.update({"kernel": "k_mesh_walk", "launch_ms": walk_avg_ms,
+                     "hbm": {"achieved": walk_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": walk_gbs / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": walk_bytes},
                      "walk_rays_per_step": rays / K, "ms_walk_sort": sort_ms / K,
                      "source_segment_ms": seg_avg_ms})
         launch_ms = walk_avg_ms
@@ -306,6 +335,11 @@ def bench_tas_scan(a, wl, sync):
         counts = np.maximum(counts * (a.histories / counts.sum()), 1).astype(np.uint64)
     total = int(counts.sum())
     path = os.path.join(ROOT, "ramp_amd", "instruments", wl["file"])
+    prefix_names = None
+    if a.prefix:
+        import tempfile
+        path, prefix_names = instrument_prefix_text(path, a.prefix, tempfile.mkdtemp(prefix="ramp_bench_"))
+        a.no_cpu = True
     b = ramp_amd.BatchInstrument(path, settings, hist_mode=a.hist_mode)
     t = time.perf_counter()
     best, tuning = b.tune(counts, scale=1.0 / 16)
@@ -320,16 +354,19 @@ def bench_tas_scan(a, wl, sync):
         return np.array(r)
 
     # one scan each way with the same seeds (also warms both paths): the
-    # histories are the same, so the statistical errors must match
-    b.reset()
-    b.execute(counts, seeds=seeds)
-    err_batch = rel_err()
-    b.reset()
-    b.execute_sequential(counts, seeds=seeds)
-    err_seq = rel_err()
-    b.reset()
-    ok = np.isfinite(err_seq)
-    err_dev = float(np.max(np.abs(err_batch[ok] / err_seq[ok] - 1.0))) if ok.any() else None
+    # histories are the same, so the statistical errors must match (an
+    # instrument prefix, timed for the cost of its components, has no tube)
+    ok, err_dev = np.zeros(len(settings), dtype=bool), None
+    if not a.prefix:
+        b.reset()
+        b.execute(counts, seeds=seeds)
+        err_batch = rel_err()
+        b.reset()
+        b.execute_sequential(counts, seeds=seeds)
+        err_seq = rel_err()
+        b.reset()
+        ok = np.isfinite(err_seq)
+        err_dev = float(np.max(np.abs(err_batch[ok] / err_seq[ok] - 1.0))) if ok.any() else None
     for _ in range(a.warmup):
         b.execute(counts, seeds=seeds)
     sync()
@@ -341,6 +378,15 @@ def bench_tas_scan(a, wl, sync):
     dt = time.perf_counter() - t0
     K = max(a.steps, 1)
     sync()
+    if a.prefix:
+        print(json.dumps({"metric": METRIC, "value": total * K / dt, "unit": "histories/s", "n_gpus": 1,
+                          "steps": K, "warmup": a.warmup, "ms_per_step": dt / K * 1e3,
+                          "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+                          "data": "synthetic", "config": {"workload": wl["name"], "prefix_components": prefix_names,
+                                                          "histories_per_step": total,
+                                                          "device_ms_per_step": dev_ms / K}}), flush=True)
+        b.close()
+        return
     t0 = time.perf_counter()
     b.execute_sequential(counts, seeds=seeds)
     sync()
@@ -368,10 +414,13 @@ def bench_tas_scan(a, wl, sync):
         "sequential": {"histories_per_s": total / dt_seq, "ms": dt_seq * 1e3,
                        "device_ms_sum": seq_ms_dev,
                        "batch_speedup": (total * K / dt) / (total / dt_seq),
+                       "batch_speedup_device": seq_ms_dev / (dev_ms / K) if dev_ms > 0 else None,
                        "settings_with_counts": int(ok.sum()),
                        "max_rel_error_mismatch": err_dev},
-        "roofline": valu_roofline("k_batch<MONO> (the whole scan in one launch)", TAS_VALU_PER_SCAN * 64,
-                                  dev_ms / K, "wave64 VALU instructions of one scan, measured (SQ_INSTS_VALU)"),
+        "roofline": valu_roofline("k_batch<MONO> (the whole scan in one launch)",
+                                  TAS_VALU_PER_SCAN["valu"] * 64, dev_ms / K,
+                                  "wave64 VALU instructions of one scan, measured (SQ_INSTS_VALU) at commit "
+                                  + TAS_VALU_PER_SCAN["commit"]),
     }
     b.close()
     if not a.no_cpu:
@@ -431,13 +480,14 @@ def bench_powder_adjoint(a, wl, sync):
         h, _ = di.histogram(th)
         di.adjoint(th, weighted_mse(h, t, t2, eps)[1])
     sync()
-    fwd_ms = adj_ms = 0.0
+    fwd_ms = fwd_seg_ms = adj_ms = 0.0
     t0 = time.perf_counter()
     for _ in range(a.steps):
         h, _ = di.histogram(th)
         L, dldh = weighted_mse(h, t, t2, eps)
         g = di.adjoint(th, dldh)
         fwd_ms += di.last_forward_ms
+        fwd_seg_ms += di.last_forward_seg_ms
         adj_ms += di.last_adjoint_ms
     sync()
     dt = time.perf_counter() - t0
@@ -463,15 +513,17 @@ def bench_powder_adjoint(a, wl, sync):
         "config": {"workload": wl["name"], "histories_per_step": N, "parameters": di.n_params,
                    "instrument": "ramp_amd/instruments/powder_diff.json", "detector_bins": wl["bins"],
                    "loss": float(L), "grad_norm": float(np.linalg.norm(g)),
-                   "device_ms": {"forward": fwd_ms / K, "adjoint": adj_ms / K}},
+                   "device_ms": {"forward": fwd_ms / K, "forward_source_segment": fwd_seg_ms / K,
+                                 "adjoint": adj_ms / K}},
         "forward_only": {"histories_per_s": N * K / dt_fwd, "ms_per_step": dt_fwd / K * 1e3,
                          "fwd_adj_over_fwd": dt / dt_fwd},
         "finite_differences": {"forwards": 2 * di.n_params,
                                "ms": 2 * di.n_params * dt_fwd / K * 1e3,
                                "adjoint_speedup": (2 * di.n_params * dt_fwd / K) / (dt / K)},
         "roofline": valu_roofline("k_segment<false,true,2> (forward source segment)",
-                                  POWDER_VALU_PER_HISTORY["forward"] * 64 * N, fwd_ms / K,
-                                  "wave64 VALU instructions per history, measured (SQ_INSTS_VALU)",
+                                  POWDER_VALU_PER_HISTORY["forward"] * 64 * N, fwd_seg_ms / K,
+                                  "wave64 VALU instructions per history, measured (SQ_INSTS_VALU) at "
+                                  "commit " + POWDER_VALU_PER_HISTORY["commit"],
                                   extra={"adjoint": {"kernel": "k_adjoint<2>", "launch_ms": adj_ms / K,
                                                      "achieved": POWDER_VALU_PER_HISTORY["adjoint"] * 64 * N
                                                      / (adj_ms / K * 1e-3) / 1e12}}),

@@ -59,6 +59,7 @@ class DiffInstrument:
         base.close()
         self._cache = None
         self.last_forward_ms = self.last_adjoint_ms = None
+        self.last_forward_seg_ms = None   # the forward's source segment alone
 
     @property
     def n_params(self):
@@ -108,6 +109,7 @@ class DiffInstrument:
         inst.reset()
         inst.execute(self.n_hist, seeds=self.seeds)
         self.last_forward_ms = inst.last_stats["ms_total"]
+        self.last_forward_seg_ms = inst.last_stats["ms_source_segment"]
         return inst.hist64(self.detector)
 
     def adjoint(self, theta, dL_dh):
