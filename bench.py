@@ -46,8 +46,8 @@ VALU_PEAK_TOPS = 2 * VALU_SINGLE_ISSUE_TOPS
 #           after the plain-copy calibration of the counters: x2.0, x1.0)
 # measured SQ_INSTS_VALU of the timed batch of the 256-setting scan, and per
 # history of the powder forward and adjoint kernels
-TAS_VALU_PER_SCAN = {"valu": 1.43062e10, "commit": "55b7f20"}
-POWDER_VALU_PER_HISTORY = {"forward": 82.36, "adjoint": 95.60, "commit": "55b7f20"}
+TAS_VALU_PER_SCAN = {"valu": 1.4339e10, "commit": "706fd75"}
+POWDER_VALU_PER_HISTORY = {"forward": 82.42, "adjoint": 95.67, "commit": "706fd75"}
 
 
 def valu_roofline(kernel, lane_ops, launch_ms, basis, extra=None):
