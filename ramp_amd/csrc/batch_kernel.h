@@ -32,7 +32,7 @@ struct BatchArgs {
 #define BATCH_SP(F) (((F) & FEAT_MONO) == 0)
 
 template <uint32_t F>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SEG_MIN_WAVES(false, F))))
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(((F) & FEAT_MONO) != 0u ? RAMP_W_BATCH_MONO : 1)))
 void k_batch(BatchArgs b)
 {
     extern __shared__ float batch_lds[];

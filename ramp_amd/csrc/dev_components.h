@@ -736,26 +736,29 @@ DEV void s_guide(NS &n, const DComp &c)
         i = 0;
         /* the reference tests walls 1..4 in order, each when vdotn < 0
          * (guide.cl:67-86).  Of a pair of opposite walls at most one faces
-         * the neutron unless the guide widens, so a pair usually needs one
-         * division, not two; the order of the tests is unchanged. */
-        if (vdotn_v1 < 0.0f && vdotn_v2 < 0.0f) {
-            if ((t2 = (cv1 - cv2) / vdotn_v1) < t1) { t1 = t2; i = 1; }
-            if ((t2 = (cv1 + cv2) / vdotn_v2) < t1) { t1 = t2; i = 2; }
-        } else if (vdotn_v1 < 0.0f || vdotn_v2 < 0.0f) {
-            const bool w1 = vdotn_v1 < 0.0f;
-            if ((t2 = (w1 ? cv1 - cv2 : cv1 + cv2) / (w1 ? vdotn_v1 : vdotn_v2)) < t1) {
-                t1 = t2;
-                i = w1 ? 1 : 2;
+         * the neutron unless the guide narrows, so a pair tests its first
+         * facing wall, and a second division runs only for the neutrons
+         * facing both walls; the order of the tests is unchanged, and a wave
+         * holding both kinds of neutrons runs two divisions per pair, not
+         * three. */
+        {
+            const bool f1 = vdotn_v1 < 0.0f, f2 = vdotn_v2 < 0.0f;
+            if (f1 || f2) {
+                if ((t2 = (f1 ? cv1 - cv2 : cv1 + cv2) / (f1 ? vdotn_v1 : vdotn_v2)) < t1) {
+                    t1 = t2;
+                    i = f1 ? 1 : 2;
+                }
+                if (f1 && f2 && (t2 = (cv1 + cv2) / vdotn_v2) < t1) { t1 = t2; i = 2; }
             }
         }
-        if (vdotn_h1 < 0.0f && vdotn_h2 < 0.0f) {
-            if ((t2 = (ch1 - ch2) / vdotn_h1) < t1) { t1 = t2; i = 3; }
-            if ((t2 = (ch1 + ch2) / vdotn_h2) < t1) { t1 = t2; i = 4; }
-        } else if (vdotn_h1 < 0.0f || vdotn_h2 < 0.0f) {
-            const bool h1 = vdotn_h1 < 0.0f;
-            if ((t2 = (h1 ? ch1 - ch2 : ch1 + ch2) / (h1 ? vdotn_h1 : vdotn_h2)) < t1) {
-                t1 = t2;
-                i = h1 ? 3 : 4;
+        {
+            const bool f3 = vdotn_h1 < 0.0f, f4 = vdotn_h2 < 0.0f;
+            if (f3 || f4) {
+                if ((t2 = (f3 ? ch1 - ch2 : ch1 + ch2) / (f3 ? vdotn_h1 : vdotn_h2)) < t1) {
+                    t1 = t2;
+                    i = f3 ? 3 : 4;
+                }
+                if (f3 && f4 && (t2 = (ch1 + ch2) / vdotn_h2) < t1) { t1 = t2; i = 4; }
             }
         }
         if (i == 0) break;

@@ -419,10 +419,41 @@ DEV void seg_tail(const NS &n, bool valid, const SegArgs &a, uint64_t beg, uint3
  * region counts are prefix-summed by k_scan before the next segment.  A
  * segment holding the deferred monochromator step drains the wave's
  * deferral queue whenever it holds 64, and at the end. */
-/* the fast-mode monochromator instances and the plain full-state ones sit
- * a few registers above the limit of three waves per SIMD; ask the
- * compiler to fit them there (the other full-state instances would spill) */
-#define SEG_MIN_WAVES(FULL, F) ((((FULL) && (F) == 0u) || (!(FULL) && ((F) & FEAT_MONO) != 0)) ? 3 : 1)
+/* Waves per SIMD each instance is compiled for (amdgpu_waves_per_eu; 1
+ * leaves the register count to the compiler).  The fast-mode kernels are
+ * latency-bound at the occupancy their register use gives by itself (119 to
+ * 168 VGPRs: three or four waves), so they are held to more waves, trading
+ * a few spills outside the inner loops for latency hiding; each value is
+ * the best of the measured sweep on its bench workload.  The plain
+ * full-state instances sit a few registers above three waves and fit there;
+ * the other full-state ones would spill. */
+#ifndef RAMP_W_SEG_F0
+#define RAMP_W_SEG_F0 6      /* LET-like, minimal */
+#endif
+#ifndef RAMP_W_SEG_SAMPLE
+#define RAMP_W_SEG_SAMPLE 4  /* powder forward */
+#endif
+#ifndef RAMP_W_SEG_MONO
+#define RAMP_W_SEG_MONO 3    /* TAS per-setting runs */
+#endif
+#ifndef RAMP_W_SEG_MESH
+#define RAMP_W_SEG_MESH 1    /* imaging */
+#endif
+#ifndef RAMP_W_BATCH_MONO
+#define RAMP_W_BATCH_MONO 3  /* TAS batch */
+#endif
+#ifndef RAMP_W_ADJ
+#define RAMP_W_ADJ 4         /* powder adjoint */
+#endif
+constexpr int seg_min_waves(bool full, uint32_t f)
+{
+    return full ? (f == 0u ? 3 : 1)
+                : (f & FEAT_MONO) != 0u ? RAMP_W_SEG_MONO
+                : f == 0u ? RAMP_W_SEG_F0
+                : f == FEAT_SAMPLE ? RAMP_W_SEG_SAMPLE
+                : f == FEAT_MESH ? RAMP_W_SEG_MESH : 1;
+}
+#define SEG_MIN_WAVES(FULL, F) seg_min_waves((FULL), (F))
 
 template <bool FULL, bool FIRST, uint32_t F>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SEG_MIN_WAVES(FULL, F))))
