@@ -87,7 +87,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RAMP_W_ADJ)
                 terminate(n, c.restore);
                 scat_dispatch<false, F>(n, c, hc);
                 tr_out<false>(n, c);
-            } else {
+            } else if constexpr ((F & FEAT_NLIN) != 0u) {
                 for (uint32_t ev = 0; ev < st.multi; ev++) {
                     for (uint32_t j = 0; j < st.count; j++) {
                         const DComp &c = plan_at(a.comps, st.first + j);

@@ -68,7 +68,10 @@ DEV const T *in_const(const T *p)
 #define FEAT_SAMPLE 2u  /* SIsotropic, SPowder1, SPowderN */
 #define FEAT_MONO 4u    /* SMonochromator */
 #define FEAT_ISIS 8u    /* MISIS */
-#define FEAT_ALL 15u
+#define FEAT_NLIN 16u   /* non-linear blocks: their two-sweep step (mcramp.py:167-217)
+                         * holds a second copy of every component's code, and
+                         * only the full instance carries it */
+#define FEAT_ALL 31u
 #define DEVNI __device__ static __attribute__((noinline))
 
 struct NS {

@@ -1046,6 +1046,8 @@ extern "C" int ramp_plan_create(ramp_ctx *ctx, const ramp_plan_desc *d, ramp_pla
             feat |= FEAT_SAMPLE;
         if (sc == RAMP_S_MONO) feat |= FEAT_MONO;
     }
+    for (uint32_t b = 0; b < d->n_blocks; b++)
+        if (d->blocks[b].count != 0 && !d->blocks[b].linear) feat |= FEAT_NLIN;
     p->feat = feat;
     for (int v = 0; v < kNumVariants; v++)
         if ((feat & ~kVariants[v]) == 0) { p->variant = v; break; }

@@ -237,7 +237,7 @@ DEV void run_range(NS &n, const SegArgs &a, HistCtx &hc, int k0, int k1, DeferQ 
             if constexpr (FULL) {
                 if (traced) trace_alive(n, a.trace, a.trace_cap, a.trace_base + n.key, c.trace);
             }
-        } else {
+        } else if constexpr ((F & FEAT_NLIN) != 0u) {
             for (uint32_t ev = 0; ev < st.multi; ev++) {
                 for (uint32_t j = 0; j < st.count; j++) {
                     const DComp &c = plan_get<SP>(a.comps, st.first + j);
@@ -428,35 +428,38 @@ DEV void seg_tail(const NS &n, bool valid, const SegArgs &a, uint64_t beg, uint3
  * full-state instances sit a few registers above three waves and fit there;
  * the other full-state ones would spill. */
 #ifndef RAMP_W_SEG_F0
-#define RAMP_W_SEG_F0 6      /* LET-like, minimal */
+#define RAMP_W_SEG_F0 7      /* LET-like, minimal: later segments */
+#endif
+#ifndef RAMP_W_SEG_F0_FIRST
+#define RAMP_W_SEG_F0_FIRST 7  /* LET-like, minimal: source segment */
 #endif
 #ifndef RAMP_W_SEG_SAMPLE
-#define RAMP_W_SEG_SAMPLE 4  /* powder forward */
+#define RAMP_W_SEG_SAMPLE 6  /* powder forward */
 #endif
 #ifndef RAMP_W_SEG_MONO
-#define RAMP_W_SEG_MONO 3    /* TAS per-setting runs */
+#define RAMP_W_SEG_MONO 4    /* TAS per-setting runs */
 #endif
 #ifndef RAMP_W_SEG_MESH
 #define RAMP_W_SEG_MESH 1    /* imaging */
 #endif
 #ifndef RAMP_W_BATCH_MONO
-#define RAMP_W_BATCH_MONO 3  /* TAS batch */
+#define RAMP_W_BATCH_MONO 4  /* TAS batch */
 #endif
 #ifndef RAMP_W_ADJ
-#define RAMP_W_ADJ 4         /* powder adjoint */
+#define RAMP_W_ADJ 6         /* powder adjoint */
 #endif
-constexpr int seg_min_waves(bool full, uint32_t f)
+constexpr int seg_min_waves(bool full, bool first, uint32_t f)
 {
     return full ? (f == 0u ? 3 : 1)
                 : (f & FEAT_MONO) != 0u ? RAMP_W_SEG_MONO
-                : f == 0u ? RAMP_W_SEG_F0
+                : f == 0u ? (first ? RAMP_W_SEG_F0_FIRST : RAMP_W_SEG_F0)
                 : f == FEAT_SAMPLE ? RAMP_W_SEG_SAMPLE
                 : f == FEAT_MESH ? RAMP_W_SEG_MESH : 1;
 }
-#define SEG_MIN_WAVES(FULL, F) seg_min_waves((FULL), (F))
+#define SEG_MIN_WAVES(FULL, FIRST, F) seg_min_waves((FULL), (FIRST), (F))
 
 template <bool FULL, bool FIRST, uint32_t F>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SEG_MIN_WAVES(FULL, F))))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SEG_MIN_WAVES(FULL, FIRST, F))))
 void k_segment(SegArgs a)
 {
     extern __shared__ float seg_lds[];
