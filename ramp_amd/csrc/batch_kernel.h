@@ -27,10 +27,6 @@ struct BatchArgs {
  * deferral queue (monochromator reflections, see run_range) is drained
  * whenever it holds 64 and at the end of each tile, so it only ever holds
  * neutrons of one setting. */
-/* the batched monochromator instance reads the plan with vector loads:
- * with scalar ones it needs scratch at three waves per SIMD */
-#define BATCH_SP(F) (((F) & FEAT_MONO) == 0)
-
 template <uint32_t F>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(((F) & FEAT_MONO) != 0u ? RAMP_W_BATCH_MONO : 1)))
 void k_batch(BatchArgs b)
@@ -65,19 +61,19 @@ void k_batch(BatchArgs b)
             if (j < tl.len) {
                 n.iidx = 0u;
                 n.key = tl.beg + j;
-                src_generate<false, F>(n, plan_get<BATCH_SP(F)>(a.src, 0), a.num_sim, a.seed, 0.0f);
+                src_generate<false, F>(n, plan_at(a.src, 0), a.num_sim, a.seed, 0.0f);
             } else {
                 n.dead = true;
                 n.pend = false;
                 n.key = 0;
                 n.iidx = 0;
             }
-            run_range<false, F, BATCH_SP(F)>(n, a, hc, a.s0, a.s1, dq, defer);
+            run_range<false, F>(n, a, hc, a.s0, a.s1, dq, defer);
             if constexpr ((F & FEAT_MONO) != 0) {
                 if (defer && dq.n >= 64u) {
                     NS r;
                     defer_pop<DQ_CAP, DQ_WORDS>(dq, r);
-                    defer_finish<F, BATCH_SP(F)>(r, a, hc, a.s1, dq);
+                    defer_finish<F>(r, a, hc, a.s1, dq);
                 }
             }
         }
@@ -85,7 +81,7 @@ void k_batch(BatchArgs b)
             while (defer && dq.n > 0u) {
                 NS r;
                 defer_pop<DQ_CAP, DQ_WORDS>(dq, r);
-                defer_finish<F, BATCH_SP(F)>(r, a, hc, a.s1, dq);
+                defer_finish<F>(r, a, hc, a.s1, dq);
             }
         }
     }

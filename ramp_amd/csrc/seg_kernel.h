@@ -165,25 +165,15 @@ DEV void defer_load(NS &n, const float *b, uint32_t i)
 
 /* Steps [k0, k1); with defer, the step defer_k parks reflecting neutrons
  * in dq (they leave the range dead with pend set). */
-/* the plan through scalar loads (plan_at), or through vector loads where
- * the SGPRs that takes would push a kernel into scratch (the batched
- * monochromator kernel) */
-template <bool SP, class T>
-DEV const T &plan_get(const T *p, size_t i)
-{
-    if constexpr (SP) return plan_at(p, i);
-    else return p[i];
-}
-
-template <bool FULL, uint32_t F, bool SP = true>
+template <bool FULL, uint32_t F>
 DEV void run_range(NS &n, const SegArgs &a, HistCtx &hc, int k0, int k1, DeferQ &dq, bool defer)
 {
     for (int k = k0; k < k1; k++) {
         /* SMCPLIn revives dead slots; its plans run in full-state mode only */
         if (!a.pilot && (!FULL || k > a.revive_k) && !__any(!n.dead)) break;
-        const DStep st = plan_get<SP>(a.steps, k);
+        const DStep st = plan_at(a.steps, k);
         if (st.kind == 0) {
-            const DComp &c = plan_get<SP>(a.comps, st.first);
+            const DComp &c = plan_at(a.comps, st.first);
             tr_in<FULL>(n, c);
             /* a mesh step at the start of a segment after a walk launch
              * takes the walk's hits (at the neutron's input queue slot,
@@ -240,13 +230,13 @@ DEV void run_range(NS &n, const SegArgs &a, HistCtx &hc, int k0, int k1, DeferQ 
         } else if constexpr ((F & FEAT_NLIN) != 0u) {
             for (uint32_t ev = 0; ev < st.multi; ev++) {
                 for (uint32_t j = 0; j < st.count; j++) {
-                    const DComp &c = plan_get<SP>(a.comps, st.first + j);
+                    const DComp &c = plan_at(a.comps, st.first + j);
                     tr_in<FULL>(n, c);
                     geom_dispatch<F>(n, c);
                     tr_out<FULL>(n, c);
                 }
                 for (uint32_t j = 0; j < st.count; j++) {
-                    const DComp &c = plan_get<SP>(a.comps, st.first + j);
+                    const DComp &c = plan_at(a.comps, st.first + j);
                     tr_in<FULL>(n, c);
                     scat_dispatch<FULL, F>(n, c, hc);
                     tr_out<FULL>(n, c);
@@ -295,17 +285,17 @@ DEV uint32_t defer_pop(DeferQ &dq, NS &r)
 }
 
 /* reflection half of the deferred step, then the rest of the range */
-template <uint32_t F, bool SP = true>
+template <uint32_t F>
 DEV void defer_finish(NS &r, const SegArgs &a, HistCtx &hc, int k1, DeferQ &dq)
 {
     if constexpr ((F & FEAT_MONO) != 0) {
-        const DComp &c = plan_get<SP>(a.comps, plan_get<SP>(a.steps, a.defer_k).first);
+        const DComp &c = plan_at(a.comps, plan_at(a.steps, a.defer_k).first);
         if (r.pend && !r.dead) {
             s_mono_b(r, c, r.aux);
             tr_out<false>(r, c);
         }
         r.pend = false;
-        run_range<false, F, SP>(r, a, hc, a.defer_k + 1, k1, dq, false);
+        run_range<false, F>(r, a, hc, a.defer_k + 1, k1, dq, false);
     }
 }
 
