@@ -46,8 +46,8 @@ VALU_PEAK_TOPS = 2 * VALU_SINGLE_ISSUE_TOPS
 #           after the plain-copy calibration of the counters: x2.0, x1.0)
 # measured SQ_INSTS_VALU of the timed batch of the 256-setting scan, and per
 # history of the powder forward and adjoint kernels
-TAS_VALU_PER_SCAN = {"valu": 1.4339e10, "commit": "706fd75"}
-POWDER_VALU_PER_HISTORY = {"forward": 82.42, "adjoint": 95.67, "commit": "706fd75"}
+TAS_VALU_PER_SCAN = {"valu": 1.43125e10, "commit": "45ade1f"}
+POWDER_VALU_PER_HISTORY = {"forward": 84.50, "adjoint": 94.48, "commit": "45ade1f"}
 
 
 def valu_roofline(kernel, lane_ops, launch_ms, basis, extra=None):
@@ -63,10 +63,10 @@ def valu_roofline(kernel, lane_ops, launch_ms, basis, extra=None):
 
 
 PMC_PER_HISTORY = {
-    "let_like": {"kernel": "k_segment<false,true>", "valu": 76.01, "lanes": 0.643, "fetch": 0.0081,
-                 "write": 1.2562, "commit": "4fb921b"},
+    "let_like": {"kernel": "k_segment<false,true>", "valu": 77.21, "lanes": 0.661, "fetch": 4.246,
+                 "write": 13.534, "commit": "45ade1f"},
     "imaging": {"kernel": "k_mesh_walk", "valu": 110.90, "lanes": 0.568, "fetch": 32.24, "write": 25.50,
-                "commit": "f4c8cd0"},
+                "commit": "45ade1f"},
     "minimal": None,
 }
 

@@ -1326,7 +1326,13 @@ static void launch_segment(const SegArgs &a, bool full, bool first, int v, unsig
                        (size_t)(wg / 64) * a.defer_lds + a.acc_bytes, s, a);
 }
 
-/* resident blocks per CU of a kernel instance (for a persistent grid) */
+/* resident blocks per CU of a kernel instance (for a persistent grid).  The
+ * mesh instances' source segment, which writes the queue and the listed
+ * walk records (about 90 B per history), runs faster at three waves per
+ * SIMD than at the four its registers allow; its grid is held there. */
+#ifndef RAMP_MESH_FIRST_WAVES
+#define RAMP_MESH_FIRST_WAVES 3
+#endif
 static unsigned blocks_per_cu(bool full, bool first, int v, unsigned wg, uint32_t defer_lds, uint32_t acc_bytes)
 {
     int nb = 0;
@@ -1334,6 +1340,10 @@ static unsigned blocks_per_cu(bool full, bool first, int v, unsigned wg, uint32_
                                                      (size_t)(wg / 64) * defer_lds + acc_bytes) != hipSuccess ||
         nb < 1)
         nb = 1;
+    if (!full && first && (kVariants[v] & FEAT_MESH) != 0u) {
+        const int cap = (int)(RAMP_MESH_FIRST_WAVES * 4 * 64 / wg);  /* 4 SIMDs per CU */
+        if (cap >= 1 && nb > cap) nb = cap;
+    }
     return (unsigned)nb;
 }
 
