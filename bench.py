@@ -50,6 +50,29 @@ TAS_VALU_PER_SCAN = {"valu": 1.43125e10, "commit": "45ade1f"}
 POWDER_VALU_PER_HISTORY = {"forward": 84.50, "adjoint": 94.48, "commit": "45ade1f"}
 
 
+# Algorithmic work of the LET-like source segment, per history: the
+# primitives its histories need, counted on the oracle's op-count build
+# (`python oracle/opcount.py 65536 7`: the instrument cut after its first
+# chopper, single-threaded; parameter-only expressions and unread values
+# excluded), and each primitive's own VALU instruction count on gfx950
+# (`python tests/native/primitive_cost.py`: static count in a one-primitive
+# kernel).  Their product is a lower bound on the lane-ops a history needs:
+# the plain adds, multiplies and compares between the primitives are not
+# counted.  tests/test_opcount.py re-derives both tables.
+ALG_PER_HISTORY = {"threefry": 5.0, "div": 41.76414489746094, "sqrt": 2.73858642578125,
+                   "exp": 0.73858642578125, "log": 0.0, "pow": 0.0, "tanh": 1.6868743896484375,
+                   "atan2": 1.995635986328125, "sin": 0.0, "cos": 0.0, "asin": 0.0, "acos": 0.0,
+                   "fmod": 0.9978179931640625, "guide_bounce": 2.9347686767578125,
+                   "guide_exit": 3.99566650390625}
+PRIMITIVE_VALU = {"threefry": 117, "div": 11, "sqrt": 17, "exp": 38, "log": 49, "pow": 121, "tanh": 64,
+                  "atan2": 127, "sin": 30, "cos": 31, "asin": 44, "acos": 46, "fmod": 63}
+
+
+def algorithmic_lane_ops():
+    """Lane-ops per history of the LET-like source segment's primitives."""
+    return sum(ALG_PER_HISTORY[k] * PRIMITIVE_VALU[k] for k in PRIMITIVE_VALU)
+
+
 def valu_roofline(kernel, lane_ops, launch_ms, basis, extra=None):
     """VALU roofline of a kernel: lane-op slots issued (measured instruction
     count x 64) per second of its launch, against the dual-issue peak."""
@@ -270,6 +293,19 @@ def main():
                      "single_issue_peak": VALU_SINGLE_ISSUE_TOPS,
                      "valu_insts_per_history": pmc["valu"], "active_lane_share": pmc["lanes"],
                      "traffic": (pmc["fetch"] + pmc["write"]) * N, "pmc_commit": pmc["commit"]})
+        if a.workload == "let_like":
+            # the same launch priced by the algorithm's primitives instead of
+            # the instructions issued (divergence, control flow, spills and
+            # the arithmetic between primitives excluded)
+            alg = algorithmic_lane_ops()
+            alg_ach = alg * N / (launch_ms * 1e-3) / 1e12
+            roof["algorithmic"] = {
+                "basis": "primitives per history counted on the oracle (oracle/opcount.py) x their "
+                         "gfx950 VALU instruction counts (tests/native/primitive_cost.py)",
+                "lane_ops_per_history": alg, "achieved": alg_ach, "frac": alg_ach / VALU_PEAK_TOPS,
+                "frac_single_issue": alg_ach / VALU_SINGLE_ISSUE_TOPS,
+                "share_of_issued_slots": alg / (pmc["valu"] * 64),
+                "primitives_per_history": {k: v for k, v in ALG_PER_HISTORY.items() if v}}
     out = {
         "metric": METRIC,
         "value": N * K / dt,

@@ -29,6 +29,14 @@ extern "C" {
  * dets[d].n_bins doubles, accumulated into (not cleared).
  * neutrons16 / isect8 / iidx (may be NULL): final state of the last chunk.
  * n_threads <= 0: OpenMP default.  Returns 0 or a negative error. */
+/* Operation counts of the ORACLE_COUNT build (liboracle_count.so; zeros in
+ * the others), accumulated over single-threaded runs since the last reset:
+ * out[0] Threefry evaluations, [1] FP32 divisions, [2] square roots,
+ * [3..12] calls of exp, log, pow, tanh, atan2, sin, cos, asin, acos, fmod,
+ * [13] guide bounce iterations, [14] guide exit iterations.  reset != 0
+ * clears them after reading. */
+void oracle_opcount(uint64_t *out, int reset);
+
 int oracle_run(const ramp_plan_desc *desc, uint64_t n_hist, uint64_t chunk,
                const uint32_t *seeds, uint32_t n_seeds, double **hist_w, double **hist_w2,
                float *neutrons16, float *isect8, uint32_t *iidx, int n_threads,

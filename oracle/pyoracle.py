@@ -51,10 +51,24 @@ def load():
         lib.oracle_last_error.restype = C.c_char_p
         lib.oracle_math_d.restype = None
         lib.oracle_math_d.argtypes = [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_uint64]
+        lib.oracle_opcount.restype = None
+        lib.oracle_opcount.argtypes = [C.POINTER(C.c_uint64), C.c_int]
         lib.oracle_threefry4x32.restype = None
         lib.oracle_threefry4x32.argtypes = [C.POINTER(C.c_uint32)] * 3
         _lib = lib
     return _lib
+
+
+OPCOUNT_NAMES = ("threefry", "div", "sqrt", "exp", "log", "pow", "tanh", "atan2", "sin", "cos", "asin", "acos",
+                 "fmod", "guide_bounce", "guide_exit")
+
+
+def opcount(reset=True):
+    """The operation counts of the ORACLE_COUNT build since the last reset
+    (zeros from the other builds), as {name: count}."""
+    out = (C.c_uint64 * len(OPCOUNT_NAMES))()
+    load().oracle_opcount(out, 1 if reset else 0)
+    return dict(zip(OPCOUNT_NAMES, (int(v) for v in out)))
 
 
 def run(inst, n_hist, seeds, chunk=0, keep_state=False, threads=0, trace=False):

@@ -49,7 +49,67 @@
 
 #include "oracle.h"
 #include "../ramp_amd/csrc/rmath.h"
+
+/* ---- operation counts (the ORACLE_COUNT build, liboracle_count.so) -----
+ * Counts, per history, the work the algorithm itself needs on the LET-like
+ * source segment's components (MCont, GPlane, SGuide, SChopper, frame
+ * transforms): Threefry evaluations whose value is used, FP32 divisions
+ * and square roots, calls of each builtin transcendental, and guide
+ * bounce-loop iterations.  Expressions that depend on parameters only (the reference
+ * re-evaluates them per neutron) and draws or values that nothing reads
+ * (the polarisation in plans without polarisation components, a chopper
+ * draw scaled by a zero jitter) are not counted: OPC_PAUSE/OPC_RESUME
+ * bracket them.  Single-threaded runs only (one counter array). */
+enum {
+    OPC_THREEFRY, OPC_DIV, OPC_SQRT, OPC_EXP, OPC_LOG, OPC_POW, OPC_TANH, OPC_ATAN2, OPC_SIN, OPC_COS,
+    OPC_ASIN, OPC_ACOS, OPC_FMOD, OPC_GUIDE_BOUNCE, OPC_GUIDE_EXIT, OPC_N
+};
+#ifdef ORACLE_COUNT
+static uint64_t g_opc[OPC_N];
+static int g_opc_pause;
+/* a function, so that two counts in one expression are sequenced */
+static void opc_add(int k, uint64_t v)
+{
+    if (!g_opc_pause) g_opc[k] += v;
+}
+#define OPC_ADD(k, v) opc_add((k), (uint64_t)(v))
+#define OPC_PAUSE() (g_opc_pause++)
+#define OPC_RESUME() (g_opc_pause--)
+#define rm_sqrt(x) (OPC_ADD(OPC_SQRT, 1), rm_sqrt(x))
+#define rm_exp(x) (OPC_ADD(OPC_EXP, 1), rm_exp(x))
+#define rm_log(x) (OPC_ADD(OPC_LOG, 1), rm_log(x))
+#define rm_pow(x, y) (OPC_ADD(OPC_POW, 1), rm_pow((x), (y)))
+#define rm_tanh(x) (OPC_ADD(OPC_TANH, 1), rm_tanh(x))
+#define rm_atan2(y, x) (OPC_ADD(OPC_ATAN2, 1), rm_atan2((y), (x)))
+#define rm_sin(x) (OPC_ADD(OPC_SIN, 1), rm_sin(x))
+#define rm_cos(x) (OPC_ADD(OPC_COS, 1), rm_cos(x))
+#define rm_asin(x) (OPC_ADD(OPC_ASIN, 1), rm_asin(x))
+#define rm_acos(x) (OPC_ADD(OPC_ACOS, 1), rm_acos(x))
+#define rm_fmod(x, y) (OPC_ADD(OPC_FMOD, 1), rm_fmod((x), (y)))
+void oracle_opcount(uint64_t *out, int reset)
+{
+    for (int k = 0; k < OPC_N; k++) {
+        if (out) out[k] = g_opc[k];
+        if (reset) g_opc[k] = 0;
+    }
+    if (reset) g_opc_pause = 0;
+}
+#else
+#define OPC_ADD(k, v) ((void)0)
+#define OPC_PAUSE() ((void)0)
+#define OPC_RESUME() ((void)0)
+void oracle_opcount(uint64_t *out, int reset)
+{
+    (void)reset;
+    if (out)
+        for (int k = 0; k < OPC_N; k++) out[k] = 0;
+}
+#endif
+
 #include "../ramp_amd/csrc/rvec.h"
+#ifdef ORACLE_COUNT
+#define v3_norm(a) (OPC_ADD(OPC_DIV, 3), v3_norm(a))
+#endif
 #include "../ramp_amd/csrc/rrng.h"
 #include "../ramp_amd/csrc/rconst.h"
 #include "../ramp_amd/csrc/rslots.h"
@@ -103,7 +163,11 @@ static inline void isect_reset(on_t *n)
     n->is[7] = RC_SENTINEL;
 }
 
-static inline float ornd(on_t *n) { return rr_draw(&n->N_SB, n->tid); }
+static inline float ornd(on_t *n)
+{
+    OPC_ADD(OPC_THREEFRY, 1);
+    return rr_draw(&n->N_SB, n->tid);
+}
 
 /* OpenCL min/max for floats as the spec defines them (box.cl:36-37) */
 static inline float ocl_min(float x, float y) { return (y < x) ? y : x; }
@@ -199,12 +263,15 @@ static void src_position_direction(on_t *n, const float *p, float vel)
 
 static void src_polarisation(on_t *n)
 {
-    /* flat_mod.cl:39-43; FP64: theta = 2*M_PI*rand() */
+    /* flat_mod.cl:39-43; FP64: theta = 2*M_PI*rand().  (Operation counts:
+     * only polarisation components read it.) */
+    OPC_PAUSE();
     float theta = twopi_u(ornd(n));
     float phi = rm_acos(1 - 2 * ornd(n));
     n->s[6] = rm_sin(phi) * rm_cos(theta);
     n->s[7] = rm_sin(phi) * rm_sin(theta);
     n->s[8] = rm_cos(phi);
+    OPC_RESUME();
 }
 
 static void src_clear(on_t *n, int seed)
@@ -234,9 +301,14 @@ static void src_flat(on_t *n, const float *p, int num_sim, int seed)
 static float maxwell_ref(float E, float T)
 {
     float rootE = rm_sqrt(E);
+    OPC_PAUSE();  /* (operation counts: parameters only) */
     float overKT = 1.0f / (RC_K_B * T);
+    OPC_RESUME();
     double r = (double)(2.0f * rootE) / sqrt(RC_M_PI_D);
+    OPC_ADD(OPC_DIV, 1);
+    OPC_PAUSE();
     r = r * (double)rm_pow(overKT, (float)(3.0f / 2.0f));
+    OPC_RESUME();
     r = r * (double)rm_exp(-E * overKT);
     return (float)r;
 }
@@ -251,20 +323,36 @@ static void src_cont(on_t *n, const float *p, int num_sim, int seed)
     float E_val = p[SP_EMIN] + E_range * ornd(n);
     float E_J = RC_MEV2J * E_val;
     float T1 = p[SP_T1], T2 = p[SP_T2], T3 = p[SP_T3];
+    OPC_PAUSE();  /* (operation counts: parameters only) */
     float dm1 = maxwell_ref(RC_K_B * T1, T1);
     float dm2 = maxwell_ref(RC_K_B * T2, T2);
     float dm3 = maxwell_ref(RC_K_B * T3, T3);
+    OPC_RESUME();
     float term1;
     float join = p[SP_TAIL_JOIN];
     float EjJ = join * (RC_K_B * T1);
-    if (join > 0.0f && E_J > EjJ)
-        term1 = p[SP_I1] * (maxwell_ref(EjJ, T1) * (EjJ / E_J)) / dm1;
-    else
+    if (join > 0.0f && E_J > EjJ) {
+        OPC_PAUSE();
+        float mj = maxwell_ref(EjJ, T1);
+        OPC_RESUME();
+        term1 = p[SP_I1] * (mj * (EjJ / E_J)) / dm1;
+        OPC_ADD(OPC_DIV, 2);
+    } else {
         term1 = p[SP_I1] * maxwell_ref(E_J, T1) / dm1;
+        OPC_ADD(OPC_DIV, 1);
+    }
     float M = term1;
+    /* (operation counts: a term of zero intensity adds a signed zero) */
+    if (p[SP_I2] == 0.0f) OPC_PAUSE();
     M += p[SP_I2] * maxwell_ref(E_J, T2) / dm2;
+    if (p[SP_I2] == 0.0f) OPC_RESUME();
+    else OPC_ADD(OPC_DIV, 1);
+    if (p[SP_I3] == 0.0f) OPC_PAUSE();
     M += p[SP_I3] * maxwell_ref(E_J, T3) / dm3;
+    if (p[SP_I3] == 0.0f) OPC_RESUME();
+    else OPC_ADD(OPC_DIV, 1);
     n->N_P = M / (float)num_sim;
+    OPC_ADD(OPC_DIV, 1);
     /* adjoint: d log w / dT1 of a single-temperature source, in FP64:
      * w ~ sqrt(E/kT) e^{1 - E/kT}, and ~ T on the 1/E tail */
     n->a_gT = 0.0;
@@ -377,7 +465,9 @@ static void src_parallel(on_t *n, const float *p, int num_sim, int seed)
 static void o_transform(on_t *n, const ramp_component_desc *c)
 {
     if (n->N_DIE > 0.0f) return;
+    OPC_PAUSE();  /* (operation counts: parameters only) */
     rmat3 R = rmat3_euler(c->rot[0], c->rot[1], c->rot[2]);
+    OPC_RESUME();
     rf3 p = v3_sub(n_pos(n), rf3_make(c->pos[0], c->pos[1], c->pos[2]));
     set_pos(n, rmat3_derotate(&R, p));
     set_vel(n, rmat3_derotate(&R, n_vel(n)));
@@ -388,7 +478,9 @@ static void o_transform(on_t *n, const ramp_component_desc *c)
 static void o_untransform(on_t *n, const ramp_component_desc *c)
 {
     if (n->N_DIE > 0.0f) return;
+    OPC_PAUSE();  /* (operation counts: parameters only) */
     rmat3 R = rmat3_euler(c->rot[0], c->rot[1], c->rot[2]);
+    OPC_RESUME();
     rf3 p = rmat3_rotate(&R, n_pos(n));
     set_pos(n, v3_add(p, rf3_make(c->pos[0], c->pos[1], c->pos[2])));
     set_vel(n, rmat3_rotate(&R, n_vel(n)));
@@ -399,7 +491,9 @@ static void o_untransform(on_t *n, const ramp_component_desc *c)
 static void o_terminate(on_t *n, uint32_t restore)
 {
     if (n->N_DIE > 0.0f) return;
+    OPC_PAUSE();  /* (operation counts: length == 0 needs no square root) */
     if (v3_len(is_pos2(n)) == 0.0f && restore == 0) n->N_DIE = 1.0f;
+    OPC_RESUME();
 }
 
 /* ======================= geometry ======================================= */
@@ -423,10 +517,12 @@ static void g_plane(on_t *n, const ramp_part_desc *g, uint32_t ci)
         t = (-pos.z) / vel.z;
         x = pos.x + t * vel.x;
         y = pos.y + t * vel.y;
+        OPC_ADD(OPC_DIV, 1);
     } else if (o == 1) {
         t = (-pos.x) / vel.x;
         x = pos.z + t * vel.z;
         y = pos.y + t * vel.y;
+        OPC_ADD(OPC_DIV, 1);
     }
     if ((rm_fabs(x) < g->p[GP_W] / 2.0f) && (rm_fabs(y) < g->p[GP_H] / 2.0f) && t < n->is[3] &&
         t < n->is[7] && t > 0.0f && v3_dot(vel, rf3_make(0.0f, 0.0f, 1.0f)) > 0.0f) {
@@ -639,7 +735,10 @@ static void s_chopper(on_t *n, const ramp_part_desc *s, uint32_t ci)
     n->N_TOF += n->is[7];
     /* FP64: Tg = 2.0f*M_PI / (float)n_slits */
     float Tg = (float)((double)2.0f * RC_M_PI_D / (double)(float)s->ni[CI_NSLITS]);
+    /* (operation counts: a draw scaled by a zero jitter is not needed) */
+    if (p[CP_JITTER] == 0.0f) OPC_PAUSE();
     float pha_error = p[CP_JITTER] * 2.0f * (ornd(n) - 0.5f);
+    if (p[CP_JITTER] == 0.0f) OPC_RESUME();
     float thi_chop = p[CP_FREQ] * (n->N_TOF - rm_fabs(p[CP_PHASE])) + pha_error;
     float thi_neut = rm_atan2(n->N_X, n->N_Y + p[CP_RADIUS]);
     float thi_diff = rm_fabs(thi_chop - thi_neut);
@@ -647,6 +746,7 @@ static void s_chopper(on_t *n, const ramp_part_desc *s, uint32_t ci)
     if (rm_fabs(ang_diff - Tg) < ang_diff) ang_diff = rm_fabs(ang_diff - Tg);
     float width = rm_atan2(p[CP_SLIT_W], n->N_Y + p[CP_RADIUS]);
     if (ang_diff > width / 2.0f) n->N_DIE = 1.0f;
+    OPC_ADD(OPC_DIV, 1);
     s_done(n, ci);
 }
 
@@ -667,6 +767,7 @@ static float reflectivity_ref(float q, float R0, float Qc, float alpha, float m,
         }
     }
     arg = W > 0 ? (q - m * Qc) / W : 11;
+    if (W > 0) OPC_ADD(OPC_DIV, 1);
     if (arg > 10 || m <= 0 || Qc <= 0 || R0 <= 0) return 0.0f;
     if (m < 1) {
         Qc *= m;
@@ -707,11 +808,17 @@ static void s_guide(on_t *n, const ramp_part_desc *s, uint32_t ci)
         if (vdotn_v2 < 0.0 && (t2 = (cv1 + cv2) / vdotn_v2) < t1) { t1 = t2; i = 2; }
         if (vdotn_h1 < 0.0 && (t2 = (ch1 - ch2) / vdotn_h1) < t1) { t1 = t2; i = 3; }
         if (vdotn_h2 < 0.0 && (t2 = (ch1 + ch2) / vdotn_h2) < t1) { t1 = t2; i = 4; }
+        OPC_ADD(OPC_DIV, 1 + (vdotn_v1 < 0.0) + (vdotn_v2 < 0.0) + (vdotn_h1 < 0.0) + (vdotn_h2 < 0.0));
+        OPC_ADD(i == 0 ? OPC_GUIDE_EXIT : OPC_GUIDE_BOUNCE, 1);
         if (i == 0) break;
         set_pos(n, v3_add(n_pos(n), v3_scale(t1, vel)));
         n->N_TOF += t1;
         float nlen2, d;
-        /* FP64: q = V2K*(-2.0)*vdotn/sqrt(nlen2); d = 2.0*vdotn/nlen2 */
+        /* FP64: q = V2K*(-2.0)*vdotn/sqrt(nlen2); d = 2.0*vdotn/nlen2.
+         * (Operation counts: two divisions; nlen2 and its root depend on
+         * the parameters only.) */
+        OPC_ADD(OPC_DIV, 2);
+        OPC_PAUSE();
         switch (i) {
         case 1:
             nlen2 = l * l + ww * ww;
@@ -742,6 +849,7 @@ static void s_guide(on_t *n, const ramp_part_desc *s, uint32_t ci)
             n->N_VZ = n->N_VZ - d * hh;
             break;
         }
+        OPC_RESUME();
         float refl = reflectivity_ref(q, p[GU_R0], p[GU_QC], p[GU_ALPHA], p[GU_M], p[GU_W]);
         n->N_P *= refl;
         /* FP64: refl < 1e-12 compares in double */
