@@ -1831,8 +1831,18 @@ static int run_plan(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uint32_
             if (sg >= 1) a.in = bufs.q[cur];
             a.out = bufs.q[sg == 0 ? 0 : cur ^ 1];
             unsigned grid;
-            if (sg == 0) grid = (unsigned)std::min<uint64_t>((torun + wg - 1) / wg, grid_first);
-            else grid = grid_rest;
+            if (sg == 0) {
+                /* the fewest waves that still take the histories in as few
+                 * 64-history rounds each as the full persistent grid would:
+                 * every wave then runs whole rounds (a small run does not
+                 * spread a last partial round over every resident wave) */
+                const uint64_t wpb = wg / 64, wmax = (uint64_t)grid_first * wpb;
+                const uint64_t rounds = std::max<uint64_t>(1, (torun + 64 * wmax - 1) / (64 * wmax));
+                const uint64_t waves = (torun + 64 * rounds - 1) / (64 * rounds);
+                grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((waves + wpb - 1) / wpb, grid_first));
+            } else {
+                grid = grid_rest;
+            }
             if (sg == 0 && ck == 0) HIPCHK(hipEventRecord(ctx->ev[2], s));
             launch_segment(a, full, sg == 0, p->variant, grid, wg, s);
             if (sg == 0 && ck == 0) HIPCHK(hipEventRecord(ctx->ev[3], s));
