@@ -228,6 +228,9 @@ DEV void run_range(NS &n, const SegArgs &a, HistCtx &hc, int k0, int k1, DeferQ 
                 if (traced) trace_alive(n, a.trace, a.trace_cap, a.trace_base + n.key, c.trace);
             }
         } else if constexpr ((F & FEAT_NLIN) != 0u) {
+            /* (a plan with a non-linear block always runs an instance that
+             * carries FEAT_NLIN: the host picks the instance from the plan's
+             * feature bits, so other instances never meet such a step) */
             for (uint32_t ev = 0; ev < st.multi; ev++) {
                 for (uint32_t j = 0; j < st.count; j++) {
                     const DComp &c = plan_at(a.comps, st.first + j);
@@ -411,12 +414,13 @@ DEV void seg_tail(const NS &n, bool valid, const SegArgs &a, uint64_t beg, uint3
  * deferral queue whenever it holds 64, and at the end. */
 /* Waves per SIMD each instance is compiled for (amdgpu_waves_per_eu; 1
  * leaves the register count to the compiler).  The fast-mode kernels are
- * latency-bound at the occupancy their register use gives by itself (119 to
- * 168 VGPRs: three or four waves), so they are held to more waves, trading
+ * latency-bound at the occupancy their register use gives by itself (111 to
+ * 161 VGPRs: three or four waves), so they are held to more waves, trading
  * a few spills outside the inner loops for latency hiding; each value is
- * the best of the measured sweep on its bench workload.  The plain
- * full-state instances sit a few registers above three waves and fit there;
- * the other full-state ones would spill. */
+ * the best of the measured sweep on its bench workload.  Of the full-state
+ * instances the plain one (155 VGPRs by itself) runs the MCPL, SRescal and
+ * plugin workloads 1-6 % faster at four waves; the others keep the count
+ * the compiler picks. */
 #ifndef RAMP_W_SEG_F0
 #define RAMP_W_SEG_F0 7      /* LET-like, minimal: later segments */
 #endif
@@ -438,9 +442,12 @@ DEV void seg_tail(const NS &n, bool valid, const SegArgs &a, uint64_t beg, uint3
 #ifndef RAMP_W_ADJ
 #define RAMP_W_ADJ 6         /* powder adjoint */
 #endif
+#ifndef RAMP_W_FULL_F0
+#define RAMP_W_FULL_F0 4     /* plain full-state (MCPL, SRescal, plugins, trace) */
+#endif
 constexpr int seg_min_waves(bool full, bool first, uint32_t f)
 {
-    return full ? (f == 0u ? 3 : 1)
+    return full ? (f == 0u ? RAMP_W_FULL_F0 : 1)
                 : (f & FEAT_MONO) != 0u ? RAMP_W_SEG_MONO
                 : f == 0u ? (first ? RAMP_W_SEG_F0_FIRST : RAMP_W_SEG_F0)
                 : f == FEAT_SAMPLE ? RAMP_W_SEG_SAMPLE
