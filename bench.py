@@ -378,7 +378,7 @@ def bench_tas_scan(a, wl, sync):
         a.no_cpu = True
     b = ramp_amd.BatchInstrument(path, settings, hist_mode=a.hist_mode)
     t = time.perf_counter()
-    best, tuning = b.tune(counts, scale=1.0 / 16)
+    best, tuning = b.tune(counts)
     tune_s = time.perf_counter() - t
     seeds = np.random.default_rng(20261203).integers(0, 4097, size=len(settings))
 
