@@ -35,7 +35,7 @@ DEV float source_dlogw_dT(const NS &n, const DPart &s)
 }
 
 template <uint32_t F>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RAMP_W_ADJ))) void k_adjoint(SegArgs a, AdjArgs x)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((F) == FEAT_SAMPLE ? RAMP_W_ADJ : 1))) void k_adjoint(SegArgs a, AdjArgs x)
 {
     __shared__ double red[4][ADJ_NG];
     __shared__ double lsum[4][ADJ_NLDS];

@@ -28,7 +28,7 @@ struct BatchArgs {
  * whenever it holds 64 and at the end of each tile, so it only ever holds
  * neutrons of one setting. */
 template <uint32_t F>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(((F) & FEAT_MONO) != 0u ? RAMP_W_BATCH_MONO : 1)))
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu((F) == FEAT_MONO ? RAMP_W_BATCH_MONO : 1)))
 void k_batch(BatchArgs b)
 {
     extern __shared__ float batch_lds[];

@@ -1326,21 +1326,22 @@ static void launch_segment(const SegArgs &a, bool full, bool first, int v, unsig
                        (size_t)(wg / 64) * a.defer_lds + a.acc_bytes, s, a);
 }
 
-/* resident blocks per CU of a kernel instance (for a persistent grid).  The
- * mesh instances' source segment, which writes the queue and the listed
- * walk records (about 90 B per history), runs faster at three waves per
- * SIMD than at the four its registers allow; its grid is held there. */
+/* resident blocks per CU of a kernel instance (for a persistent grid).  In
+ * a plan with a GMesh the source segment, which writes the queue and the
+ * listed walk records (about 90 B per history), runs faster at three waves
+ * per SIMD than at the four its registers allow; its grid is held there. */
 #ifndef RAMP_MESH_FIRST_WAVES
 #define RAMP_MESH_FIRST_WAVES 3
 #endif
-static unsigned blocks_per_cu(bool full, bool first, int v, unsigned wg, uint32_t defer_lds, uint32_t acc_bytes)
+static unsigned blocks_per_cu(bool full, bool first, int v, unsigned wg, uint32_t defer_lds, uint32_t acc_bytes,
+                              bool mesh)
 {
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pick_kernel(full, first, v), (int)wg,
                                                      (size_t)(wg / 64) * defer_lds + acc_bytes) != hipSuccess ||
         nb < 1)
         nb = 1;
-    if (!full && first && (kVariants[v] & FEAT_MESH) != 0u) {
+    if (!full && first && mesh) {
         const int cap = (int)(RAMP_MESH_FIRST_WAVES * 4 * 64 / wg);  /* 4 SIMDs per CU */
         if (cap >= 1 && nb > cap) nb = cap;
     }
@@ -1698,8 +1699,9 @@ static int run_plan(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uint32_
     }
     int nseg = (int)p->bounds.size() - 1;
     /* persistent grids: as many blocks as are co-resident */
-    unsigned grid_first = (unsigned)ctx->n_cu * blocks_per_cu(full, true, p->variant, wg, a.defer_lds, p->acc_bytes);
-    unsigned grid_rest = (unsigned)ctx->n_cu * blocks_per_cu(full, false, p->variant, wg, a.defer_lds, p->acc_bytes);
+    const bool mesh = (p->feat & FEAT_MESH) != 0u;
+    unsigned grid_first = (unsigned)ctx->n_cu * blocks_per_cu(full, true, p->variant, wg, a.defer_lds, p->acc_bytes, mesh);
+    unsigned grid_rest = (unsigned)ctx->n_cu * blocks_per_cu(full, false, p->variant, wg, a.defer_lds, p->acc_bytes, mesh);
     uint64_t max_waves = (uint64_t)std::max(grid_first, grid_rest) * (wg / 64);
     if (nseg > 1 && noalloc) {
         if (max_waves > NA_MAX_WAVES || chunk + max_waves > NA_CAP)

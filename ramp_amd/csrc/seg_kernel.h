@@ -417,10 +417,14 @@ DEV void seg_tail(const NS &n, bool valid, const SegArgs &a, uint64_t beg, uint3
  * latency-bound at the occupancy their register use gives by itself (111 to
  * 161 VGPRs: three or four waves), so they are held to more waves, trading
  * a few spills outside the inner loops for latency hiding; each value is
- * the best of the measured sweep on its bench workload.  Of the full-state
+ * the best of the measured sweep on its bench workload.  The instance with
+ * every feature (ISIS sources, non-linear blocks) keeps the compiler's two
+ * waves: an ISIS S(q,w) run of 1e8 histories took 136.7 ms so and 146.2 ms
+ * held to three (168 VGPRs, 120 B of spills).  Of the full-state
  * instances the plain one (155 VGPRs by itself) runs the MCPL, SRescal and
  * plugin workloads 1-6 % faster at four waves; the others keep the count
- * the compiler picks. */
+ * the compiler picks, as do the batch and adjoint kernels of the instances
+ * no bench workload measures. */
 #ifndef RAMP_W_SEG_F0
 #define RAMP_W_SEG_F0 7      /* LET-like, minimal: later segments */
 #endif
@@ -448,7 +452,7 @@ DEV void seg_tail(const NS &n, bool valid, const SegArgs &a, uint64_t beg, uint3
 constexpr int seg_min_waves(bool full, bool first, uint32_t f)
 {
     return full ? (f == 0u ? RAMP_W_FULL_F0 : 1)
-                : (f & FEAT_MONO) != 0u ? RAMP_W_SEG_MONO
+                : f == FEAT_MONO ? RAMP_W_SEG_MONO
                 : f == 0u ? (first ? RAMP_W_SEG_F0_FIRST : RAMP_W_SEG_F0)
                 : f == FEAT_SAMPLE ? RAMP_W_SEG_SAMPLE
                 : f == FEAT_MESH ? RAMP_W_SEG_MESH : 1;
