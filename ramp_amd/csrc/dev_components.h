@@ -699,7 +699,9 @@ DEV void s_chopper(NS &n, const DComp &c)
 
 /* reflectivity_func (include/ref.h:3-54) with the parameter-only part
  * precomputed: d[4] m, d[5] W, d[6] alpha, d[7] beta, d[14] Qc (after the
- * m<1 rescale), d[15] m (after it), d[16] 1 when the parameters force R=0 */
+ * m<1 rescale), d[15] m (after it), d[16] 1 when the parameters force R=0.
+ * The reference's double chain R0*0.5*(1-tanh)*(poly) is evaluated in
+ * double. */
 DEV float guide_refl(const DPart &s, float q)
 {
     float m = s.d[4], W = s.d[5], alpha = s.d[6], beta = s.d[7];
@@ -708,22 +710,25 @@ DEV float guide_refl(const DPart &s, float q)
     if (arg > 10 || s.d[16] != 0.0f) return 0.0f;
     float Qc2 = s.d[14];
     if (q <= Qc2) return R0;
-    /* double chain R0*0.5*(1-tanh)*(poly): the first product is exact */
-    float a = R0 * 0.5f;
-    float b = 1 - rm_tanh(arg);
-    float poly = 1 - alpha * (q - Qc2) + beta * (q - Qc2) * (q - Qc2);
-    rdf ab = df_prod(a, b);
-    return df_mul_f(ab, poly);
+    double R = (double)R0 * 0.5 * (double)(1 - rm_tanh(arg));
+    return (float)(R * (double)(1 - alpha * (q - Qc2) + beta * (q - Qc2) * (q - Qc2)));
 }
 
 /* scat/guide.cl:9-150.  d[0..3] ww, hh, whalf, hhalf; d[8] the float
  * threshold equivalent to refl < 1e-12 (double); d[9] -2*V2K;
- * d[10]/d[11] sqrt(l*l+ww*ww), sqrt(l*l+hh*hh); d[12]/d[13] the nlen2s */
+ * d[10]/d[11] sqrt(l*l+ww*ww), sqrt(l*l+hh*hh); d[12]/d[13] the nlen2s;
+ * d[18..19]/d[20..21] the reciprocals of the two roots, as doubles (guides
+ * of positive length) */
 DEV void s_guide(NS &n, const DComp &c)
 {
     const DPart &s = c.s;
     float l = s.p[GU_L];
     float ww = s.d[0], hh = s.d[1], whalf = s.d[2], hhalf = s.d[3];
+    /* wall constants of the two orientations, loaded once (the per-lane
+     * choice between them is a select) */
+    const float sq_v = s.d[10], sq_h = s.d[11], n2_v = s.d[12], n2_h = s.d[13];
+    const double rsq_v = dp_f64(s.d, 18), rsq_h = dp_f64(s.d, 20);
+    const double k2 = (double)s.d[9];
     setpos(n, ipos2(n));
     n.t += n.is[7];
     uint32_t max_bounces = s.ni[GUI_MAXB];
@@ -772,10 +777,13 @@ DEV void s_guide(NS &n, const DComp &c)
         n.t += t1;
         float vdn = (i == 1) ? vdotn_v1 : (i == 2) ? vdotn_v2 : (i == 3) ? vdotn_h1 : vdotn_h2;
         bool vert = (i <= 2);
-        float sq = vert ? s.d[10] : s.d[11];
-        float nlen2 = vert ? s.d[12] : s.d[13];
-        float q = df_div(df_prod(s.d[9], vdn), sq);
-        float d = (2.0f * vdn) / nlen2;
+        /* q = V2K*(-2.0)*vdotn/sqrt(nlen2) is a double expression in the
+         * reference: the product is exact in double and the quotient is
+         * formed in double from the host's reciprocal of the root, then
+         * rounded to float.  d = 2.0*vdotn/nlen2 is a double quotient of two
+         * floats, whose float rounding equals the float quotient's. */
+        float q = (float)rd_div_rc(k2 * (double)vdn, (double)(vert ? sq_v : sq_h), vert ? rsq_v : rsq_h);
+        float d = (2.0f * vdn) / (vert ? n2_v : n2_h);
         if (i == 1) { n.vx = n.vx - d * l; n.vz = n.vz - d * ww; }
         else if (i == 2) { n.vx = n.vx + d * l; n.vz = n.vz - d * ww; }
         else if (i == 3) { n.vy = n.vy - d * l; n.vz = n.vz - d * hh; }

@@ -12,6 +12,8 @@
 
 #include <stdint.h>
 
+#include "rdefs.h"
+
 #define DP_ND 32
 
 struct DPart {
@@ -24,6 +26,14 @@ struct DPart {
     const void *aux;         /* mesh: BVH nodes */
     const void *aux2;        /* mesh: triangle order */
 };
+
+/* a double kept in two consecutive slots of a derived-constant table */
+RHD double dp_f64(const float *d, int k)
+{
+    double v;
+    __builtin_memcpy(&v, d + k, sizeof v);
+    return v;
+}
 
 struct DComp {
     DPart g;

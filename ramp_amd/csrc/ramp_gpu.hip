@@ -597,6 +597,12 @@ static void set_df(float *d, int i, double v)
     d[i + 1] = s.lo;
 }
 
+/* a double in slots i, i+1 (read by dp_f64) */
+static void set_d64(float *d, int i, double v)
+{
+    std::memcpy(d + i, &v, sizeof v);
+}
+
 /* smallest float threshold T with: refl < T  <=>  (double)refl < 1e-12 */
 static float thresh_lt_double(double c)
 {
@@ -681,6 +687,10 @@ static void derive_scat(DPart &s)
         float nv = l * l + ww * ww, nh = l * l + hh * hh;
         s.d[10] = rm_sqrt(nv); s.d[11] = rm_sqrt(nh);
         s.d[12] = nv; s.d[13] = nh;
+        /* reciprocals of the roots, the double-precision divisors of q
+         * (rd_div_rc) */
+        set_d64(s.d, 18, 1.0 / (double)s.d[10]);
+        set_d64(s.d, 20, 1.0 / (double)s.d[11]);
         float Qc = s.p[GU_QC];
         float m2 = m;
         bool zero = (m <= 0 || Qc <= 0 || s.p[GU_R0] <= 0);

@@ -551,6 +551,19 @@ RHD double rd_sin_kernel(double r)
     return r + (r * z) * p;
 }
 
+/* a / c rounded to nearest double, for a divisor c whose reciprocal
+ * rc = 1/c (rounded to nearest) is computed once on the host: q = a*rc is
+ * within an ulp of a/c, the residual a - c*q is exact in one FMA, and
+ * q + r*rc rounds to the double nearest a/c (Markstein's theorem).  Valid
+ * for nonzero finite c and finite a with magnitudes in the float range (the
+ * residual cannot underflow); a zero a gives +0. */
+RHD double rd_div_rc(double a, double c, double rc)
+{
+    double q = a * rc;
+    double r = __builtin_fma(-q, c, a);
+    return __builtin_fma(r, rc, q);
+}
+
 RHD double rd_cos(double x)
 {
     if (!(x - x == 0.0)) return x - x;   /* NaN for NaN and infinities */
