@@ -286,8 +286,22 @@ DEV double mono_gauss(float x, float rms)
     return (double)e / (RC_SQRT_2PI_D * (double)rms);
 }
 
+/* the same for one of the component's two mosaic widths, from the host's
+ * double constants (dp_f64 slots k..k+5): 1/(2.0f*rms*rms), the divisor
+ * sqrt(2.0f*M_PI)*rms and its reciprocal.  The exponent's float quotient
+ * is the float nearest the product with the reciprocal when it is normal
+ * (a float over a float: the double product is within 2^-52 of it, closer
+ * than any rounding boundary); a subnormal exponent gives exp = 1 either
+ * way.  The double quotient is exact (rd_div_rc). */
+DEV double mono_gauss_c(float x, const float *d, int k)
+{
+    float e = rm_exp((float)((double)(-(x) * (x)) * dp_f64(d, k)));
+    return rd_div_rc((double)e, dp_f64(d, k + 2), dp_f64(d, k + 4));
+}
+
 /* scat/monochromator.cl:22-197.  d[0] mos_rms_y, d[1] mos_rms_z,
- * d[2] mos_rms_max, d[3] Q, d[4] zmax, d[5] ymax.  The reference's double
+ * d[2] mos_rms_max, d[3] Q, d[4] zmax, d[5] ymax; doubles for mono_gauss_c
+ * at d[6..11] (mos_rms_y) and d[12..17] (mos_rms_z).  The reference's double
  * sites (GAUSS's normalisation and the products and quotient of GAUSS
  * values, Box-Muller's cos(2.0f*M_PI*u2)) are evaluated in FP64, the cos
  * with the shared rd_cos: only reflected neutrons reach them. */
@@ -482,7 +496,7 @@ DEV bool s_mono_a(NS &n, const DComp &c, uint32_t &aux)
     const DPart &s = c.s;
     setpos(n, ipos2(n));
     n.t += n.is[7];
-    float mry = s.d[0], mrz = s.d[1], Q = s.d[3];
+    float Q = s.d[3];
     float sw = s.p[MO_SLAB_W], sh = s.p[MO_SLAB_H], gap = s.p[MO_GAP];
     float zmax = s.d[4], zmin = -zmax, ymax = s.d[5], ymin = -ymax;
     if (n.z > zmin && n.z < zmax && n.y > ymin && n.y < ymax) {
@@ -494,9 +508,13 @@ DEV bool s_mono_a(NS &n, const DComp &c, uint32_t &aux)
         setpos(n, rmat3_derotate(&m.SR, npos(n)));
         if (rm_fabs(n.z) <= sw / 2.0f && rm_fabs(n.y) <= sh / 2.0f) {
             MonoBragg g = mono_bragg(n, Q);
+            /* the exponents' last division, by 2*mry*mry and 2*mrz*mrz, as
+             * a product with the reciprocal (see mono_gauss_c) */
             float p_reflect = rm_fabs(s.p[MO_R0]) *
-                              rm_exp(-g.ki.z * g.ki.z / (g.l12 * g.l12) * g.delta * g.delta / (2.0f * mry * mry)) *
-                              rm_exp(-g.ki.y * g.ki.y / (g.l12 * g.l12) * g.delta * g.delta / (2.0f * mrz * mrz));
+                              rm_exp((float)((double)(-g.ki.z * g.ki.z / (g.l12 * g.l12) * g.delta * g.delta) *
+                                             dp_f64(s.d, 6))) *
+                              rm_exp((float)((double)(-g.ki.y * g.ki.y / (g.l12 * g.l12) * g.delta * g.delta) *
+                                             dp_f64(s.d, 12)));
             if (drw(n) <= p_reflect) {
                 aux = (uint32_t)col | ((uint32_t)row << 16);
                 return true;
@@ -540,7 +558,7 @@ DEV void s_mono_b(NS &n, const DComp &c, uint32_t aux)
         float phi = w * gx[i];
         float cphi = rm_cos(phi), sphi = rm_sin(phi);
         qv = v3_add(v3_add(c1, v3_scale(cphi, a)), v3_scale(sphi, b));
-        float pr = (float)(mono_gauss(qv.z / qv.x, mry) * mono_gauss(qv.y / qv.x, mrz));
+        float pr = (float)(mono_gauss_c(qv.z / qv.x, s.d, 6) * mono_gauss_c(qv.y / qv.x, s.d, 12));
         total += gw[i] * pr;
     }
     total *= w;
@@ -550,7 +568,7 @@ DEV void s_mono_b(NS &n, const DComp &c, uint32_t aux)
     float phi = mos_sample * rn;
     float cphi = rm_cos(phi), sphi = rm_sin(phi);
     qv = v3_add(v3_add(c1, v3_scale(cphi, a)), v3_scale(sphi, b));
-    float p_reflect = (float)(mono_gauss(qv.z / qv.x, mry) * mono_gauss(qv.y / qv.x, mrz));
+    float p_reflect = (float)(mono_gauss_c(qv.z / qv.x, s.d, 6) * mono_gauss_c(qv.y / qv.x, s.d, 12));
     setvel(n, v3_scale(RC_K2V, v3_add(ki, qv)));
     p_reflect = (float)((double)p_reflect / ((double)total * mono_gauss(phi, mos_sample)));
     if (p_reflect > 1.0f) p_reflect = 1.0f;

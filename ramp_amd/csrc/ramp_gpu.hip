@@ -715,6 +715,16 @@ static void derive_scat(DPart &s)
         float mry = (float)(RC_MIN2RAD_D * (double)s.p[MO_MOS_H] / (double)l8);
         float mrz = (float)(RC_MIN2RAD_D * (double)s.p[MO_MOS_V] / (double)l8);
         s.d[0] = mry; s.d[1] = mrz; s.d[2] = mry > mrz ? mry : mrz;
+        /* GAUSS constants per mosaic width (mono_gauss_c): 1/(2*rms*rms)
+         * of the float product, sqrt(2.0f*M_PI)*rms and its reciprocal */
+        for (int j = 0; j < 2; j++) {
+            const float rms = j == 0 ? mry : mrz;
+            const float c2 = 2.0f * rms * rms;
+            const double dn = RC_SQRT_2PI_D * (double)rms;
+            set_d64(s.d, 6 + 6 * j, 1.0 / (double)c2);
+            set_d64(s.d, 8 + 6 * j, dn);
+            set_d64(s.d, 10 + 6 * j, 1.0 / dn);
+        }
         s.d[3] = (float)((double)2.0f * RC_M_PI_D / (double)s.p[MO_D]);
         float sw = s.p[MO_SLAB_W], sh = s.p[MO_SLAB_H], gap = s.p[MO_GAP];
         s.d[4] = (((float)s.ni[MOI_NH] * (sw + gap)) - gap) / 2.0f;

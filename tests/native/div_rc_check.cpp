@@ -4,7 +4,9 @@
 // divisors and over roots of floats, and divisors and numerators next to
 // powers of two.  Also checks that the float quotient of two floats equals
 // the float rounding of their double quotient (the reference's d =
-// 2.0*vdotn/nlen2, computed as a float division) and of rd_div_rc's.
+// 2.0*vdotn/nlen2, computed as a float division) and of rd_div_rc's, and,
+// where it is a normal float, the float rounding of the numerator times the
+// divisor's double reciprocal (the monochromator's GAUSS exponents).
 // Prints the number of mismatches.
 // Usage: div_rc_check <cases> <seed>
 #include <cmath>
@@ -62,7 +64,10 @@ int main(int argc, char **argv)
             const float fa = (float)a, fc = (float)c;
             const float fq = (float)rd_div_rc((double)fa, (double)fc, 1.0 / (double)fc), fw = fa / fc;
             const float fd = (float)((double)fa / (double)fc);
-            if (std::memcmp(&fq, &fw, sizeof fq) != 0 || std::memcmp(&fd, &fw, sizeof fd) != 0) {
+            const float fm = (float)((double)fa * (1.0 / (double)fc));
+            const bool normal = std::fabs(fw) >= 0x1p-126f || fw == 0.0f;
+            if (std::memcmp(&fq, &fw, sizeof fq) != 0 || std::memcmp(&fd, &fw, sizeof fd) != 0 ||
+                (normal && std::memcmp(&fm, &fw, sizeof fm) != 0)) {
                 if (bad_f < 5) std::printf("float mismatch a=%a c=%a got %a want %a\n", fa, fc, fq, fw);
                 bad_f++;
             }

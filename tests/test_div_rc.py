@@ -4,7 +4,10 @@ uses for the reference's double-precision q = V2K*(-2.0)*vdotn/sqrt(nlen2)
 exact-product, root and near-power-of-two cases.  Also that a float
 quotient of two floats equals the float rounding of their double quotient,
 which lets the guide compute the reference's double d = 2.0*vdotn/nlen2 as
-a float division (tests/native/div_rc_check.cpp)."""
+a float division, and that a float quotient with a normal result equals
+the float rounding of the numerator times the divisor's double reciprocal
+(the monochromator's GAUSS exponents, monochromator.cl:5-8)
+(tests/native/div_rc_check.cpp)."""
 import os
 import shutil
 import subprocess
