@@ -40,6 +40,7 @@ def instrument_cases():
         "let_like": (os.path.join(ROOT, "ramp_amd", "instruments", "let_like.json"), {}),
         "guide_chopper": (os.path.join(ROOT, "tests", "instruments", "guide_chopper.json"),
                           {"v_foc": 841.3}),
+        "guide_variants": (os.path.join(ROOT, "tests", "instruments", "guide_variants.json"), {}),
         "powder": (os.path.join(ROOT, "tests", "instruments", "powder.json"), {"Ei": 25.0}),
         "tas": (os.path.join(ROOT, "tests", "instruments", "tas.json"), {"Mono_angle": ang}),
         "powder_diff": (os.path.join(ROOT, "ramp_amd", "instruments", "powder_diff.json"),
