@@ -1396,14 +1396,16 @@ static void s_powderdiff(on_t *n, const float *Lt, const ramp_part_desc *s, uint
     float inc = 100.0f * p[PD_SINC], inc0 = 100.0f * p[PD_SINC0];
     float ab = 100.0f * p[PD_SABS] * (lam / p[PD_LREF]);
     float ab0 = 100.0f * p[PD_SABS0] * (lam / p[PD_LREF]);
-    float coh = 0.0f, coh0 = 0.0f;
+    float rvv = 1.0f / vv;
+    float scoh = 0.0f, scoh0 = 0.0f;
     for (uint32_t j = 0; j < nl; j++) {
         const float *L = Lt + j * PD_LW;
         if (L[PDL_Q0] < twok) {
-            coh += L[PDL_SSV2] / vv;
-            coh0 += L[PDL_SSV2_0] / vv;
+            scoh += L[PDL_SSV2];
+            scoh0 += L[PDL_SSV2_0];
         }
     }
+    float coh = scoh * rvv, coh0 = scoh0 * rvv;
     float mus = inc + coh, mu = mus + ab;
     float mus0 = inc0 + coh0, mu0 = mus0 + ab0;
     (void)mus;
@@ -1418,18 +1420,19 @@ static void s_powderdiff(on_t *n, const float *Lt, const ramp_part_desc *s, uint
     int ch = -1;
     float sig = inc, sig0 = inc0;
     if (uc >= inc0) {
-        float acc = inc0;
+        float tc = (uc - inc0) * vv;
+        float acc = 0.0f;
         for (uint32_t j = 0; j < nl; j++) {
             const float *L = Lt + j * PD_LW;
             if (L[PDL_Q0] < twok) {
                 ch = (int)j;
-                acc += L[PDL_SSV2_0] / vv;
-                if (uc < acc) break;
+                acc += L[PDL_SSV2_0];
+                if (tc < acc) break;
             }
         }
         if (ch >= 0) {
-            sig = Lt[ch * PD_LW + PDL_SSV2] / vv;
-            sig0 = Lt[ch * PD_LW + PDL_SSV2_0] / vv;
+            sig = Lt[ch * PD_LW + PDL_SSV2] * rvv;
+            sig0 = Lt[ch * PD_LW + PDL_SSV2_0] * rvv;
         }
     }
     n->N_P *= (sig * rm_exp(-mu * x)) * pint0 * mus0 / ((mu0 * rm_exp(-mu0 * x)) * sig0);

@@ -331,14 +331,18 @@ DEV void s_powderdiff(NS &n, const DComp &c, HC &hc)
     float inc = 100.0f * s.p[PD_SINC], inc0 = 100.0f * s.p[PD_SINC0];
     float ab = 100.0f * s.p[PD_SABS] * (lam / s.p[PD_LREF]);
     float ab0 = 100.0f * s.p[PD_SABS0] * (lam / s.p[PD_LREF]);
-    float coh = 0.0f, coh0 = 0.0f;
+    /* the active lines' sigma*v^2 summed, then divided by v^2 once (one
+     * reciprocal per neutron, not a division per line) */
+    const float rvv = 1.0f / vv;
+    float scoh = 0.0f, scoh0 = 0.0f;
     for (uint32_t j = 0; j < nl; j++) {
         const float *L = Lt + j * PD_LW;
         if (L[PDL_Q0] < twok) {
-            coh += L[PDL_SSV2] / vv;
-            coh0 += L[PDL_SSV2_0] / vv;
+            scoh += L[PDL_SSV2];
+            scoh0 += L[PDL_SSV2_0];
         }
     }
+    float coh = scoh * rvv, coh0 = scoh0 * rvv;
     float mus = inc + coh, mu = mus + ab;
     float mus0 = inc0 + coh0, mu0 = mus0 + ab0;
     if (!(mus0 > 0.0f) || !(Lp > 0.0f)) {
@@ -353,18 +357,21 @@ DEV void s_powderdiff(NS &n, const DComp &c, HC &hc)
     int ch = -1;
     float sig = inc, sig0 = inc0;
     if (uc >= inc0) {
-        float acc = inc0;
+        /* the first active line whose running proposal sum of sigma*v^2
+         * exceeds (uc - inc0) v^2 */
+        const float tc = (uc - inc0) * vv;
+        float acc = 0.0f;
         for (uint32_t j = 0; j < nl; j++) {
             const float *L = Lt + j * PD_LW;
             if (L[PDL_Q0] < twok) {
                 ch = (int)j;
-                acc += L[PDL_SSV2_0] / vv;
-                if (uc < acc) break;
+                acc += L[PDL_SSV2_0];
+                if (tc < acc) break;
             }
         }
         if (ch >= 0) {
-            sig = Lt[ch * PD_LW + PDL_SSV2] / vv;
-            sig0 = Lt[ch * PD_LW + PDL_SSV2_0] / vv;
+            sig = Lt[ch * PD_LW + PDL_SSV2] * rvv;
+            sig0 = Lt[ch * PD_LW + PDL_SSV2_0] * rvv;
         } else {
             sig = inc;
             sig0 = inc0;
@@ -410,15 +417,16 @@ DEV void s_powderdiff(NS &n, const DComp &c, HC &hc)
     }
     if constexpr (HC::adjoint) {
         /* d log w: sig_ch and -mu x depend on the model */
-        float sumq2 = 0.0f;
+        float sq2 = 0.0f;
         int m = 0;
         for (uint32_t j = 0; j < nl; j++) {
             const float *L = Lt + j * PD_LW;
             if (L[PDL_Q0] < twok) {
-                sumq2 += (L[PDL_SSV2] / vv) * L[PDL_Q2];
+                sq2 += L[PDL_SSV2] * L[PDL_Q2];
                 m++;
             }
         }
+        const float sumq2 = sq2 * rvv;
         hc.cs = (ch < 0 && s.p[PD_SINC] > 0.0f ? 1.0f / s.p[PD_SINC] : 0.0f) - 100.0f * x;
         hc.ca = -100.0f * x * (lam / s.p[PD_LREF]);
         hc.cU = (ch >= 0 ? -Lt[ch * PD_LW + PDL_Q2] : 0.0f) + x * sumq2;
