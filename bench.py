@@ -46,8 +46,8 @@ VALU_PEAK_TOPS = 2 * VALU_SINGLE_ISSUE_TOPS
 #           after the plain-copy calibration of the counters: x2.0, x1.0)
 # measured SQ_INSTS_VALU of the timed batch of the 256-setting scan, and per
 # history of the powder forward and adjoint kernels
-TAS_VALU_PER_SCAN = {"valu": 1.43125e10, "commit": "45ade1f"}
-POWDER_VALU_PER_HISTORY = {"forward": 84.50, "adjoint": 94.48, "commit": "45ade1f"}
+TAS_VALU_PER_SCAN = {"valu": 1.38435e10, "commit": "d9e6eea"}
+POWDER_VALU_PER_HISTORY = {"forward": 63.43, "adjoint": 66.93, "commit": "d9e6eea"}
 
 
 # Algorithmic work of the LET-like source segment, per history: the
