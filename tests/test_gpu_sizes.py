@@ -1,4 +1,5 @@
-"""Ragged and tiny history counts on the GPU against the oracle: the source
+"""Ragged and tiny history counts on the GPU against the oracle (minimal,
+LET-like, the guide variants and the triple-axis instrument): the source
 segment's persistent grid shrinks to the fewest waves that keep its number
 of 64-history rounds, so counts below one wave, just above a wave and just
 above a round of the whole grid all run with partly filled waves.  Full
@@ -22,7 +23,7 @@ def _hist_equal(gh, oh):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case", ["minimal", "let_like"])
+@pytest.mark.parametrize("case", ["minimal", "let_like", "guide_variants", "tas"])
 @pytest.mark.parametrize("n", [1, 63, 65, 4097, 100003])
 def test_ragged_counts(case, n):
     path, kw = CASES[case]
