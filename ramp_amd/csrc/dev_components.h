@@ -212,17 +212,16 @@ DEV void src_pol(NS &n, const DPart &s)
 }
 
 /* maxwell_energy_distn (cont_mod.cl:4-9) for a per-T constant set:
- * overKT = d[i], pow(overKT,1.5) = d[3+i]; the double product chain is
- * carried as a double-float pair */
-DEV float maxwell_dev(float E, float overKT, float pf, rdf rsqpi)
+ * overKT = d[i], pow(overKT,1.5) = d[3+i]; the reference's double chain
+ * 2.0f*rootE/sqrt(M_PI)*pow(..)*exp(..) is evaluated in double, the
+ * quotient exactly from the host's sqrt(M_PI) and its reciprocal */
+DEV float maxwell_dev(float E, float overKT, float pf, double sqpi, double rsqpi)
 {
     float rootE = rm_sqrt(E);
-    float a = 2.0f * rootE;
-    rdf b = df_prod(a, rsqpi.hi);
-    b.lo = b.lo + a * rsqpi.lo;
-    rdf c = df_mul_df(b, pf);
-    float ef = rm_exp(-E * overKT);
-    return df_mul_f(c, ef);
+    double r = rd_div_rc((double)(2.0f * rootE), sqpi, rsqpi);
+    r = r * (double)pf;
+    r = r * (double)rm_exp(-E * overKT);
+    return (float)r;
 }
 
 template <bool FULL, uint32_t F>
@@ -242,17 +241,17 @@ DEV void src_generate(NS &n, const DPart &s, int num_sim, int seed, float sb_old
         float E_range = s.p[SP_EMAX] - s.p[SP_EMIN];
         float E_val = s.p[SP_EMIN] + E_range * drw(n);
         float E_J = RC_MEV2J * E_val;
-        rdf rsq; rsq.hi = s.d[11]; rsq.lo = s.d[12];
+        const double sqpi = dp_f64(s.d, 18), rsqpi = dp_f64(s.d, 20);
         float term1;
         if (s.p[SP_TAIL_JOIN] > 0.0f && E_J > s.d[9])
             term1 = s.p[SP_I1] * (s.d[10] * (s.d[9] / E_J)) / s.d[6];
         else
-            term1 = s.p[SP_I1] * maxwell_dev(E_J, s.d[0], s.d[3], rsq) / s.d[6];
+            term1 = s.p[SP_I1] * maxwell_dev(E_J, s.d[0], s.d[3], sqpi, rsqpi) / s.d[6];
         float M = term1;
         /* d[16]/d[17]: the term has I = 0 and a finite Maxwellian value, so
          * adding it leaves M unchanged and it is not evaluated */
-        if (s.d[16] == 0.0f) M += s.p[SP_I2] * maxwell_dev(E_J, s.d[1], s.d[4], rsq) / s.d[7];
-        if (s.d[17] == 0.0f) M += s.p[SP_I3] * maxwell_dev(E_J, s.d[2], s.d[5], rsq) / s.d[8];
+        if (s.d[16] == 0.0f) M += s.p[SP_I2] * maxwell_dev(E_J, s.d[1], s.d[4], sqpi, rsqpi) / s.d[7];
+        if (s.d[17] == 0.0f) M += s.p[SP_I3] * maxwell_dev(E_J, s.d[2], s.d[5], sqpi, rsqpi) / s.d[8];
         n.w = M / (float)num_sim;
         src_posdir(n, s, RC_SE2V * rm_sqrt(E_val));
         src_pol<FULL>(n, s);
@@ -554,14 +553,12 @@ DEV void hist_acc_end(const HistCtx &hc, int n_det, uint32_t bins)
 
 #include "dev_adjoint.h"
 
-/* wavelength variable: 2*M_PI/(V2K*|v|) with a double numerator */
+/* wavelength variable: 2*M_PI/(V2K*|v|), a double quotient in the
+ * reference, evaluated in double (d[0..1]: 2*M_PI as a double) */
 DEV float det_wavelength(const NS &n, const DPart &s)
 {
-    rdf tp; tp.hi = s.d[0]; tp.lo = s.d[1];
     float x = RC_V2K * v3_len(nvel(n));
-    float q = tp.hi / x;
-    float r = rm_fma(-q, x, tp.hi) + tp.lo;
-    return q + r / x;
+    return (float)(dp_f64(s.d, 0) / (double)x);
 }
 
 DEV void s_done(NS &n, uint32_t ci)

@@ -626,7 +626,9 @@ static void derive_source(DPart &s)
         float EjJ = s.p[SP_TAIL_JOIN] * (RC_K_B * T[0]);
         s.d[9] = EjJ;
         s.d[10] = s.p[SP_TAIL_JOIN] > 0.0f ? maxwell_ref_host(EjJ, T[0]) : 0.0f;
-        set_df(s.d, 11, 1.0 / sqrt(RC_M_PI_D));
+        /* sqrt(M_PI) and its reciprocal, doubles (maxwell_dev) */
+        set_d64(s.d, 18, sqrt(RC_M_PI_D));
+        set_d64(s.d, 20, 1.0 / sqrt(RC_M_PI_D));
         /* terms 2 and 3 with I == 0 add exactly +-0 when the Maxwellian is
          * finite, which holds for 1 K <= T <= 1e5 K and E <= 1e4 meV */
         for (int i = 1; i < 3; i++) {
@@ -733,7 +735,7 @@ static void derive_scat(DPart &s)
     }
     case RAMP_S_DET1D:
     case RAMP_S_DET2D:
-        set_df(s.d, 0, 2 * RC_M_PI_D);
+        set_d64(s.d, 0, 2 * RC_M_PI_D);
         break;
     case RAMP_S_RESCAL: {
         /* rescal.cl:36-50, the parameter-only part: cos of the cone
