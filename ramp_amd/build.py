@@ -26,7 +26,7 @@ FLAGS = [
 # system view of the package; they are embedded in the library)
 PLUGIN_HEADERS = ["plugin_kernel.h", "ramp_plugin.h", "dev_queue.h", "dev_components.h", "dev_plan.h",
                   "dev_mesh.h", "bvh_types.h", "dev_adjoint.h", "dev_samples.h", "dev_mcpl.h", "rmath.h", "rvec.h",
-                  "rrng.h", "rdf.h", "rconst.h", "rslots.h", "rdefs.h"]
+                  "rrng.h", "rconst.h", "rslots.h", "rdefs.h"]
 # hipRTC provides the fixed-width integer types in its own namespace and
 # no <stddef.h>/<stdint.h>; these stand in for the two system headers
 RTC_STDDEF = "#pragma once\n"

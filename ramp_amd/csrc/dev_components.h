@@ -7,10 +7,11 @@
  *     re-read from a 100-byte AoS record by five launches per component;
  *   - rotation matrices and parameter-only expressions come precomputed
  *     from the plan (same bits as per-neutron evaluation);
- *   - double-precision expression sites of the reference are evaluated with
- *     the error-free FP32 transformations of rdf.h, except where noted as
- *     "FP32 approximation" (weights and mosaic directions of the
- *     monochromator, detailed balance of the isotropic sample).
+ *   - double-precision expression sites of the reference (double literals
+ *     and M_PI promote float operands) are evaluated in FP64 as the
+ *     reference evaluates them; parameter-only ones once on the host, and
+ *     quotients by a parameter-only divisor from its host-computed
+ *     reciprocal with one correction step (rd_div_rc, exact).
  */
 #ifndef RAMP_DEV_COMPONENTS_H
 #define RAMP_DEV_COMPONENTS_H
@@ -18,7 +19,6 @@
 #include "rmath.h"
 #include "rvec.h"
 #include "rrng.h"
-#include "rdf.h"
 #include "rconst.h"
 #include "rslots.h"
 #include "dev_plan.h"
@@ -202,8 +202,7 @@ DEV void src_pol(NS &n, const DPart &s)
     float u1 = drw(n);
     float u2 = drw(n);
     if (FULL) {
-        rdf twopi; twopi.hi = s.d[30]; twopi.lo = s.d[31];
-        float theta = df_mulc(u1, twopi);
+        float theta = (float)(2 * RC_M_PI_D * (double)u1);
         float phi = rm_acos(1 - 2 * u2);
         n.px = rm_sin(phi) * rm_cos(theta);
         n.py = rm_sin(phi) * rm_sin(theta);

@@ -37,9 +37,10 @@ DEV void iso_sample(NS &n, const float *base, uint32_t nq, uint32_t nw, float &o
     }
 }
 
-/* scat/isotropic.cl:75-323.  d[0]/d[1]: 2*pi split; d[2]: kB as float
- * (detailed balance is an FP32 approximation of the reference's double
- * exp; exact for elastic channels, omega = 0) */
+/* scat/isotropic.cl:75-323.  d[0..1]: 2*pi as a double.  The reference's
+ * double expressions (the cross-sections over 2*ki*ki and the velocity, the
+ * path length, the channel thresholds, 2*M_PI*rand() and the detailed
+ * balance) are evaluated in double as it evaluates them. */
 template <bool FULL>
 DEV void s_isotropic(NS &n, const DComp &c)
 {
@@ -49,10 +50,10 @@ DEV void s_isotropic(NS &n, const DComp &c)
     float ki = RC_V2K * vlen;
     rf3 normvel = v3_norm(nvel(n));
     float cs = s.p[IS_SSCAT(0)], is_ = s.p[IS_SSCAT(1)], ms = s.p[IS_SSCAT(2)];
-    rdf twok2 = df_prod(2.0f * ki, ki);
-    float sigma_s = df_rdiv((cs + is_) + ms, twok2);
-    rdf sa = df_prod((s.p[IS_SABS(0)] + s.p[IS_SABS(1)]) + s.p[IS_SABS(2)], 2200.0f);
-    float sigma_a = df_div(sa, vlen);
+    const double twok2 = 2. * (double)ki * (double)ki;
+    float sigma_s = (float)((double)((cs + is_) + ms) / twok2);
+    float sigma_a = (float)((double)((s.p[IS_SABS(0)] + s.p[IS_SABS(1)]) + s.p[IS_SABS(2)]) * 2200. /
+                            (double)vlen);
     float sigma_tot = sigma_a + sigma_s;
     float mu = (((s.p[IS_RHO(0)] + s.p[IS_RHO(1)]) + s.p[IS_RHO(2)]) * sigma_tot) * 100.0f;
     float plen = v3_len(path);
@@ -70,22 +71,14 @@ DEV void s_isotropic(NS &n, const DComp &c)
     n.w *= (1 - p_trans) * sigma_s / sigma_tot;
     float eta = drw(n);
     float lg = rm_log(1 - eta * (1 - rm_exp(-mu * plen)));
-    /* (-1./mu)*lg in double */
-    float inv = 1.0f / mu;
-    rdf im; im.hi = inv; im.lo = rm_fma(-inv, mu, 1.0f) / mu;
-    float path_length = -df_mul_f(im, lg);
+    float path_length = (float)((-1. / (double)mu) * (double)lg);
     float u = drw(n);
     u *= sigma_s;
     uint32_t flag = 0;
-    /* thresholds cs/(2ki^2) and (cs+is)/(2ki^2) in double */
-    float q1h = cs / twok2.hi;
-    rdf th1; th1.hi = q1h; th1.lo = (rm_fma(-q1h, twok2.hi, cs) - q1h * twok2.lo) / twok2.hi;
-    float csis = cs + is_;
-    float q2h = csis / twok2.hi;
-    rdf th2; th2.hi = q2h; th2.lo = (rm_fma(-q2h, twok2.hi, csis) - q2h * twok2.lo) / twok2.hi;
-    if (df_lt(u, th1)) flag = 1;
-    else if (df_gt(u, th1) && df_lt(u, th2)) flag = 2;
-    else if (df_gt(u, th2)) flag = 3;
+    const double th1 = (double)cs / twok2, th2 = (double)(cs + is_) / twok2;
+    if ((double)u < th1) flag = 1;
+    else if ((double)u > th1 && (double)u < th2) flag = 2;
+    else if ((double)u > th2) flag = 3;
     float omega = 0.0f, Q = 0.0f;
     if (flag >= 1) {
         int ch = (int)flag - 1;
@@ -119,8 +112,7 @@ DEV void s_isotropic(NS &n, const DComp &c)
     rf3 oldvel = nvel(n);
     float z = -(n.vx + n.vy) / (n.vz);
     rf3 perp = v3_norm(rf3_make(1.0f, 1.0f, z));
-    rdf tp; tp.hi = s.d[0]; tp.lo = s.d[1];
-    float alpha = df_mulc(drw(n), tp);
+    float alpha = (float)(dp_f64(s.d, 0) * (double)drw(n));
     perp = rot_about_axis(alpha, normvel, perp);
     normvel = rot_about_axis(theta, perp, normvel);
     float TOF = path_length / vlen;
@@ -151,21 +143,17 @@ DEV void s_isotropic(NS &n, const DComp &c)
     isect_reset(n);
 }
 
-/* azimuth of the Debye-Scherrer cone (powder1.cl:68-79, powderN.cl:117-130):
- * d[0]/d[1] span = d_phi*M_PI/180.0f, d[2]/d[3] M_PI, d[4]/d[5] M_PI/2.0f,
- * d[6]/d[7] 2.0f*M_PI (double constants, split) */
+/* azimuth of the Debye-Scherrer cone (powder1.cl:68-79, powderN.cl:117-130),
+ * its double expressions in double; d[0..1] span = d_phi*M_PI/180.0f as a
+ * double */
 DEV float powder_phi(NS &n, const DPart &s, float d_phi)
 {
     if (d_phi != 0.0f) {
-        rdf span; span.hi = s.d[0]; span.lo = s.d[1];
-        rdf pi; pi.hi = s.d[2]; pi.lo = s.d[3];
-        rdf pio2; pio2.hi = s.d[4]; pio2.lo = s.d[5];
-        float ph = df_mulc(2.0f * drw(n) - 1.0f, span);
-        if (drw(n) > 0.5f) ph = df_addc(ph, pi);
-        return df_addc(ph, pio2);
+        float ph = (float)((double)(2.0f * drw(n) - 1.0f) * dp_f64(s.d, 0));
+        if (drw(n) > 0.5f) ph = (float)((double)ph + RC_M_PI_D);
+        return (float)((double)ph + RC_M_PI_D / 2.0f);
     }
-    rdf tp; tp.hi = s.d[6]; tp.lo = s.d[7];
-    return df_mulc(drw(n), tp);
+    return (float)(2.0f * RC_M_PI_D * (double)drw(n));
 }
 
 DEV void powder_dirs(rf3 vel, rf3 &para, rf3 &perp)
@@ -176,8 +164,8 @@ DEV void powder_dirs(rf3 vel, rf3 &para, rf3 &perp)
     else perp = v3_norm(perp);
 }
 
-/* sigma_abs_v * 2200.0 / vel with a double constant (powder1.cl:37) */
-DEV float powder_sabs(float sav, float vel) { return df_div(df_prod(sav, 2200.0f), vel); }
+/* sigma_abs_v * 2200.0 / vel, a double expression (powder1.cl:37) */
+DEV float powder_sabs(float sav, float vel) { return (float)((double)sav * 2200.0 / (double)vel); }
 
 /* scat/powder1.cl:5-101 */
 DEV void s_powder1(NS &n, const DComp &c)

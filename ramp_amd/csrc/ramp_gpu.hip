@@ -590,13 +590,6 @@ static float maxwell_ref_host(float E, float T)
     return (float)r;
 }
 
-static void set_df(float *d, int i, double v)
-{
-    rdf s = df_split(v);
-    d[i] = s.hi;
-    d[i + 1] = s.lo;
-}
-
 /* a double in slots i, i+1 (read by dp_f64) */
 static void set_d64(float *d, int i, double v)
 {
@@ -614,7 +607,6 @@ static float thresh_lt_double(double c)
 static void derive_source(DPart &s)
 {
     std::memset(s.d, 0, sizeof s.d);
-    set_df(s.d, 30, 2 * RC_M_PI_D);
     if (s.type == RAMP_SRC_CONT) {
         float T[3] = {s.p[SP_T1], s.p[SP_T2], s.p[SP_T3]};
         for (int i = 0; i < 3; i++) {
@@ -701,15 +693,12 @@ static void derive_scat(DPart &s)
         break;
     }
     case RAMP_S_ISOTROPIC:
-        set_df(s.d, 0, 2 * RC_M_PI_D);
+        set_d64(s.d, 0, 2 * RC_M_PI_D);
         break;
     case RAMP_S_POWDER1:
     case RAMP_S_POWDERN: {
         float d_phi = s.type == RAMP_S_POWDER1 ? s.p[P1_DPHI] : s.p[PN_DPHI];
-        set_df(s.d, 0, (double)d_phi * RC_M_PI_D / 180.0f);
-        set_df(s.d, 2, RC_M_PI_D);
-        set_df(s.d, 4, RC_M_PI_D / 2.0f);
-        set_df(s.d, 6, 2.0f * RC_M_PI_D);
+        set_d64(s.d, 0, (double)d_phi * RC_M_PI_D / 180.0f);
         break;
     }
     case RAMP_S_MONO: {
