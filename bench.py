@@ -86,8 +86,8 @@ def valu_roofline(kernel, lane_ops, launch_ms, basis, extra=None):
 
 
 PMC_PER_HISTORY = {
-    "let_like": {"kernel": "k_segment<false,true>", "valu": 75.47, "lanes": 0.666, "fetch": 7.676,
-                 "write": 19.495, "commit": "00ae820"},
+    "let_like": {"kernel": "k_segment<false,true>", "valu": 75.41, "lanes": 0.666, "fetch": 7.585,
+                 "write": 19.444, "commit": "0fe9900"},
     "imaging": {"kernel": "k_mesh_walk", "valu": 110.90, "lanes": 0.568, "fetch": 32.24, "write": 25.50,
                 "commit": "45ade1f"},
     "minimal": None,
