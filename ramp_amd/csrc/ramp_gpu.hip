@@ -477,6 +477,7 @@ struct ramp_plan {
     uint64_t trace_n = 0;     /* histories of the last traced run (0: the last run was not traced) */
     void *pin = nullptr;      /* pinned host staging for histogram reads (hipHostMalloc) */
     size_t pin_bytes = 0;
+    unsigned long long *cnt_host = nullptr;  /* pinned copy of the run counters (hipHostMalloc) */
     std::vector<int> plugin_step;           /* per step: the component number of a plugin step, else -1 */
     std::vector<ramp_plugin *> plugin_of;   /* per component: the bound plugin */
     std::vector<uint32_t> rescal_comp; /* their component numbers */
@@ -1121,6 +1122,8 @@ extern "C" int ramp_plan_create(ramp_ctx *ctx, const ramp_plan_desc *d, ramp_pla
     PCHK(dev_alloc(p->src, sizeof(DPart)));
     PCHK(dev_alloc(p->dets, std::max<size_t>(1, p->hdets.size()) * sizeof(DDet)));
     PCHK(dev_alloc(p->cnt, CNT_N * sizeof(unsigned long long)));
+    if (hipHostMalloc((void **)&p->cnt_host, CNT_N * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
+        PCHK(fail(RAMP_E_DEVICE, "hipHostMalloc of the pinned counter buffer failed"));
     PCHK(dev_alloc(p->count, 64 * sizeof(uint32_t)));
     if (!p->hcomps.empty())
         HIPCHK(hipMemcpy(p->comps.p, p->hcomps.data(), p->hcomps.size() * sizeof(DComp), hipMemcpyHostToDevice));
@@ -1148,6 +1151,7 @@ extern "C" int ramp_plan_destroy(ramp_plan *p)
                         &p->ev_list, &p->ev_list_n, &p->trace};
     for (DevBuf *b : single) dev_free(*b);
     if (p->pin) (void)hipHostFree(p->pin);
+    if (p->cnt_host) (void)hipHostFree(p->cnt_host);
     for (auto *v : {&p->hw, &p->hw2, &p->hq, &p->hq2, &p->mesh_nodes})
         for (auto &b : *v) dev_free(b);
     for (int i = 0; i < 2; i++) {
@@ -1925,9 +1929,10 @@ static int run_plan(ramp_plan *p, uint64_t n_hist, uint64_t chunk, const uint32_
         }
     }
     HIPCHK(hipEventRecord(ctx->ev[1], s));
-    HIPCHK(hipEventSynchronize(ctx->ev[1]));
-    std::vector<unsigned long long> cnt(CNT_N);
-    HIPCHK(hipMemcpy(cnt.data(), p->cnt.p, CNT_N * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    /* the counters come back on the stream into pinned memory: one wait */
+    HIPCHK(hipMemcpyAsync(p->cnt_host, p->cnt.p, CNT_N * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    const unsigned long long *cnt = p->cnt_host;
     float ms = 0, ms0 = 0;
     HIPCHK(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
     HIPCHK(hipEventElapsedTime(&ms0, ctx->ev[2], ctx->ev[3]));
